@@ -1,0 +1,740 @@
+// oracle.cpp -- CPU restatement of the Bonsai2 gravity step, used ONLY as the
+// checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+// Nothing in the product (bonsai2_amd/, include/) links, loads or calls it.
+//
+// Each function cites the reference file:line whose behaviour it restates.
+// Paths are relative to the reference repository root.
+//
+// Parity pins (what checks each function against something independent):
+//   or_bounding_box      brute-force min/max + closed-form box rules (tests)
+//   or_build_tree        treebuild/readme:35-39 known-answer counts on the
+//                        reference Plummer sampler; invariants of
+//                        SWtest/treebuild.cu:1477; order == stable sort of
+//                        or_geo_keys (independent route)
+//   or_multipoles        FP64 brute force + parallel-axis theorem (tests)
+//   or_ph_keys           exhaustive small-NBITS Hilbert bijection/adjacency,
+//                        hand-derived golden keys
+//   or_make_groups       closed-form chopping of sorted keys (tests)
+//   or_walk              theta->0 equals direct sum; two-body and quadrupole
+//                        closed forms; FP64 direct sum error bound
+//   or_direct_fp64       long-double brute force on tiny inputs
+//   or_plummer_ref       bit-exact against the reference's own plummer.h
+//                        compiled into oracle/_ref (when that builds)
+//   or_energies          closed forms (two-body)
+// Every function has at least one such check; none is "parity unpinned".
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+struct F4 { float x, y, z, w; };
+struct F2 { float x, y; };
+struct U4 { uint32_t x, y, z, w; };
+struct I2 { int32_t x, y; };
+struct D4 { double x, y, z, w; };
+
+constexpr int kMaxLevel = 21;   // depth of the geometric octant key (63 bits)
+
+// ---------------------------------------------------------------------------
+// Octant / ChildBox: code/Particle4.h:129-148.  Bit b of the octant is set
+// iff centre <= pos on that axis (x -> 1, y -> 2, z -> 4).  The child centre
+// moves by +-hsize/2 per axis; all values are exact dyadic floats.
+// ---------------------------------------------------------------------------
+inline int octant_of(float cx, float cy, float cz, float px, float py, float pz) {
+  return (int)(cx <= px) | ((int)(cy <= py) << 1) | ((int)(cz <= pz) << 2);
+}
+
+struct BoxF { float cx, cy, cz, h; };
+
+inline BoxF child_box(const BoxF &b, int oct) {
+  const float s = 0.5f * b.h;
+  BoxF c;
+  c.cx = b.cx + s * ((oct & 1) ? 1.0f : -1.0f);
+  c.cy = b.cy + s * ((oct & 2) ? 1.0f : -1.0f);
+  c.cz = b.cz + s * ((oct & 4) ? 1.0f : -1.0f);
+  c.h = s;
+  return c;
+}
+
+// CellData packing: code/Treecode.h:45-89.
+inline U4 pack_cell(int level, uint32_t parent, uint32_t pbeg, uint32_t pend,
+                    bool leaf, uint32_t first, uint32_t nchild) {
+  U4 c;
+  c.x = parent | ((uint32_t)level << 27);
+  c.y = leaf ? 0xFFFFFFFFu : (first | ((nchild - 1u) << 29));
+  c.z = pbeg;
+  c.w = pend;
+  return c;
+}
+inline bool cell_is_leaf(const U4 &c) { return c.y == 0xFFFFFFFFu; }
+inline int cell_first(const U4 &c) { return (int)(c.y & 0x1FFFFFFFu); }
+inline int cell_n(const U4 &c) { return (int)(c.y >> 29) + 1; }
+
+}  // namespace
+
+extern "C" {
+
+int or_version() { return 3; }
+
+// ---------------------------------------------------------------------------
+// Root box: code/buildTree.cu:80-166 (computeBoundingBox).  Min/max of the
+// particle coordinates (initial +-1e10), centre and half extent in float,
+// hsize = smallest power of two >= max half extent (loop from 1.0,
+// :147-150), centre quantised toward zero to hsize/2^20 (:152-159).
+// A zero extent (the reference's loop would underflow to 0 and divide by
+// it) is given hsize = 1.
+// out box = {cx, cy, cz, hsize}; bmin/bmax = raw extremes.
+// ---------------------------------------------------------------------------
+void or_bounding_box(int n, const F4 *pos, float *box, float *bmin, float *bmax) {
+  float mn[3] = {1e10f, 1e10f, 1e10f}, mx[3] = {-1e10f, -1e10f, -1e10f};
+  for (int i = 0; i < n; i++) {
+    mn[0] = std::fmin(mn[0], pos[i].x); mx[0] = std::fmax(mx[0], pos[i].x);
+    mn[1] = std::fmin(mn[1], pos[i].y); mx[1] = std::fmax(mx[1], pos[i].y);
+    mn[2] = std::fmin(mn[2], pos[i].z); mx[2] = std::fmax(mx[2], pos[i].z);
+  }
+  float c[3], hv[3];
+  for (int k = 0; k < 3; k++) {
+    c[k] = (mx[k] + mn[k]) * 0.5f;
+    hv[k] = (mx[k] - mn[k]) * 0.5f;
+  }
+  const float h = std::fmax(hv[2], std::fmax(hv[1], hv[0]));
+  float hsize = 1.0f;
+  if (h > 0.0f) {
+    while (hsize > h) hsize *= 0.5f;
+    while (hsize < h) hsize *= 2.0f;
+  }
+  const float hquant = hsize / (float)(1 << 20);
+  for (int k = 0; k < 3; k++) {
+    const long long q = (long long)(c[k] / hquant);
+    box[k] = hquant * (float)q;
+  }
+  box[3] = hsize;
+  for (int k = 0; k < 3; k++) { bmin[k] = mn[k]; bmax[k] = mx[k]; }
+}
+
+// ---------------------------------------------------------------------------
+// Geometric octant key: the sequence of Octant() digits a particle takes on
+// the way down the ChildBox recursion of code/buildTree.cu:233-247, 21
+// levels, most significant digit first (bits 62..60 = level 1).
+// ---------------------------------------------------------------------------
+void or_geo_keys(int n, const F4 *pos, const float *box, uint64_t *keys) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    BoxF b{box[0], box[1], box[2], box[3]};
+    uint64_t k = 0;
+    for (int l = 0; l < kMaxLevel; l++) {
+      const int o = octant_of(b.cx, b.cy, b.cz, pos[i].x, pos[i].y, pos[i].z);
+      k = (k << 3) | (uint64_t)o;
+      b = child_box(b, o);
+    }
+    keys[i] = k;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Octree build: code/buildTree.cu:170-584 (buildOctant), :624-722
+// (buildOctree), level sort and level table :725-804, :932-951.
+//
+// Restated as a recursive STABLE partition of particle indices by Octant()
+// against each cell's ChildBox centre, in octant order 0..7 (:444-471).
+// The reference's within-leaf order is nondeterministic (block races on
+// atomicAdd, :262); the canonical order used here keeps partitioning below
+// the leaves down to level 21, i.e. leaf particles are ordered by their
+// deeper octant digits and then by input index.
+//
+// Cells: a child with count > nleaf is a node, 0 < count <= nleaf a leaf,
+// empty children make no cell (:394-395); the root (level 0) always splits
+// and is not stored, so level-1 cells start at index 0 (:424).  Cells are
+// numbered level by level (stable sort by level, :940-944); within a level,
+// children of earlier parents come first and each parent's children are
+// [nodes in octant order][leaves in octant order] (:393-400, :446, :549).
+// A cell at level 21 that still holds > nleaf particles becomes an
+// (oversized) leaf and the depth-cap flag is raised; the reference has no
+// depth guard at all (5-bit level field, code/Treecode.h:50-51).
+//
+// Forest: nroot roots given by offsets[0..nroot] over the input, each with
+// its own box (4 floats).  Root r's particles stay in [off[r], off[r+1]).
+//
+// Outputs: order[n] (tree position -> input index), cells[ncell_max] U4,
+// level_range[2*32] = (first, last inclusive) per level (code/buildTree.cu
+// :748-780), root_cells[2*nroot] = (first level-1 cell, count),
+// stats[6] = ncells, nnodes, nleaves, nlevels, depth_cap_hit, nbodies_leaf.
+// Returns 0, or -1 if ncell_max is too small.
+// ---------------------------------------------------------------------------
+static void stable_partition_rec(std::vector<int> &idx, std::vector<int> &tmp,
+                                 const F4 *pos, int b, int e, BoxF box, int level) {
+  if (e - b <= 1 || level >= kMaxLevel) return;
+  int cnt[8] = {0};
+  std::vector<unsigned char> oc(e - b);
+  for (int i = b; i < e; i++) {
+    const F4 &p = pos[idx[i]];
+    const int o = octant_of(box.cx, box.cy, box.cz, p.x, p.y, p.z);
+    oc[i - b] = (unsigned char)o;
+    cnt[o]++;
+  }
+  int off[9];
+  off[0] = b;
+  for (int o = 0; o < 8; o++) off[o + 1] = off[o] + cnt[o];
+  int cur[8];
+  for (int o = 0; o < 8; o++) cur[o] = off[o];
+  for (int i = b; i < e; i++) tmp[cur[oc[i - b]]++] = idx[i];
+  for (int i = b; i < e; i++) idx[i] = tmp[i];
+  const bool par = (e - b) > 65536 && level < 3;
+  for (int o = 0; o < 8; o++) {
+    if (cnt[o] == 0) continue;
+    const BoxF cb = child_box(box, o);
+    const int cb0 = off[o], ce0 = off[o + 1];
+    if (par) {
+#pragma omp task shared(idx, tmp)
+      stable_partition_rec(idx, tmp, pos, cb0, ce0, cb, level + 1);
+    } else {
+      stable_partition_rec(idx, tmp, pos, cb0, ce0, cb, level + 1);
+    }
+  }
+  if (par) {
+#pragma omp taskwait
+  }
+}
+
+int or_build_tree(int n, const F4 *pos, int nroot, const int *offsets,
+                  const float *boxes, int nleaf, int ncell_max,
+                  int *order, U4 *cells, int *level_range, int *root_cells,
+                  int *stats) {
+  std::vector<int> idx(n), tmp(n);
+  for (int i = 0; i < n; i++) idx[i] = i;
+#pragma omp parallel
+#pragma omp single
+  for (int r = 0; r < nroot; r++) {
+    const BoxF b{boxes[4 * r], boxes[4 * r + 1], boxes[4 * r + 2], boxes[4 * r + 3]};
+    stable_partition_rec(idx, tmp, pos, offsets[r], offsets[r + 1], b, 0);
+  }
+  for (int i = 0; i < n; i++) order[i] = idx[i];
+
+  struct Par { int b, e; BoxF box; int cell; int level; };
+  std::vector<Par> parents, next;
+  for (int r = 0; r < nroot; r++) {
+    const BoxF b{boxes[4 * r], boxes[4 * r + 1], boxes[4 * r + 2], boxes[4 * r + 3]};
+    parents.push_back({offsets[r], offsets[r + 1], b, -1 - r, 0});
+  }
+  for (int l = 0; l < 64; l++) { level_range[l] = 0; }
+  int ncells = 0, nnodes = 0, nleaves = 0, nlevels = 0, capped = 0;
+  long long nb_leaf = 0;
+  for (int level = 1; !parents.empty(); level++) {
+    next.clear();
+    const int lvl_first = ncells;
+    for (const Par &p : parents) {
+      int cnt[8] = {0};
+      int prev = -1;
+      for (int i = p.b; i < p.e; i++) {
+        const F4 &q = pos[order[i]];
+        const int o = octant_of(p.box.cx, p.box.cy, p.box.cz, q.x, q.y, q.z);
+        if (o < prev) return -2;  // partition order broken (cannot happen)
+        prev = o;
+        cnt[o]++;
+      }
+      int off[9];
+      off[0] = p.b;
+      for (int o = 0; o < 8; o++) off[o + 1] = off[o] + cnt[o];
+      int nchild = 0;
+      for (int o = 0; o < 8; o++) nchild += cnt[o] > 0;
+      const int first = ncells;
+      if (ncells + nchild > ncell_max) return -1;
+      if (p.cell >= 0) {
+        const U4 pc = cells[p.cell];
+        cells[p.cell] = pack_cell(level - 1, pc.x & 0x07FFFFFFu, pc.z, pc.w, false,
+                                  (uint32_t)first, (uint32_t)nchild);
+      } else {
+        root_cells[2 * (-1 - p.cell)] = first;
+        root_cells[2 * (-1 - p.cell) + 1] = nchild;
+      }
+      const uint32_t parent_idx = p.cell >= 0 ? (uint32_t)p.cell : 0u;
+      for (int pass = 0; pass < 2; pass++) {
+        for (int o = 0; o < 8; o++) {
+          const int c = cnt[o];
+          if (c == 0) continue;
+          const bool split = c > nleaf && level < kMaxLevel;
+          if ((pass == 0) != split) continue;
+          if (c > nleaf && level >= kMaxLevel) capped = 1;
+          const int ci = ncells++;
+          // node records get first/nchild when their own children are made
+          cells[ci] = pack_cell(level, parent_idx, (uint32_t)off[o], (uint32_t)off[o + 1],
+                                !split, 0u, 1u);
+          if (split) {
+            nnodes++;
+            next.push_back({off[o], off[o + 1], child_box(p.box, o), ci, level});
+          } else {
+            nleaves++;
+            nb_leaf += c;
+          }
+        }
+      }
+    }
+    if (ncells > lvl_first) {
+      level_range[2 * level] = lvl_first;
+      level_range[2 * level + 1] = ncells - 1;
+      nlevels = level;
+    }
+    parents.swap(next);
+  }
+  stats[0] = ncells; stats[1] = nnodes; stats[2] = nleaves; stats[3] = nlevels;
+  stats[4] = capped; stats[5] = (int)nb_leaf;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Cell multipoles: code/computeMultipoles.cu:81-190 (computeCellMultipoles),
+// addMonopole/addQuadrupole :12-68, addBoxSize code/cuda_primitives.h:56-83.
+// Per cell over its particles [pbeg, pend) in tree order: particle bbox,
+// mass M, centre of mass, second moments about the centre of mass divided
+// by M (Q_ab = sum m x_a x_b / M - com_a com_b), l = max(2*max half
+// extent, 1e-6), s = |bbox centre - com|, cellOp = l*inv_theta + s,
+// outputs (:180-188): size = (com, cellOp^2), monopole = (com, M),
+// quad0 = (xx, yy, zz, xy), quad1 = (xz, yz).
+//
+// mode 0: the moments are summed in FP64 with the two-pass form
+//         sum m (x-com)(x-com)^T / M (the same quantity, well conditioned).
+// mode 1: the reference's own arithmetic: FP32 products m*x, m*x*x, a
+//         32-lane xor-butterfly FP32 sum per chunk of 32 particles, chunk
+//         sums accumulated in FP64, then Q = S2/M - com*com in FP64
+//         (:113-114, :127-158).  Kept to document that this form loses
+//         most digits of Q for small cells far from the origin.
+// The bbox/size part is FP32 and identical in both modes.
+// inv_theta is passed as the reference passes it: float(1.0/double(theta)).
+// ls_out (may be null) receives (l, s) per cell.
+// ---------------------------------------------------------------------------
+static float butterfly32(float v[32]) {
+  float a[32];
+  for (int l = 0; l < 32; l++) a[l] = v[l];
+  for (int i = 4; i >= 0; i--) {
+    float b[32];
+    for (int l = 0; l < 32; l++) b[l] = a[l] + a[l ^ (1 << i)];
+    for (int l = 0; l < 32; l++) a[l] = b[l];
+  }
+  return a[0];
+}
+
+void or_multipoles(int ncells, const U4 *cells, const F4 *pos, float inv_theta, int mode,
+                   F4 *size, F4 *mono, F4 *quad0, F2 *quad1, F2 *ls_out) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int c = 0; c < ncells; c++) {
+    const int b = (int)cells[c].z, e = (int)cells[c].w;
+    float mn[3] = {1e10f, 1e10f, 1e10f}, mx[3] = {-1e10f, -1e10f, -1e10f};
+    for (int i = b; i < e; i++) {
+      mn[0] = std::fmin(mn[0], pos[i].x); mx[0] = std::fmax(mx[0], pos[i].x);
+      mn[1] = std::fmin(mn[1], pos[i].y); mx[1] = std::fmax(mx[1], pos[i].y);
+      mn[2] = std::fmin(mn[2], pos[i].z); mx[2] = std::fmax(mx[2], pos[i].z);
+    }
+    double M = 0, cx = 0, cy = 0, cz = 0;
+    double qxx = 0, qyy = 0, qzz = 0, qxy = 0, qxz = 0, qyz = 0;
+    if (mode == 0) {
+      for (int i = b; i < e; i++) {
+        const double m = pos[i].w;
+        M += m; cx += m * pos[i].x; cy += m * pos[i].y; cz += m * pos[i].z;
+      }
+      const double im = 1.0 / M;
+      cx *= im; cy *= im; cz *= im;
+      for (int i = b; i < e; i++) {
+        const double m = pos[i].w;
+        const double dx = pos[i].x - cx, dy = pos[i].y - cy, dz = pos[i].z - cz;
+        qxx += m * dx * dx; qyy += m * dy * dy; qzz += m * dz * dz;
+        qxy += m * dx * dy; qxz += m * dx * dz; qyz += m * dy * dz;
+      }
+      qxx *= im; qyy *= im; qzz *= im; qxy *= im; qxz *= im; qyz *= im;
+    } else {
+      for (int i0 = b; i0 < e; i0 += 32) {
+        float v[10][32];
+        for (int l = 0; l < 32; l++) {
+          const F4 p = pos[std::min(i0 + l, e - 1)];
+          const float m = (i0 + l < e) ? p.w : 0.0f;
+          v[0][l] = m * p.x; v[1][l] = m * p.y; v[2][l] = m * p.z; v[3][l] = m;
+          v[4][l] = m * p.x * p.x; v[5][l] = m * p.y * p.y; v[6][l] = m * p.z * p.z;
+          v[7][l] = m * p.x * p.y; v[8][l] = m * p.x * p.z; v[9][l] = m * p.y * p.z;
+        }
+        cx += butterfly32(v[0]); cy += butterfly32(v[1]); cz += butterfly32(v[2]);
+        M += butterfly32(v[3]);
+        qxx += butterfly32(v[4]); qyy += butterfly32(v[5]); qzz += butterfly32(v[6]);
+        qxy += butterfly32(v[7]); qxz += butterfly32(v[8]); qyz += butterfly32(v[9]);
+      }
+      const double im = 1.0 / M;
+      cx *= im; cy *= im; cz *= im;
+      qxx = qxx * im - cx * cx; qyy = qyy * im - cy * cy; qzz = qzz * im - cz * cz;
+      qxy = qxy * im - cx * cy; qxz = qxz * im - cx * cz; qyz = qyz * im - cy * cz;
+    }
+    const float comx = (float)cx, comy = (float)cy, comz = (float)cz;
+    const float bcx = (mx[0] + mn[0]) * 0.5f, bcy = (mx[1] + mn[1]) * 0.5f,
+                bcz = (mx[2] + mn[2]) * 0.5f;
+    const float hx = (mx[0] - mn[0]) * 0.5f, hy = (mx[1] - mn[1]) * 0.5f,
+                hz = (mx[2] - mn[2]) * 0.5f;
+    const float dx = bcx - comx, dy = bcy - comy, dz = bcz - comz;
+    const float s = std::sqrt(dx * dx + dy * dy + dz * dz);
+    const float l = std::max(2.0f * std::max(hx, std::max(hy, hz)), 1.0e-6f);
+    const float op = l * inv_theta + s;
+    size[c] = F4{comx, comy, comz, op * op};
+    mono[c] = F4{comx, comy, comz, (float)M};
+    quad0[c] = F4{(float)qxx, (float)qyy, (float)qzz, (float)qxy};
+    quad1[c] = F2{(float)qxz, (float)qyz};
+    if (ls_out) ls_out[c] = F2{l, s};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Peano-Hilbert key: code/makeGroups.cu:19-106 (get_key<21>) and the
+// quantisation of computeKeys :108-135.  Coordinates are quantised in
+// float to 21-bit integers in the root box; then, MSB first, each 3-bit
+// digit (x<<2 | y<<1 | z) selects the Hilbert state change (coordinate
+// swaps and bitwise complements) and the output digit from the table
+// {0,1,7,6,3,2,4,5}.  box = {cx, cy, cz, hsize}.
+// ---------------------------------------------------------------------------
+static uint64_t hilbert_key(int x, int y, int z, int nbits) {
+  static const int kDigit[8] = {0, 1, 7, 6, 3, 2, 4, 5};
+  uint64_t key = 0;
+  for (int lev = nbits - 1; lev >= 0; lev--) {
+    const int bit = 1 << lev;
+    const int bx = (x & bit) ? 1 : 0, by = (y & bit) ? 1 : 0, bz = (z & bit) ? 1 : 0;
+    const int idx = (bx << 2) | (by << 1) | bz;
+    int t;
+    switch (idx) {
+      case 0: t = z; z = y; y = t; break;                       // swap y,z
+      case 1: case 5: t = x; x = y; y = t; break;               // swap x,y
+      case 4: case 6: x = ~x; z = ~z; break;                    // flip x,z
+      case 3: case 7: t = ~x; x = ~y; y = t; break;             // x<-~y, y<-~x
+      default: t = ~z; z = ~y; y = t; break;                    // idx 2: z<-~y, y<-~z
+    }
+    key = (key << 3) + (uint64_t)kDigit[idx];
+  }
+  return key;
+}
+
+uint64_t or_hilbert_key(int x, int y, int z, int nbits) { return hilbert_key(x, y, z, nbits); }
+
+void or_ph_keys(int n, const F4 *pos, const float *box, uint64_t *keys) {
+  const float inv = 0.5f / box[3];
+  const float bx = box[0] - box[3], by = box[1] - box[3], bz = box[2] - box[3];
+  const float scale = (float)(1 << 21);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    const int xc = (int)((pos[i].x - bx) * inv * scale);
+    const int yc = (int)((pos[i].y - by) * inv * scale);
+    const int zc = (int)((pos[i].z - bz) * inv * scale);
+    keys[i] = hilbert_key(xc, yc, zc, 21);
+  }
+}
+
+// Stable sort of (key, index): the contract of thrust::sort_by_key at
+// code/makeGroups.cu:270-277 (nondecreasing keys, ties keep input order).
+void or_sort_keys(int n, const uint64_t *keys, int *perm) {
+  for (int i = 0; i < n; i++) perm[i] = i;
+  std::stable_sort(perm, perm + n, [&](int a, int b) { return keys[a] < keys[b]; });
+}
+
+// ---------------------------------------------------------------------------
+// Groups: code/makeGroups.cu:137-223 (mask_keys, make_groups) and
+// :259-309.  The sorted keys are masked to their top level_split octal
+// digits (mask built at :259-266); each run [s,e) of equal masked keys is
+// chopped into groups [s + k*ncrit, min(s + (k+1)*ncrit, e)).  The
+// reference's group slots come from atomicAdd (:219); here groups are
+// listed in order of pbeg.  seg_id (may be null) splits runs further
+// (different systems of a forest never share a group).
+// Returns the group count; groups[] = (pbeg, np).
+// ---------------------------------------------------------------------------
+int or_make_groups(int n, const uint64_t *sorted_keys, const int *seg_id,
+                   int level_split, int ncrit, I2 *groups) {
+  level_split = std::max(1, level_split);
+  uint64_t mask = 0;
+  for (int i = 0; i < 21; i++) {
+    mask <<= 3;
+    if (i < level_split) mask |= 7;
+  }
+  int ng = 0, s = 0;
+  while (s < n) {
+    int e = s + 1;
+    while (e < n && (sorted_keys[e] & mask) == (sorted_keys[s] & mask) &&
+           (!seg_id || seg_id[e] == seg_id[s]))
+      e++;
+    for (int b = s; b < e; b += ncrit) groups[ng++] = I2{b, std::min(ncrit, e - b)};
+    s = e;
+  }
+  return ng;
+}
+
+// ---------------------------------------------------------------------------
+// Tree walk: code/computeForces.cu:30-54 (opening test), :58-78 (P2P),
+// :83-128 (P2C, monopole + quadrupole), :229-463 (treewalk_warp),
+// :475-619 (treewalk), :622-727 (host).
+//
+// For each group: the i-particles are pos_ph[pbeg, pbeg+np) (PH order);
+// the group box is their min/max (:539-555).  Breadth-first from the
+// group's level-1 cells (:259-263, :511-512): a cell is opened iff
+// ds^2 < cellOp^2 (improved Barnes-Hut distance from the group box) or it
+// holds fewer than 3 particles (:284-285).  Opened nodes push their
+// children; unopened cells (nodes or leaves) go to the approximate (P2C)
+// list; opened leaves put every particle (tree order) on the direct (P2P)
+// list.  Lists are evaluated in list order, approximate list first.
+// acc_out[k] = (ax, ay, az, phi) at PH index k; phi includes the self
+// term -m_i/eps (P2P with dr = 0).
+// Stats (:567-609): per valid i-particle add its group's list lengths;
+// stats_out = {direct_sum, direct_max, approx_sum, approx_max} (integers),
+// pot_out = 0.5 * sum m_i phi_i.
+// group_cells[2*g] = (first level-1 cell, count) for group g's tree
+// (null: use level_range[2], [3]).
+// acc_type 0 = FP32 accumulation (as the reference), 1 = FP64.
+// group_inv_theta (may be null): per-group 1/theta_g; the cell's opening
+// radius is then recomputed as (l * inv_theta_g + s)^2 from cell_ls[c] =
+// (l, s), the same float operations as code/computeMultipoles.cu:174-177
+// (used by the learned-opening-angle path; null = the stored cellOp^2).
+// counts_out (may be null): per-group (n_approx, n_direct).
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+template <typename A>
+static void p2p(A acc[4], float px, float py, float pz, const F4 &j, float eps2) {
+  const float dx = j.x - px, dy = j.y - py, dz = j.z - pz;
+  const float r2 = dx * dx + dy * dy + dz * dz + eps2;
+  const float rinv = 1.0f / std::sqrt(r2);
+  const float rinv2 = rinv * rinv;
+  const float mrinv = j.w * rinv;
+  const float mrinv3 = mrinv * rinv2;
+  acc[3] -= mrinv;
+  acc[0] += mrinv3 * dx;
+  acc[1] += mrinv3 * dy;
+  acc[2] += mrinv3 * dz;
+}
+
+template <typename A>
+static void p2c(A acc[4], float px, float py, float pz, const F4 &m, const F4 &q0,
+                const F2 &q1, float eps2) {
+  const float dx = px - m.x, dy = py - m.y, dz = pz - m.z;
+  const float r2 = dx * dx + dy * dy + dz * dz + eps2;
+  const float rinv = 1.0f / std::sqrt(r2);
+  const float rinv2 = rinv * rinv;
+  const float mrinv = m.w * rinv;
+  const float mrinv3 = rinv2 * mrinv;
+  const float mrinv5 = rinv2 * mrinv3;
+  const float mrinv7 = rinv2 * mrinv5;
+  const float D0 = mrinv, D1 = -mrinv3, D2 = mrinv5 * 3.0f, D3 = mrinv7 * (-15.0f);
+  const float q11 = q0.x, q22 = q0.y, q33 = q0.z, q12 = q0.w, q13 = q1.x, q23 = q1.y;
+  const float q = q11 + q22 + q33;
+  const float qRx = q11 * dx + q12 * dy + q13 * dz;
+  const float qRy = q12 * dx + q22 * dy + q23 * dz;
+  const float qRz = q13 * dx + q23 * dy + q33 * dz;
+  const float qRR = qRx * dx + qRy * dy + qRz * dz;
+  acc[3] -= D0 + 0.5f * (D1 * q + D2 * qRR);
+  const float C = D1 + 0.5f * (D2 * q + D3 * qRR);
+  acc[0] += C * dx + D2 * qRx;
+  acc[1] += C * dy + D2 * qRy;
+  acc[2] += C * dz + D2 * qRz;
+}
+
+static inline bool open_test(const F4 &cs, float gcx, float gcy, float gcz, float gsx,
+                             float gsy, float gsz, const F2 *ls, float inv_theta) {
+  float dx = std::fabs(gcx - cs.x) - gsx;
+  float dy = std::fabs(gcy - cs.y) - gsy;
+  float dz = std::fabs(gcz - cs.z) - gsz;
+  dx += std::fabs(dx); dx *= 0.5f;
+  dy += std::fabs(dy); dy *= 0.5f;
+  dz += std::fabs(dz); dz *= 0.5f;
+  const float ds2 = dx * dx + dy * dy + dz * dz;
+  if (!ls) return ds2 < std::fabs(cs.w);
+  const float op = ls->x * inv_theta + ls->y;
+  return ds2 < op * op;
+}
+
+template <typename A>
+static void walk_impl(int ngroups, const I2 *groups, const int *group_cells,
+                      const int *level_range, const F4 *pos_ph, const F4 *pos_tree,
+                      const U4 *cells, const F4 *csize, const F4 *mono, const F4 *quad0,
+                      const F2 *quad1, float eps2, const F2 *cell_ls,
+                      const float *group_inv_theta, F4 *acc_out,
+                      long long *stats_out, double *pot_out, int *counts_out) {
+  long long dsum = 0, asum = 0, dmax = 0, amax = 0;
+  double pot = 0;
+#pragma omp parallel reduction(+ : dsum, asum, pot) reduction(max : dmax, amax)
+  {
+    std::vector<int> cur, nxt, alist, dlist;
+#pragma omp for schedule(dynamic, 16)
+    for (int g = 0; g < ngroups; g++) {
+      const int pbeg = groups[g].x, np = groups[g].y;
+      float mn[3] = {pos_ph[pbeg].x, pos_ph[pbeg].y, pos_ph[pbeg].z};
+      float mx[3] = {mn[0], mn[1], mn[2]};
+      for (int i = pbeg; i < pbeg + np; i++) {
+        mn[0] = std::min(mn[0], pos_ph[i].x); mx[0] = std::max(mx[0], pos_ph[i].x);
+        mn[1] = std::min(mn[1], pos_ph[i].y); mx[1] = std::max(mx[1], pos_ph[i].y);
+        mn[2] = std::min(mn[2], pos_ph[i].z); mx[2] = std::max(mx[2], pos_ph[i].z);
+      }
+      const float gcx = 0.5f * (mx[0] + mn[0]), gcy = 0.5f * (mx[1] + mn[1]),
+                  gcz = 0.5f * (mx[2] + mn[2]);
+      const float gsx = 0.5f * (mx[0] - mn[0]), gsy = 0.5f * (mx[1] - mn[1]),
+                  gsz = 0.5f * (mx[2] - mn[2]);
+      const float it = group_inv_theta ? group_inv_theta[g] : 0.0f;
+      const F2 *lsp = group_inv_theta ? cell_ls : nullptr;
+      int c0, nc;
+      if (group_cells) { c0 = group_cells[2 * g]; nc = group_cells[2 * g + 1]; }
+      else { c0 = level_range[2]; nc = level_range[3] - level_range[2] + 1; }
+      cur.clear(); alist.clear(); dlist.clear();
+      for (int c = c0; c < c0 + nc; c++) cur.push_back(c);
+      while (!cur.empty()) {
+        nxt.clear();
+        for (int c : cur) {
+          const U4 cd = cells[c];
+          const bool split = open_test(csize[c], gcx, gcy, gcz, gsx, gsy, gsz,
+                                       lsp ? lsp + c : nullptr, it) ||
+                             ((int)cd.w - (int)cd.z < 3);
+          const bool leaf = cell_is_leaf(cd);
+          if (!split) {
+            alist.push_back(c);
+          } else if (!leaf) {
+            const int fc = cell_first(cd), n = cell_n(cd);
+            for (int k = 0; k < n; k++) nxt.push_back(fc + k);
+          } else {
+            for (int p = (int)cd.z; p < (int)cd.w; p++) dlist.push_back(p);
+          }
+        }
+        cur.swap(nxt);
+      }
+      const long long na = (long long)alist.size(), nd = (long long)dlist.size();
+      if (counts_out) { counts_out[2 * g] = (int)na; counts_out[2 * g + 1] = (int)nd; }
+      for (int i = pbeg; i < pbeg + np; i++) {
+        A acc[4] = {0, 0, 0, 0};
+        const float px = pos_ph[i].x, py = pos_ph[i].y, pz = pos_ph[i].z;
+        for (int c : alist) p2c(acc, px, py, pz, mono[c], quad0[c], quad1[c], eps2);
+        for (int p : dlist) p2p(acc, px, py, pz, pos_tree[p], eps2);
+        acc_out[i] = F4{(float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]};
+        pot += 0.5 * (double)pos_ph[i].w * (double)(float)acc[3];
+      }
+      dsum += nd * np; asum += na * np;
+      dmax = std::max(dmax, nd); amax = std::max(amax, na);
+    }
+  }
+  stats_out[0] = dsum; stats_out[1] = dmax; stats_out[2] = asum; stats_out[3] = amax;
+  *pot_out = pot;
+}
+
+extern "C" {
+
+void or_walk(int ngroups, const I2 *groups, const int *group_cells, const int *level_range,
+             const F4 *pos_ph, const F4 *pos_tree, const U4 *cells, const F4 *csize,
+             const F4 *mono, const F4 *quad0, const F2 *quad1, float eps2, int acc_type,
+             const F2 *cell_ls, const float *group_inv_theta, F4 *acc_out,
+             long long *stats_out, double *pot_out, int *counts_out) {
+  if (acc_type == 0)
+    walk_impl<float>(ngroups, groups, group_cells, level_range, pos_ph, pos_tree, cells, csize,
+                     mono, quad0, quad1, eps2, cell_ls, group_inv_theta, acc_out, stats_out,
+                     pot_out, counts_out);
+  else
+    walk_impl<double>(ngroups, groups, group_cells, level_range, pos_ph, pos_tree, cells,
+                      csize, mono, quad0, quad1, eps2, cell_ls, group_inv_theta, acc_out,
+                      stats_out, pot_out, counts_out);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 direct sum (no reference counterpart; the error oracle).  For each
+// target t in idx[0..ns): a = sum_j m_j (x_j - x_t) / (r^2+eps2)^{3/2},
+// phi = -sum_j m_j / sqrt(r^2+eps2), over ALL j including j == t (so phi
+// carries the same self term as the walk).  Sources pos[0..n).
+// ---------------------------------------------------------------------------
+void or_direct_fp64(int n, const F4 *pos, int ns, const int *idx, double eps2, D4 *out) {
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int s = 0; s < ns; s++) {
+    const F4 pt = pos[idx[s]];
+    double ax = 0, ay = 0, az = 0, ph = 0;
+    for (int j = 0; j < n; j++) {
+      const double dx = (double)pos[j].x - pt.x, dy = (double)pos[j].y - pt.y,
+                   dz = (double)pos[j].z - pt.z;
+      const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+      const double ri = 1.0 / std::sqrt(r2);
+      const double mr = pos[j].w * ri;
+      const double mr3 = mr * ri * ri;
+      ax += mr3 * dx; ay += mr3 * dy; az += mr3 * dz; ph -= mr;
+    }
+    out[s] = D4{ax, ay, az, ph};
+  }
+}
+
+// Long-double brute force for pinning or_direct_fp64 on tiny inputs.
+void or_direct_ld(int n, const F4 *pos, int ns, const int *idx, double eps2, D4 *out) {
+  for (int s = 0; s < ns; s++) {
+    const F4 pt = pos[idx[s]];
+    long double ax = 0, ay = 0, az = 0, ph = 0;
+    for (int j = 0; j < n; j++) {
+      const long double dx = (long double)pos[j].x - pt.x, dy = (long double)pos[j].y - pt.y,
+                        dz = (long double)pos[j].z - pt.z;
+      const long double r2 = dx * dx + dy * dy + dz * dz + (long double)eps2;
+      const long double ri = 1.0L / std::sqrt(r2);
+      ax += pos[j].w * ri * ri * ri * dx; ay += pos[j].w * ri * ri * ri * dy;
+      az += pos[j].w * ri * ri * ri * dz; ph -= pos[j].w * ri;
+    }
+    out[s] = D4{(double)ax, (double)ay, (double)az, (double)ph};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Energies (the reference's computeEnergies is a stub, code/computeEnergies
+// .cu:3-7).  FP64: K = 1/2 sum m v^2, W = 1/2 sum m (phi + m/eps) (the walk's
+// phi includes the self term -m/eps, code/computeForces.cu:64-72).
+// ---------------------------------------------------------------------------
+void or_energies(int n, const F4 *pos, const F4 *vel, const F4 *acc, double eps,
+                 double *ekin, double *epot) {
+  double k = 0, w = 0;
+#pragma omp parallel for reduction(+ : k, w)
+  for (int i = 0; i < n; i++) {
+    const double m = pos[i].w;
+    k += 0.5 * m * ((double)vel[i].x * vel[i].x + (double)vel[i].y * vel[i].y +
+                    (double)vel[i].z * vel[i].z);
+    w += 0.5 * m * ((double)acc[i].w + (eps > 0 ? m / eps : 0.0));
+  }
+  *ekin = k;
+  *epot = w;
+}
+
+// ---------------------------------------------------------------------------
+// Reference Plummer sampler: code/plummer.h:37-117 with glibc srand/rand
+// (my_rand = rand()/(1+RAND_MAX), :135-137).  Radius from the cumulative
+// mass, isotropic direction, Aarseth-Henon-Wielen velocity rejection,
+// Henon scaling 3pi/16, m = 1/n, centre-of-mass position and velocity
+// removed in FP64.  rmax is the truncation radius in structural units
+// (the reference uses 100).  Output in double: pos/vel [3n], mass [n].
+// ---------------------------------------------------------------------------
+void or_plummer_ref(long n, unsigned seed, double rmax, double *mass, double *pos,
+                    double *vel) {
+  srand(seed);
+  auto urand = []() { return rand() / (1. + RAND_MAX); };
+  long i = 0;
+  while (i < n) {
+    const double X1 = urand(), X2 = urand(), X3 = urand();
+    const double R = 1.0 / std::sqrt(std::pow(X1, -2.0 / 3.0) - 1.0);
+    if (!(R < rmax)) continue;
+    const double Z = (1.0 - 2.0 * X2) * R;
+    const double X = std::sqrt(R * R - Z * Z) * std::cos(2.0 * M_PI * X3);
+    const double Y = std::sqrt(R * R - Z * Z) * std::sin(2.0 * M_PI * X3);
+    const double Ve = std::sqrt(2.0) * std::pow(1.0 + R * R, -0.25);
+    double X4 = 0.0, X5 = 0.0;
+    while (0.1 * X5 >= X4 * X4 * std::pow(1.0 - X4 * X4, 3.5)) { X4 = urand(); X5 = urand(); }
+    const double V = Ve * X4;
+    const double X6 = urand(), X7 = urand();
+    const double Vz = (1.0 - 2.0 * X6) * V;
+    const double Vx = std::sqrt(V * V - Vz * Vz) * std::cos(2.0 * M_PI * X7);
+    const double Vy = std::sqrt(V * V - Vz * Vz) * std::sin(2.0 * M_PI * X7);
+    const double conv = 3.0 * M_PI / 16.0;
+    mass[i] = 1.0 / n;
+    pos[3 * i] = X * conv; pos[3 * i + 1] = Y * conv; pos[3 * i + 2] = Z * conv;
+    vel[3 * i] = Vx / std::sqrt(conv); vel[3 * i + 1] = Vy / std::sqrt(conv);
+    vel[3 * i + 2] = Vz / std::sqrt(conv);
+    i++;
+  }
+  double mc = 0, xc[3] = {0, 0, 0}, vc[3] = {0, 0, 0};
+  for (long k = 0; k < n; k++) {
+    mc += mass[k];
+    for (int d = 0; d < 3; d++) { xc[d] += mass[k] * pos[3 * k + d]; vc[d] += mass[k] * vel[3 * k + d]; }
+  }
+  for (int d = 0; d < 3; d++) { xc[d] /= mc; vc[d] /= mc; }
+  for (long k = 0; k < n; k++)
+    for (int d = 0; d < 3; d++) { pos[3 * k + d] -= xc[d]; vel[3 * k + d] -= vc[d]; }
+}
+
+}  // extern "C"
