@@ -1,0 +1,196 @@
+"""Benchmark: one full gravity step (buildTree -> computeMultipoles -> makeGroups
+-> computeForces -> kick-drift-kick) per timed step on one MI355X.
+
+Default workload: N = 2^24 Plummer halo (mass 0.8) + exponential disk
+(mass 0.2, scale length 0.3, sech^2 height 0.03), float32, eps = 0.002,
+fixed theta = 0.4, NLEAF 16, NCRIT 64, levelSplit 5, dt = 1/128.  Synthetic
+seeded input, resident on the device before timing starts.
+
+Prints ONE JSON line.  value = force evaluations per second of the whole step
+(N / step time).  The walk kernel's roofline uses the reference's flop count
+(20 per P2P, 64 per P2C; code/treecode.cu:150-158) over the walk's own
+duration, taken with HIP events on the stream the kernels run on.
+cpu_baseline = the OpenMP C++ oracle (a transcription of the reference path)
+on the host cores: build, multipoles and groups in full, the walk on a seeded
+sample of the groups, scaled to all groups.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("force-evals/s at N=2^24, p99 rel force err ≤1e-3 (learned vs fixed θ); "
+          "interactions/s")
+FP32_PEAK_TFLOPS = 157.3
+
+WORKLOADS = {
+    "halo_disk_2p24": dict(n=1 << 24, ic="halo_disk", theta=0.4, eps=0.002),
+    "plummer_2p24": dict(n=1 << 24, ic="plummer", theta=0.5, eps=0.002),
+    "plummer_2p22": dict(n=1 << 22, ic="plummer", theta=0.5, eps=0.005),
+    "plummer_2p20": dict(n=1 << 20, ic="plummer", theta=0.5, eps=0.005),
+}
+
+
+def make_ic(bt, ic, n, seed):
+    if ic == "plummer":
+        return bt.plummer(n, seed=seed)
+    if ic == "clustered":
+        return bt.clustered(n, seed=seed)
+    return bt.halo_disk(n, seed=seed)
+
+
+def cpu_baseline(pos, nleaf, ncrit, theta, eps, level_split, sample_groups, seed=5):
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    box, _, _ = O.bounding_box(pos)
+    tree = O.build_tree(pos, nleaf, box)
+    pos_tree = pos[tree.order]
+    t1 = time.perf_counter()
+    size, mono, q0, q1, _ = O.multipoles(tree.cells, pos_tree, theta)
+    t2 = time.perf_counter()
+    keys = O.ph_keys(pos_tree, box)
+    perm = O.sort_keys(keys)
+    pos_ph = pos_tree[perm]
+    groups = O.make_groups(keys[perm], level_split, ncrit)
+    t3 = time.perf_counter()
+    rng = np.random.default_rng(seed)
+    ns = min(len(groups), sample_groups)
+    sel = np.sort(rng.choice(len(groups), size=ns, replace=False))
+    O.walk(groups[sel], tree.level_range, pos_ph, pos_tree, tree.cells, size, mono, q0, q1,
+           eps)
+    t4 = time.perf_counter()
+    frac = ns / len(groups)
+    t_walk = (t4 - t3) / frac
+    total = (t3 - t0) + t_walk
+    return dict(value=len(pos) / total, unit="force-evals/s",
+                cores=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                kind="port",
+                sample="full build/multipoles/groups (%.1f s); walk on %d of %d groups "
+                       "(seeded, %.2f%%, %.1f s) scaled x%.0f" % (
+                           t3 - t0, ns, len(groups), 100 * frac, t4 - t3, 1 / frac),
+                walk_s_scaled=t_walk, build_s=t1 - t0, multipoles_s=t2 - t1, groups_s=t3 - t2)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="halo_disk_2p24", choices=sorted(WORKLOADS))
+    ap.add_argument("--n", type=int, default=0, help="override the particle count")
+    ap.add_argument("--nleaf", type=int, default=16)
+    ap.add_argument("--ncrit", type=int, default=64)
+    ap.add_argument("--level-split", type=int, default=5)
+    ap.add_argument("--dt", type=float, default=1.0 / 128)
+    ap.add_argument("--samples", type=int, default=4096)
+    ap.add_argument("--cpu-groups", type=int, default=1024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-kahan", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X "
+              "(multi-GPU sharding is design-only); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import bonsai2_amd as bt
+    w = dict(WORKLOADS[args.workload])
+    if args.n:
+        w["n"] = args.n
+    n = w["n"]
+    seed = 19810614
+    pos, vel = make_ic(bt, w["ic"], n, seed)
+
+    t = bt.Treecode(w["eps"], w["theta"], args.ncrit)
+    if args.no_kahan:
+        t.set_walk_options(kahan=0)
+    t.alloc(n)
+    t.ptcl_h2d(pos, vel)
+    # first force evaluation (not timed), then warmup steps
+    t.buildTree(args.nleaf)
+    t.computeMultipoles()
+    t.makeGroups(args.level_split, args.ncrit)
+    t.computeForces()
+    for _ in range(args.warmup):
+        t.step(args.dt, args.nleaf, args.level_split, args.ncrit)
+
+    walk_ms, phase = [], []
+    flops, inter = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t.step(args.dt, args.nleaf, args.level_split, args.ncrit)  # synchronises at the end
+        tm = t.timings()
+        phase.append(tm)
+        walk_ms.append(tm["walk"])
+        _, tot = t.walk_stats()
+        flops.append(20.0 * tot[0] + 64.0 * tot[1])
+        inter.append(tot[0] + tot[1])
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+
+    walk_ms = np.array(walk_ms)
+    flops = np.array(flops)
+    inter = np.array(inter)
+    achieved = float((flops / (walk_ms * 1e-3)).mean() / 1e12)
+    # accuracy against the FP64 direct sum on seeded samples (outside the timed region)
+    rng = np.random.default_rng(seed + 1)
+    idx = np.sort(rng.choice(n, size=min(n, args.samples), replace=False)).astype(np.int32)
+    d = t.direct_fp64(idx)
+    a = t.acc(bt.ORDER_PH)[idx, :3].astype(np.float64)
+    err = np.linalg.norm(a - d[:, :3], axis=1) / np.linalg.norm(d[:, :3], axis=1)
+    p50, p99 = (float(x) for x in np.percentile(err, [50, 99]))
+    ek, ep = t.computeEnergies()
+    c = t.counts()
+
+    phase_ms = {k: float(np.mean([p[k] for p in phase])) for k in phase[0]}
+    out = {
+        "metric": METRIC,
+        "value": n / (ms_step * 1e-3),
+        "unit": "force-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded %s, resident in HBM)" % w["ic"],
+        "config": {"workload": args.workload, "n": n, "theta": w["theta"], "eps": w["eps"],
+                   "nleaf": args.nleaf, "ncrit": args.ncrit, "level_split": args.level_split,
+                   "dt": args.dt, "opening": "fixed theta", "kahan": not args.no_kahan},
+        "interactions_per_s": float((inter / (walk_ms * 1e-3)).mean()),
+        "walk_ms": float(walk_ms.mean()),
+        "force_evals_per_s_walk": float(n / (walk_ms.mean() * 1e-3)),
+        "p50_rel_err": p50,
+        "p99_rel_err": p99,
+        "error_samples": int(len(idx)),
+        "energy": {"kinetic": float(ek[0]), "potential": float(ep[0])},
+        "tree": {"ncells": c["ncells"], "nlevels": c["nlevels"], "ngroups": c["ngroups"]},
+        "phase_ms": phase_ms,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_walk",
+                     "flops_per_launch": float(flops.mean()),
+                     "avg_launch_ms": float(walk_ms.mean())},
+    }
+    st, tot = t.walk_stats()
+    out["interactions_per_particle"] = {"p2p": st[0], "p2c": st[2], "p2p_max": st[1],
+                                        "p2c_max": st[3]}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(pos, args.nleaf, args.ncrit, w["theta"], w["eps"],
+                                           args.level_split, args.cpu_groups)
+    else:
+        out["cpu_baseline"] = None
+    t.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

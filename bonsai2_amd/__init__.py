@@ -1,0 +1,354 @@
+"""bonsai2_amd -- MI355X Barnes-Hut treecode (host-side Python mirror).
+
+``Treecode`` mirrors the reference class Treecode<float>
+(code/Treecode.h:123-223): the same method names, argument meanings and call
+order (alloc -> particles -> buildTree -> computeMultipoles -> makeGroups ->
+computeForces -> moveParticles / computeEnergies).  Every call goes through
+the C ABI of libbonsai2_amd.so (include/bonsai2_amd.h), which runs the
+hand-written HIP kernels; there is no CPU fallback -- if the library is not
+built, importing the binding raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["Treecode", "lib", "BonsaiError", "ORDER_INPUT", "ORDER_TREE", "ORDER_PH",
+           "plummer", "clustered", "halo_disk", "read_tipsy", "write_tipsy",
+           "EXPORTED_SYMBOLS"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbonsai2_amd.so")
+
+ORDER_INPUT, ORDER_TREE, ORDER_PH = 0, 1, 2
+P = ctypes.c_void_p
+I = ctypes.c_int
+F = ctypes.c_float
+D = ctypes.c_double
+
+# name -> argtypes (restype int for all)
+_SIGS = {
+    "bt_create": [ctypes.POINTER(P), F, F, I],
+    "bt_destroy": [P],
+    "bt_alloc": [P, I, I, P],
+    "bt_set_stream": [P, P],
+    "bt_set_params": [P, F, F],
+    "bt_set_particles": [P, P, P],
+    "bt_set_particles_device": [P, P, P],
+    "bt_get_particles": [P, P, P, I],
+    "bt_build_tree": [P, I],
+    "bt_compute_multipoles": [P],
+    "bt_make_groups": [P, I, I],
+    "bt_compute_forces": [P, I, P],
+    "bt_move_particles": [P, F],
+    "bt_kick": [P, F],
+    "bt_compute_energies": [P, P, P],
+    "bt_step": [P, F, I, I, I],
+    "bt_get_acc": [P, P, I],
+    "bt_get_potential": [P, P],
+    "bt_get_walk_stats": [P, P, P],
+    "bt_get_counts": [P, P],
+    "bt_get_tree": [P, P, P, P, P, P, P],
+    "bt_get_cell_ls": [P, P],
+    "bt_get_groups": [P, P],
+    "bt_get_group_counts": [P, P],
+    "bt_get_ph": [P, P, P],
+    "bt_get_ids": [P, P, I],
+    "bt_get_boxes": [P, P],
+    "bt_direct_fp64": [P, I, P, P],
+    "bt_get_timings": [P, P],
+    "bt_set_walk_options": [P, I, I, I],
+    "bt_ic_plummer": [I, ctypes.c_uint64, D, P, P],
+    "bt_ic_clustered": [I, I, ctypes.c_uint64, D, D, D, P, P],
+    "bt_ic_halo_disk": [I, ctypes.c_uint64, D, D, D, P, P],
+    "bt_read_tipsy": [ctypes.c_char_p, P, P, P, I],
+    "bt_write_tipsy": [ctypes.c_char_p, I, P, P, D, F],
+}
+EXPORTED_SYMBOLS = sorted(list(_SIGS) + ["bt_last_error"])
+
+_LIB = None
+
+
+class BonsaiError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded libbonsai2_amd.so (raises if it is not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise BonsaiError("libbonsai2_amd.so is not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = I
+        L.bt_last_error.argtypes = [P]
+        L.bt_last_error.restype = ctypes.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+def _f4(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] != 4:
+        raise ValueError("expected an (n, 4) float32 array")
+    return a
+
+
+class Treecode:
+    """Drop-in mirror of Treecode<float> (code/Treecode.h:164-221).
+
+    Treecode(eps=0.01, theta=0.75, ncrit=64) as the reference constructor.
+    """
+
+    def __init__(self, eps=0.01, theta=0.75, ncrit=64):
+        self._L = lib()
+        h = P()
+        rc = self._L.bt_create(ctypes.byref(h), float(eps), float(theta), int(ncrit))
+        if rc != 0:
+            raise BonsaiError("bt_create failed (%d): no HIP device or runtime error" % rc)
+        self._h = h
+        self.eps = float(np.float32(eps))
+        self.theta = float(np.float32(theta))
+        self.nCrit = int(ncrit)
+        self.nLeaf = 16
+        self.nPtcl = 0
+        self.nsys = 1
+        self.offsets = None
+        self.grav_potential = 0.0
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.bt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            msg = self._L.bt_last_error(self._h)
+            raise BonsaiError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def alloc(self, n, offsets=None):
+        """Treecode::alloc(n) (code/Treecode.h:175-198); offsets -> a forest of systems."""
+        if offsets is None:
+            self._ck(self._L.bt_alloc(self._h, int(n), 1, None), "alloc")
+            self.nsys, self.offsets = 1, np.array([0, n], np.int32)
+        else:
+            off = np.ascontiguousarray(offsets, np.int32)
+            self._ck(self._L.bt_alloc(self._h, int(n), len(off) - 1, _p(off)), "alloc")
+            self.nsys, self.offsets = len(off) - 1, off
+        self.nPtcl = int(n)
+
+    def set_params(self, eps=None, theta=None):
+        if eps is not None:
+            self.eps = float(np.float32(eps))
+        if theta is not None:
+            self.theta = float(np.float32(theta))
+        self._ck(self._L.bt_set_params(self._h, self.eps, self.theta), "set_params")
+
+    def set_walk_options(self, kahan=1, stack_ints=16384, blocks_per_cu=0):
+        self._ck(self._L.bt_set_walk_options(self._h, int(kahan), int(stack_ints),
+                                             int(blocks_per_cu)), "set_walk_options")
+
+    # -- particles ---------------------------------------------------------
+    def ptcl_h2d(self, pos, vel):
+        """ptcl_h2d() (code/Treecode.h:207-214): pos/vel are (n,4) float32."""
+        pos, vel = _f4(pos), _f4(vel)
+        if len(pos) != self.nPtcl or len(vel) != self.nPtcl:
+            raise ValueError("particle count differs from alloc()")
+        self._ck(self._L.bt_set_particles(self._h, _p(pos), _p(vel)), "ptcl_h2d")
+
+    def ptcl_d2h(self, order=ORDER_INPUT):
+        """ptcl_d2h() (code/Treecode.h:200-204) -> (pos, vel) in the given order."""
+        pos = np.empty((self.nPtcl, 4), np.float32)
+        vel = np.empty((self.nPtcl, 4), np.float32)
+        self._ck(self._L.bt_get_particles(self._h, _p(pos), _p(vel), int(order)), "ptcl_d2h")
+        return pos, vel
+
+    # -- the gravity step (reference method names) ----------------------------
+    def buildTree(self, nLeaf=16):
+        self._ck(self._L.bt_build_tree(self._h, int(nLeaf)), "buildTree")
+        self.nLeaf = int(nLeaf)
+
+    def computeMultipoles(self):
+        self._ck(self._L.bt_compute_multipoles(self._h), "computeMultipoles")
+
+    def makeGroups(self, levelSplit=1, nCrit=64):
+        self._ck(self._L.bt_make_groups(self._h, int(levelSplit), int(nCrit)), "makeGroups")
+        self.nCrit = int(nCrit)
+
+    def computeForces(self, INTCOUNT=True):
+        """Returns (direct <n>, direct max, approx <n>, approx max) (code/computeForces.cu:701-715)."""
+        st = np.zeros(4, np.float64)
+        self._ck(self._L.bt_compute_forces(self._h, 1 if INTCOUNT else 0, _p(st)),
+                 "computeForces")
+        pot = np.zeros(1, np.float64)
+        self._ck(self._L.bt_get_potential(self._h, _p(pot)), "grav_potential")
+        self.grav_potential = float(pot[0])
+        return tuple(float(x) for x in st)
+
+    def moveParticles(self, dt):
+        self._ck(self._L.bt_move_particles(self._h, float(dt)), "moveParticles")
+
+    def kick(self, dt_half):
+        self._ck(self._L.bt_kick(self._h, float(dt_half)), "kick")
+
+    def computeEnergies(self):
+        """FP64 (K, W) per system (arrays of length nsys)."""
+        ek = np.zeros(self.nsys, np.float64)
+        ep = np.zeros(self.nsys, np.float64)
+        self._ck(self._L.bt_compute_energies(self._h, _p(ek), _p(ep)), "computeEnergies")
+        return ek, ep
+
+    def step(self, dt, nLeaf=16, levelSplit=5, nCrit=64):
+        self._ck(self._L.bt_step(self._h, float(dt), int(nLeaf), int(levelSplit), int(nCrit)),
+                 "step")
+
+    # -- results -------------------------------------------------------------
+    def get_nPtcl(self):
+        return self.nPtcl
+
+    def get_nCrit(self):
+        return self.nCrit
+
+    def get_nLeaf(self):
+        return self.nLeaf
+
+    def walk_stats(self):
+        """(stats[4], totals) of the last walk: totals = (n_P2P, n_P2C) summed over particles."""
+        st = np.zeros(4, np.float64)
+        tot = np.zeros(2, np.float64)
+        self._ck(self._L.bt_get_walk_stats(self._h, _p(st), _p(tot)), "get_walk_stats")
+        return st, tot
+
+    def counts(self):
+        c = np.zeros(8, np.int32)
+        self._ck(self._L.bt_get_counts(self._h, _p(c)), "counts")
+        keys = ["n", "ncells", "nnodes", "nleaves", "nlevels", "ngroups", "nsys", "flags"]
+        return dict(zip(keys, (int(x) for x in c)))
+
+    def acc(self, order=ORDER_PH):
+        a = np.empty((self.nPtcl, 4), np.float32)
+        self._ck(self._L.bt_get_acc(self._h, _p(a), int(order)), "get_acc")
+        return a
+
+    def tree(self, multipoles=True):
+        """Cells (CellData uint4), level table int2[32] and, optionally, the moments."""
+        nc = self.counts()["ncells"]
+        cells = np.empty((nc, 4), np.uint32)
+        lr = np.empty((32, 2), np.int32)
+        if multipoles:
+            size = np.empty((nc, 4), np.float32)
+            mono = np.empty((nc, 4), np.float32)
+            q0 = np.empty((nc, 4), np.float32)
+            q1 = np.empty((nc, 2), np.float32)
+            self._ck(self._L.bt_get_tree(self._h, _p(cells), _p(lr), _p(size), _p(mono),
+                                         _p(q0), _p(q1)), "get_tree")
+            return dict(cells=cells, level_range=lr, size=size, mono=mono, q0=q0, q1=q1)
+        self._ck(self._L.bt_get_tree(self._h, _p(cells), _p(lr), None, None, None, None),
+                 "get_tree")
+        return dict(cells=cells, level_range=lr)
+
+    def cell_ls(self):
+        nc = self.counts()["ncells"]
+        ls = np.empty((nc, 2), np.float32)
+        self._ck(self._L.bt_get_cell_ls(self._h, _p(ls)), "get_cell_ls")
+        return ls
+
+    def groups(self):
+        ng = self.counts()["ngroups"]
+        g = np.empty((ng, 2), np.int32)
+        self._ck(self._L.bt_get_groups(self._h, _p(g)), "get_groups")
+        return g
+
+    def group_counts(self):
+        ng = self.counts()["ngroups"]
+        g = np.empty((ng, 2), np.int32)
+        self._ck(self._L.bt_get_group_counts(self._h, _p(g)), "get_group_counts")
+        return g
+
+    def ph(self):
+        keys = np.empty(self.nPtcl, np.uint64)
+        perm = np.empty(self.nPtcl, np.int32)
+        self._ck(self._L.bt_get_ph(self._h, _p(keys), _p(perm)), "get_ph")
+        return keys, perm
+
+    def ids(self, order):
+        ids = np.empty(self.nPtcl, np.int32)
+        self._ck(self._L.bt_get_ids(self._h, _p(ids), int(order)), "get_ids")
+        return ids
+
+    def boxes(self):
+        b = np.empty((self.nsys, 4), np.float32)
+        self._ck(self._L.bt_get_boxes(self._h, _p(b)), "get_boxes")
+        return b
+
+    def direct_fp64(self, ph_idx):
+        idx = np.ascontiguousarray(ph_idx, np.int32)
+        out = np.empty((len(idx), 4), np.float64)
+        self._ck(self._L.bt_direct_fp64(self._h, len(idx), _p(idx), _p(out)), "direct_fp64")
+        return out
+
+    def timings(self):
+        t = np.zeros(10, np.float32)
+        self._ck(self._L.bt_get_timings(self._h, _p(t)), "get_timings")
+        names = ["bbox_keys", "sort", "levels", "multipoles", "ph_keys", "ph_sort", "groups",
+                 "walk", "integrate", "energies"]
+        return dict(zip(names, (float(x) for x in t)))
+
+
+# ---- initial conditions and tipsy I/O (host only) ---------------------------
+
+def _ic(fn, n, *args):
+    pos = np.empty((n, 4), np.float32)
+    vel = np.empty((n, 4), np.float32)
+    rc = fn(int(n), *args, _p(pos), _p(vel))
+    if rc != 0:
+        raise BonsaiError("initial-condition generator failed (%d)" % rc)
+    return pos, vel
+
+
+def plummer(n, seed=19810614, rmax=22.8):
+    """Plummer sphere in Henon units, truncated at rmax (structural units)."""
+    return _ic(lib().bt_ic_plummer, n, ctypes.c_uint64(seed), float(rmax))
+
+
+def clustered(n, nclust=64, seed=19810614, box=10.0, mmin=1e-3, a0=0.5):
+    return _ic(lib().bt_ic_clustered, n, int(nclust), ctypes.c_uint64(seed), float(box),
+               float(mmin), float(a0))
+
+
+def halo_disk(n, seed=19810614, mh=0.8, rd=0.3, z0=0.03):
+    return _ic(lib().bt_ic_halo_disk, n, ctypes.c_uint64(seed), float(mh), float(rd), float(z0))
+
+
+def read_tipsy(path):
+    n = np.zeros(1, np.int32)
+    rc = lib().bt_read_tipsy(path.encode(), _p(n), None, None, 0)
+    if rc != 0:
+        raise BonsaiError("read_tipsy failed (%d)" % rc)
+    pos = np.empty((int(n[0]), 4), np.float32)
+    vel = np.empty((int(n[0]), 4), np.float32)
+    rc = lib().bt_read_tipsy(path.encode(), _p(n), _p(pos), _p(vel), int(n[0]))
+    if rc != 0:
+        raise BonsaiError("read_tipsy failed (%d)" % rc)
+    return pos, vel
+
+
+def write_tipsy(path, pos, vel, time=0.0, eps=0.0):
+    pos, vel = _f4(pos), _f4(vel)
+    rc = lib().bt_write_tipsy(path.encode(), len(pos), _p(pos), _p(vel), float(time), float(eps))
+    if rc != 0:
+        raise BonsaiError("write_tipsy failed (%d)" % rc)

@@ -1,0 +1,150 @@
+// bt_internal.h -- context state and kernel launchers of libbonsai2_amd.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/bonsai2_amd.h"
+
+namespace bt {
+
+constexpr int kLevels = 32;   // level table slots (reference int2[32], code/Treecode.h:171)
+#ifndef BT_LEVEL_BLOCKS
+#define BT_LEVEL_BLOCKS 512
+#endif
+constexpr int kLevelBlocks = BT_LEVEL_BLOCKS;  // fixed grid of the per-level tree kernels
+constexpr int kFp64Levels = 4;     // cells at levels <= this combine moments in FP64
+
+// Device-resident counters of one context (one allocation).
+struct DevCounters {
+  int lvl_start[kLevels + 2];
+  int lvl_count[kLevels + 2];
+  int nnodes, nleaves, nlevels, ncells;
+  int ngroups;
+  int flags;   // BT_FLAG_*
+  int err;     // 1 = cell capacity, 2 = group capacity, 4 = walk stack
+  int queue;   // walk group queue head
+  int pad[8];
+};
+
+struct Ctx {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  float eps = 0.01f, theta = 0.75f;
+  int ncrit = 64, nleaf = 16, level_split = 1;
+
+  // sizes
+  int n = 0, cap_n = 0, nsys = 1, max_sys_n = 0;
+  std::vector<int> h_off;
+  int *d_off = nullptr;
+
+  // particles: "current" arrays hold input order until the first build, then
+  // tree order.  *_2 are the reorder targets.
+  float4 *pos = nullptr, *vel = nullptr, *pos2 = nullptr, *vel2 = nullptr;
+  int *id = nullptr, *id2 = nullptr;
+
+  // sort buffers
+  uint64_t *keyB = nullptr;   // sort scratch
+  int *valA = nullptr, *valB = nullptr;
+  int *sort_hist = nullptr;   // 256 x blocks
+  int sort_hist_cap = 0;
+  uint64_t *geo_keys = nullptr;   // sorted geometric keys (tree order)
+
+  // scans
+  int *scan_blk = nullptr;
+  int scan_blk_cap = 0;
+
+  // boxes
+  float4 *box = nullptr;       // per system {cx, cy, cz, hsize}
+  float4 *bbox_part = nullptr; // partial min/max
+  int bbox_blocks = 1;
+
+  // cells
+  int cap_cells = 0;
+  uint4 *cells = nullptr;
+  float4 *csize = nullptr, *cmono = nullptr, *cq0 = nullptr;
+  float2 *cq1 = nullptr, *cls = nullptr;
+  float4 *cscr = nullptr;      // 6 float4 per cell (bottom-up moments)
+  int *child_bnd = nullptr;    // 8 ints per parent of the current level
+  int *lvl_blk = nullptr;      // block sums / offsets of the level scan
+  int2 *level_beg = nullptr;   // int2[32] (reference d_level_begIdx)
+  int2 *root_cells = nullptr;  // per system (first level-1 cell, count)
+
+  // PH order
+  float4 *pos_ph = nullptr;
+  int *perm = nullptr;         // PH index -> tree index
+  int *id_ph = nullptr;        // PH index -> input index
+  uint64_t *ph_keys = nullptr; // sorted PH keys
+  float4 *acc = nullptr;       // (ax, ay, az, phi) in PH order
+
+  // groups
+  int cap_groups = 0;
+  int2 *groups = nullptr;
+  int *group_sys = nullptr;
+  int *gtmp0 = nullptr, *gtmp1 = nullptr;  // N-sized scan scratch
+  int2 *gcounts = nullptr;     // per group (n_approx, n_direct)
+  double *gpot = nullptr;      // per group sum m*phi
+
+  // walk
+  int *stack = nullptr;
+  size_t stack_alloc = 0;
+  int stack_ints = 16384;
+  int walk_blocks = 0;         // persistent grid (blocks of 256)
+  int blocks_per_cu = 0;
+  int kahan = 1;
+  int num_cus = 256;
+  unsigned long long *d_wstats = nullptr;  // direct_sum, direct_max, approx_sum, approx_max
+  double *d_wpot = nullptr;
+
+  // energies / direct sum scratch
+  double *d_energy = nullptr;  // 2 * nsys
+  double *d_direct = nullptr;
+  int direct_cap = 0;
+
+  DevCounters *dc = nullptr;
+  DevCounters hc;
+
+  // state
+  bool have_particles = false, have_tree = false, have_mp = false, have_groups = false,
+       have_acc = false;
+  int tree_version = 0;
+
+  // timing
+  hipEvent_t ev[12] = {};
+  float ms[10] = {};
+  bool timing = true;
+
+  std::string err;
+};
+
+// ---- launchers (each queues work on c.stream) --------------------------
+void launch_bbox(Ctx &c);
+void launch_geo_keys(Ctx &c);
+void launch_gather_tree(Ctx &c);
+void launch_levels(Ctx &c);
+void launch_multipoles(Ctx &c);
+
+void radix_sort_pairs(Ctx &c, uint64_t *keys, int *vals, uint64_t *keys_tmp, int *vals_tmp,
+                      int n, int key_bits, bool by_system_after);
+void scan_exclusive_sum(Ctx &c, const int *in, int *out, int n, int *total_dev);
+void scan_inclusive_max(Ctx &c, const int *in, int *out, int n);
+void fill_int(Ctx &c, int *p, int value, int n);
+
+void launch_ph_keys(Ctx &c);
+void launch_gather_ph(Ctx &c);
+void launch_groups(Ctx &c);
+
+void launch_walk(Ctx &c);
+
+void launch_kick(Ctx &c, float dt_half);
+void launch_drift(Ctx &c, float dt);
+void launch_energies(Ctx &c);
+void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out);
+
+// host-side helpers
+int grid_for(int n, int block, int cap);
+
+}  // namespace bt

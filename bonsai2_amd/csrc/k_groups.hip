@@ -1,0 +1,168 @@
+// k_groups.hip -- makeGroups on CDNA4 (reference code/makeGroups.cu:236-341).
+//
+// PH keys of the tree-ordered particles (code/makeGroups.cu:108-135, the
+// Hilbert state machine of :19-106), a stable radix sort carrying the tree
+// index (:270-277), the gather into PH order (:280), then groups: every run
+// of equal keys masked to the top level_split octal digits (:137-192) is cut
+// into chunks of ncrit (:196-223).  The reference finds run bounds with two
+// inclusive_scan_by_key calls and takes group slots from an atomic counter;
+// here a max-scan gives each particle its run start and a sum-scan over the
+// "first particle of a group" flags gives deterministic group slots in PH
+// order.
+#include "bt_device.h"
+#include "bt_internal.h"
+
+namespace bt {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// Output digit per Hilbert index (x<<2 | y<<1 | z): {0,1,7,6,3,2,4,5}, packed
+// 3 bits per entry.
+constexpr uint32_t kHilbertDigits = (0u << 0) | (1u << 3) | (7u << 6) | (6u << 9) | (3u << 12) |
+                                    (2u << 15) | (4u << 18) | (5u << 21);
+
+__device__ __forceinline__ uint64_t hilbert_key21(int x, int y, int z) {
+  uint64_t key = 0;
+#pragma unroll
+  for (int lev = 20; lev >= 0; lev--) {
+    const int bit = 1 << lev;
+    const int idx = ((x & bit) ? 4 : 0) | ((y & bit) ? 2 : 0) | ((z & bit) ? 1 : 0);
+    int nx = x, ny = y, nz = z;
+    if (idx == 0) {               // swap y and z
+      ny = z; nz = y;
+    } else if (idx == 1 || idx == 5) {  // swap x and y
+      nx = y; ny = x;
+    } else if (idx == 4 || idx == 6) {  // complement x and z
+      nx = ~x; nz = ~z;
+    } else if (idx == 3 || idx == 7) {  // x <- ~y, y <- ~x
+      nx = ~y; ny = ~x;
+    } else {                      // idx 2: y <- ~z, z <- ~y
+      ny = ~z; nz = ~y;
+    }
+    x = nx; y = ny; z = nz;
+    key = (key << 3) + ((kHilbertDigits >> (3 * idx)) & 7u);
+  }
+  return key;
+}
+
+__global__ __launch_bounds__(kBlock) void k_ph_keys(int n, int nsys, const int *__restrict__ off,
+                                                    const float4 *__restrict__ box,
+                                                    const float4 *__restrict__ pos,
+                                                    uint64_t *__restrict__ keys,
+                                                    int *__restrict__ vals) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const int r = nsys > 1 ? system_of(i, nsys, off) : 0;
+    const float4 b = box[r];
+    const float4 p = pos[i];
+    int xc, yc, zc;
+    {
+#pragma clang fp contract(off)
+      // code/makeGroups.cu:123-131, evaluated left to right in float
+      const float inv = 0.5f / b.w;
+      const float bx = b.x - b.w, by = b.y - b.w, bz = b.z - b.w;
+      const float scale = 2097152.0f;  // 1 << 21
+      xc = (int)((p.x - bx) * inv * scale);
+      yc = (int)((p.y - by) * inv * scale);
+      zc = (int)((p.z - bz) * inv * scale);
+    }
+    keys[i] = hilbert_key21(xc, yc, zc);
+    vals[i] = i;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_ph(int n, const int *__restrict__ perm,
+                                                      const float4 *__restrict__ pos,
+                                                      const int *__restrict__ id,
+                                                      float4 *__restrict__ pos_ph,
+                                                      int *__restrict__ id_ph) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const int j = perm[i];
+    pos_ph[i] = pos[j];
+    id_ph[i] = id[j];
+  }
+}
+
+// v[i] = i at the start of a run of equal masked keys (or of a system), else 0.
+__global__ __launch_bounds__(kBlock) void k_run_starts(int n, int nsys, const int *__restrict__ off,
+                                                       const uint64_t *__restrict__ keys,
+                                                       uint64_t mask, int *__restrict__ v) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    bool start = i == 0;
+    if (!start) {
+      start = (keys[i] & mask) != (keys[i - 1] & mask);
+      if (!start && nsys > 1) start = system_of(i, nsys, off) != system_of(i - 1, nsys, off);
+    }
+    v[i] = start ? i : 0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_group_flags(int n, int ncrit,
+                                                        const int *__restrict__ run_start,
+                                                        int *__restrict__ flag) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    flag[i] = ((i - run_start[i]) % ncrit) == 0 ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_group_emit(int n, int nsys, const int *__restrict__ off,
+                                                       const int *__restrict__ flag,
+                                                       const int *__restrict__ slot,
+                                                       int cap, int2 *__restrict__ groups,
+                                                       int *__restrict__ group_sys,
+                                                       DevCounters *dc) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    if (!flag[i]) continue;
+    const int g = slot[i];
+    if (g >= cap) {
+      atomicOr(&dc->err, 2);
+      continue;
+    }
+    groups[g] = make_int2(i, 0);
+    group_sys[g] = nsys > 1 ? system_of(i, nsys, off) : 0;
+  }
+}
+
+// Groups tile [0, n) in order, so a group ends where the next begins.
+__global__ __launch_bounds__(kBlock) void k_group_sizes(int n, int cap, int2 *__restrict__ groups,
+                                                        DevCounters *dc) {
+  const int ng = min(dc->ngroups, cap);
+  for (int g = blockIdx.x * kBlock + threadIdx.x; g < ng; g += gridDim.x * kBlock) {
+    const int next = g + 1 < ng ? groups[g + 1].x : n;
+    groups[g].y = next - groups[g].x;
+  }
+}
+
+}  // namespace
+
+void launch_ph_keys(Ctx &c) {
+  hipLaunchKernelGGL(k_ph_keys, dim3(grid_for(c.n, kBlock, 8192)), dim3(kBlock), 0, c.stream,
+                     c.n, c.nsys, c.d_off, c.box, c.pos, c.ph_keys, c.perm);
+}
+
+void launch_gather_ph(Ctx &c) {
+  hipLaunchKernelGGL(k_gather_ph, dim3(grid_for(c.n, kBlock, 8192)), dim3(kBlock), 0, c.stream,
+                     c.n, c.perm, c.pos, c.id, c.pos_ph, c.id_ph);
+}
+
+void launch_groups(Ctx &c) {
+  const int ls = c.level_split < 1 ? 1 : c.level_split;
+  uint64_t mask = 0;  // top ls octal digits of the 63-bit key (code/makeGroups.cu:259-266)
+  for (int i = 0; i < 21; i++) {
+    mask <<= 3;
+    if (i < ls) mask |= 7;
+  }
+  const int g = grid_for(c.n, kBlock, 8192);
+  hipLaunchKernelGGL(k_run_starts, dim3(g), dim3(kBlock), 0, c.stream, c.n, c.nsys, c.d_off,
+                     c.ph_keys, mask, c.gtmp0);
+  scan_inclusive_max(c, c.gtmp0, c.gtmp1, c.n);
+  hipLaunchKernelGGL(k_group_flags, dim3(g), dim3(kBlock), 0, c.stream, c.n, c.ncrit, c.gtmp1,
+                     c.gtmp0);
+  scan_exclusive_sum(c, c.gtmp0, c.gtmp1, c.n, &c.dc->ngroups);
+  hipLaunchKernelGGL(k_group_emit, dim3(g), dim3(kBlock), 0, c.stream, c.n, c.nsys, c.d_off,
+                     c.gtmp0, c.gtmp1, c.cap_groups, c.groups, c.group_sys, c.dc);
+  hipLaunchKernelGGL(k_group_sizes, dim3(grid_for(c.cap_groups, kBlock, 4096)), dim3(kBlock), 0,
+                     c.stream, c.n, c.cap_groups, c.groups, c.dc);
+}
+
+}  // namespace bt

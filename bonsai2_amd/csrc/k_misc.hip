@@ -1,0 +1,192 @@
+// k_misc.hip -- moveParticles, computeEnergies and the FP64 direct sum.
+//
+// The reference's moveParticles and computeEnergies are stubs that only
+// print (code/moveParticles.cu:3-7, code/computeEnergies.cu:3-7).  Here:
+//   * kick:  v += a * dt/2, with a in PH order mapped to the tree-ordered
+//            velocities through perm (PH index -> tree index);
+//   * drift: x += v * dt (tree order);
+//   * energies: FP64 per-system sums K = 1/2 sum m v^2 and
+//     W = 1/2 sum m (phi + m/eps) -- the walk's phi includes the self term
+//     -m/eps of the P2P kernel with dr = 0 (code/computeForces.cu:64-72);
+//     partial sums are reduced in a fixed order (deterministic);
+//   * FP64 direct sum: the error oracle for the walk, sources staged through
+//     LDS, 1/sqrt by a float v_rsq seed refined with two Newton steps in FP64.
+#include <cfloat>
+
+#include "bt_device.h"
+#include "bt_internal.h"
+
+namespace bt {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kEnergyBlocks = 64;
+constexpr int kDirectSlices = 64;
+
+__global__ __launch_bounds__(kBlock) void k_kick(int n, const int *__restrict__ perm,
+                                                 const float4 *__restrict__ acc,
+                                                 float4 *__restrict__ vel, float dth) {
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+    const int t = perm[k];
+    const float4 a = acc[k];
+    float4 v = vel[t];
+    v.x += a.x * dth;
+    v.y += a.y * dth;
+    v.z += a.z * dth;
+    vel[t] = v;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_drift(int n, const float4 *__restrict__ vel,
+                                                  float4 *__restrict__ pos, float dt) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const float4 v = vel[i];
+    float4 p = pos[i];
+    p.x += v.x * dt;
+    p.y += v.y * dt;
+    p.z += v.z * dt;
+    pos[i] = p;
+  }
+}
+
+__device__ __forceinline__ double block_sum_d(double v, double *sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) t += sh[k];
+  __syncthreads();
+  return t;
+}
+
+// grid (kEnergyBlocks, nsys): partial[(r*B + b)*2 + {0,1}] = (K, W) partial sums
+__global__ __launch_bounds__(kBlock) void k_energy_partial(
+    const int *__restrict__ off, const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+    const float4 *__restrict__ pos_ph, const float4 *__restrict__ acc, double inv_eps,
+    double *__restrict__ part) {
+  __shared__ double sh[4];
+  const int r = blockIdx.y;
+  const int b = off[r], e = off[r + 1];
+  double k = 0.0, w = 0.0;
+  for (int i = b + blockIdx.x * kBlock + threadIdx.x; i < e; i += gridDim.x * kBlock) {
+    const float4 p = pos[i], v = vel[i];
+    const double m = p.w;
+    k += 0.5 * m * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
+    const double mp = pos_ph[i].w;
+    w += 0.5 * mp * ((double)acc[i].w + mp * inv_eps);
+  }
+  k = block_sum_d(k, sh);
+  w = block_sum_d(w, sh);
+  if (threadIdx.x == 0) {
+    part[2 * (r * gridDim.x + blockIdx.x)] = k;
+    part[2 * (r * gridDim.x + blockIdx.x) + 1] = w;
+  }
+}
+
+__global__ void k_energy_final(int nsys, int nb, const double *__restrict__ part,
+                               double *__restrict__ out) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nsys; r += gridDim.x * blockDim.x) {
+    double k = 0.0, w = 0.0;
+    for (int b = 0; b < nb; b++) {
+      k += part[2 * (r * nb + b)];
+      w += part[2 * (r * nb + b) + 1];
+    }
+    out[2 * r] = k;
+    out[2 * r + 1] = w;
+  }
+}
+
+__device__ __forceinline__ double rsqrt_fp64(double r2) {
+  const float f = (float)fmin(fmax(r2, (double)FLT_MIN), (double)FLT_MAX);
+  double y = (double)__builtin_amdgcn_rsqf(f);
+  y = y * (1.5 - 0.5 * r2 * y * y);
+  y = y * (1.5 - 0.5 * r2 * y * y);
+  y = y * (1.5 - 0.5 * r2 * y * y);
+  return y;
+}
+
+// grid (ceil(ns/256), kDirectSlices): each block sums a slice of the source
+// index range for 256 targets.  Sources outside a target's system are skipped.
+__global__ __launch_bounds__(kBlock) void k_direct(int ns, const int *__restrict__ idx, int n,
+                                                   int nsys, const int *__restrict__ off,
+                                                   const float4 *__restrict__ pos, double eps2,
+                                                   double4 *__restrict__ part) {
+  __shared__ float4 tile[kBlock];
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const bool active = t < ns;
+  const int k = active ? idx[t] : 0;
+  const int r = nsys > 1 ? system_of(k, nsys, off) : 0;
+  const int lo = off[r], hi = off[r + 1];
+  const float4 pt = pos[k];
+  const int chunk = (n + gridDim.y - 1) / gridDim.y;
+  const int s0 = blockIdx.y * chunk, s1 = min(n, s0 + chunk);
+  double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
+  for (int j0 = s0; j0 < s1; j0 += kBlock) {
+    const int j = j0 + threadIdx.x;
+    tile[threadIdx.x] = j < s1 ? pos[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    const int cnt = min(kBlock, s1 - j0);
+    for (int q = 0; q < cnt; q++) {
+      const int jj = j0 + q;
+      if (jj < lo || jj >= hi) continue;
+      const float4 p = tile[q];
+      const double dx = (double)p.x - pt.x, dy = (double)p.y - pt.y, dz = (double)p.z - pt.z;
+      const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+      const double ri = rsqrt_fp64(r2);
+      const double mr = (double)p.w * ri;
+      const double mr3 = mr * ri * ri;
+      ax += mr3 * dx; ay += mr3 * dy; az += mr3 * dz; ph -= mr;
+    }
+    __syncthreads();
+  }
+  if (active) part[(size_t)blockIdx.y * ns + t] = make_double4(ax, ay, az, ph);
+}
+
+__global__ __launch_bounds__(kBlock) void k_direct_reduce(int ns, int nslices,
+                                                          const double4 *__restrict__ part,
+                                                          double4 *__restrict__ out) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= ns) return;
+  double4 s = make_double4(0, 0, 0, 0);
+  for (int q = 0; q < nslices; q++) {
+    const double4 v = part[(size_t)q * ns + t];
+    s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+  }
+  out[t] = s;
+}
+
+}  // namespace
+
+void launch_kick(Ctx &c, float dth) {
+  hipLaunchKernelGGL(k_kick, dim3(grid_for(c.n, kBlock, 8192)), dim3(kBlock), 0, c.stream, c.n,
+                     c.perm, c.acc, c.vel, dth);
+}
+
+void launch_drift(Ctx &c, float dt) {
+  hipLaunchKernelGGL(k_drift, dim3(grid_for(c.n, kBlock, 8192)), dim3(kBlock), 0, c.stream, c.n,
+                     c.vel, c.pos, dt);
+}
+
+void launch_energies(Ctx &c) {
+  const double inv_eps = c.eps > 0.f ? 1.0 / (double)c.eps : 0.0;
+  double *part = c.d_energy + 2 * c.nsys;
+  hipLaunchKernelGGL(k_energy_partial, dim3(kEnergyBlocks, c.nsys), dim3(kBlock), 0, c.stream,
+                     c.d_off, c.pos, c.vel, c.pos_ph, c.acc, inv_eps, part);
+  hipLaunchKernelGGL(k_energy_final, dim3(grid_for(c.nsys, 64, 1024)), dim3(64), 0, c.stream,
+                     c.nsys, kEnergyBlocks, part, c.d_energy);
+}
+
+// d_out must hold ns * (kDirectSlices + 1) double4: the partials then the result.
+void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out) {
+  double4 *part = reinterpret_cast<double4 *>(d_out) + ns;
+  const double eps2 = (double)c.eps * (double)c.eps;
+  hipLaunchKernelGGL(k_direct, dim3((ns + kBlock - 1) / kBlock, kDirectSlices), dim3(kBlock), 0,
+                     c.stream, ns, d_idx, c.n, c.nsys, c.d_off, c.pos_ph, eps2, part);
+  hipLaunchKernelGGL(k_direct_reduce, dim3((ns + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream,
+                     ns, kDirectSlices, part, reinterpret_cast<double4 *>(d_out));
+}
+
+}  // namespace bt

@@ -1,0 +1,558 @@
+// k_tree.hip -- buildTree and computeMultipoles on CDNA4.
+//
+// The reference builds the octree by recursive dynamic parallelism
+// (code/buildTree.cu:170-584: every split cell's last block launches the
+// children's grid from the device).  HIP has no device-side launch, so the
+// build here is level-synchronous and key-based:
+//
+//   1. root box per system (code/buildTree.cu:80-166), bit-exact float ops;
+//   2. per particle, its geometric key: the 21 octant digits it takes down the
+//      same float ChildBox/Octant recursion the reference evaluates per cell
+//      (code/Particle4.h:129-148, code/buildTree.cu:233-247);
+//   3. a stable radix sort of the keys = the reference's scatter into child
+//      ranges in octant order (:444-471), within-leaf order canonical;
+//   4. cells, level by level: the children of a level-(L-1) cell are the runs
+//      of equal digit L inside its particle range; count > NLEAF -> node,
+//      0 < count <= NLEAF -> leaf (:394-395); children of one parent are
+//      [nodes in octant order][leaves in octant order] (:393-400, :549) and
+//      levels are contiguous, parents in order (the level sort, :932-951).
+//      Each level is three fixed-size launches (count, scan, emit) whose work
+//      sizes are read on the device, so the whole build can be graph-captured.
+//
+// Multipoles (code/computeMultipoles.cu:81-190) are computed bottom-up, one
+// launch per level from the deepest: a leaf sums its particles, a node
+// combines its children with the parallel-axis theorem.  Moments are carried
+// relative to each cell's exact geometric centre in FP32 (so the sums do not
+// cancel for small cells far from the origin); levels <= kFp64Levels combine
+// in FP64.  Outputs keep the reference layout (:180-188).
+#include "bt_device.h"
+#include "bt_internal.h"
+
+namespace bt {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---- root box -----------------------------------------------------------
+
+__device__ __forceinline__ void block_minmax(float3 &mn, float3 &mx, float3 *sh) {
+  mn.x = wave_min(mn.x); mn.y = wave_min(mn.y); mn.z = wave_min(mn.z);
+  mx.x = wave_max(mx.x); mx.y = wave_max(mx.y); mx.z = wave_max(mx.z);
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 0) { sh[2 * w] = mn; sh[2 * w + 1] = mx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
+      mn.x = fminf(mn.x, sh[2 * k].x); mn.y = fminf(mn.y, sh[2 * k].y);
+      mn.z = fminf(mn.z, sh[2 * k].z);
+      mx.x = fmaxf(mx.x, sh[2 * k + 1].x); mx.y = fmaxf(mx.y, sh[2 * k + 1].y);
+      mx.z = fmaxf(mx.z, sh[2 * k + 1].z);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bbox_partial(const float4 *__restrict__ pos,
+                                                         const int *__restrict__ off, int nb,
+                                                         float4 *__restrict__ part) {
+  __shared__ float3 sh[8];
+  const int r = blockIdx.y;
+  const int b = off[r], e = off[r + 1];
+  float3 mn = make_float3(1e10f, 1e10f, 1e10f), mx = make_float3(-1e10f, -1e10f, -1e10f);
+  for (int i = b + blockIdx.x * kBlock + threadIdx.x; i < e; i += nb * kBlock) {
+    const float4 p = pos[i];
+    mn.x = fminf(mn.x, p.x); mn.y = fminf(mn.y, p.y); mn.z = fminf(mn.z, p.z);
+    mx.x = fmaxf(mx.x, p.x); mx.y = fmaxf(mx.y, p.y); mx.z = fmaxf(mx.z, p.z);
+  }
+  block_minmax(mn, mx, sh);
+  if (threadIdx.x == 0) {
+    part[2 * (r * nb + blockIdx.x)] = make_float4(mn.x, mn.y, mn.z, 0.f);
+    part[2 * (r * nb + blockIdx.x) + 1] = make_float4(mx.x, mx.y, mx.z, 0.f);
+  }
+}
+
+// Box rules of code/buildTree.cu:145-161, same float operations.
+__device__ float4 box_from_extent(float3 mn, float3 mx) {
+#pragma clang fp contract(off)
+  const float cx = (mx.x + mn.x) * 0.5f, cy = (mx.y + mn.y) * 0.5f, cz = (mx.z + mn.z) * 0.5f;
+  const float hx = (mx.x - mn.x) * 0.5f, hy = (mx.y - mn.y) * 0.5f, hz = (mx.z - mn.z) * 0.5f;
+  const float h = fmaxf(hz, fmaxf(hy, hx));
+  float hs = 1.0f;
+  if (h > 0.0f) {
+    while (hs > h) hs *= 0.5f;
+    while (hs < h) hs *= 2.0f;
+  }
+  const float hq = hs / 1048576.0f;
+  const long long qx = (long long)(cx / hq), qy = (long long)(cy / hq), qz = (long long)(cz / hq);
+  return make_float4(hq * (float)qx, hq * (float)qy, hq * (float)qz, hs);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bbox_final(const float4 *__restrict__ part, int nb,
+                                                       float4 *__restrict__ box) {
+  __shared__ float3 sh[8];
+  const int r = blockIdx.x;
+  float3 mn = make_float3(1e10f, 1e10f, 1e10f), mx = make_float3(-1e10f, -1e10f, -1e10f);
+  for (int k = threadIdx.x; k < nb; k += kBlock) {
+    const float4 a = part[2 * (r * nb + k)], b = part[2 * (r * nb + k) + 1];
+    mn.x = fminf(mn.x, a.x); mn.y = fminf(mn.y, a.y); mn.z = fminf(mn.z, a.z);
+    mx.x = fmaxf(mx.x, b.x); mx.y = fmaxf(mx.y, b.y); mx.z = fmaxf(mx.z, b.z);
+  }
+  block_minmax(mn, mx, sh);
+  if (threadIdx.x == 0) box[r] = box_from_extent(mn, mx);
+}
+
+// ---- geometric keys -----------------------------------------------------
+
+// One ChildBox step (code/Particle4.h:138-148): the centre moves by hsize/2.
+__device__ __forceinline__ void child_step(float &cx, float &cy, float &cz, float &h, int o) {
+#pragma clang fp contract(off)
+  const float s = 0.5f * h;
+  cx = (o & 1) ? cx + s : cx - s;
+  cy = (o & 2) ? cy + s : cy - s;
+  cz = (o & 4) ? cz + s : cz - s;
+  h = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_geo_keys(int n, int nsys, const int *__restrict__ off,
+                                                     const float4 *__restrict__ box,
+                                                     const float4 *__restrict__ pos,
+                                                     uint64_t *__restrict__ keys,
+                                                     int *__restrict__ vals) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const int r = nsys > 1 ? system_of(i, nsys, off) : 0;
+    const float4 b = box[r];
+    const float4 p = pos[i];
+    float cx = b.x, cy = b.y, cz = b.z, h = b.w;
+    uint64_t k = 0;
+#pragma unroll
+    for (int l = 0; l < kMaxLevel; l++) {
+      // Octant (code/Particle4.h:129-136): bit set iff centre <= pos
+      const int o = (int)(cx <= p.x) | ((int)(cy <= p.y) << 1) | ((int)(cz <= p.z) << 2);
+      k = (k << 3) | (uint64_t)o;
+      child_step(cx, cy, cz, h, o);
+    }
+    keys[i] = k;
+    vals[i] = i;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather3(int n, const int *__restrict__ perm,
+                                                    const float4 *__restrict__ a,
+                                                    const float4 *__restrict__ b,
+                                                    const int *__restrict__ c,
+                                                    float4 *__restrict__ a2,
+                                                    float4 *__restrict__ b2,
+                                                    int *__restrict__ c2) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const int j = perm[i];
+    a2[i] = a[j];
+    b2[i] = b[j];
+    c2[i] = c[j];
+  }
+}
+
+// ---- level-synchronous cell emission ------------------------------------
+//
+// Parents of level L are the cells of level L-1 (the roots for L = 1).  Eight
+// lanes serve one parent; lane o finds the end of child octant o (first index
+// in the parent's range whose digit L exceeds o) by binary search on the
+// sorted keys.  Parents are split into kLevelBlocks contiguous chunks so the
+// block sums give each parent's first child slot in parent order.
+
+struct LevelArgs {
+  int L, nsys, nleaf, cap_cells, n;
+  const int *off;
+  const uint64_t *keys;
+  uint4 *cells;
+  int *bnd;          // 8 ints per parent: end of each child octant
+  int *blk;          // kLevelBlocks block sums -> offsets
+  int2 *root_cells;
+  DevCounters *dc;
+};
+
+__device__ __forceinline__ int level_parent_count(const LevelArgs &a) {
+  return a.L == 1 ? a.nsys : a.dc->lvl_count[a.L - 1];
+}
+
+__device__ __forceinline__ void parent_range(const LevelArgs &a, int p, int &b, int &e,
+                                             bool &splits) {
+  if (a.L == 1) {
+    b = a.off[p];
+    e = a.off[p + 1];
+    splits = e > b;  // the root always splits (code/buildTree.cu:702)
+  } else {
+    const uint4 c = a.cells[a.dc->lvl_start[a.L - 1] + p];
+    b = (int)c.z;
+    e = (int)c.w;
+    splits = !cell_leaf(c);
+  }
+}
+
+// Per-lane child boundaries of one parent; returns the child count of octant o.
+__device__ __forceinline__ int child_count(const LevelArgs &a, int p, bool active, int o,
+                                           int *end_out) {
+  int b = 0, e = 0;
+  bool splits = false;
+  if (active) parent_range(a, p, b, e, splits);
+  int end = b;
+  if (active && splits) {
+    const int shift = 63 - 3 * a.L;
+    // first index in [b, e) with digit > o
+    int lo = b, hi = e;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      const int d = (int)((a.keys[mid] >> shift) & 7ull);
+      if (d <= o) lo = mid + 1; else hi = mid;
+    }
+    end = lo;
+  }
+  const int beg_lane = __shfl_up(end, 1, 8);
+  const int beg = (o == 0) ? b : beg_lane;
+  *end_out = end;
+  return (active && splits) ? end - beg : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_level_count(LevelArgs a) {
+  __shared__ int shw[4];
+  const int np = level_parent_count(a);
+  const int chunk = (np + kLevelBlocks - 1) / kLevelBlocks;
+  const int p0 = blockIdx.x * chunk, p1 = min(np, p0 + chunk);
+  const int o = threadIdx.x & 7;
+  int sum = 0;
+  for (int pb = p0; pb < p1; pb += kBlock / 8) {
+    const int p = pb + (threadIdx.x >> 3);
+    const bool active = p < p1;
+    int end;
+    const int cnt = child_count(a, p, active, o, &end);
+    if (active) a.bnd[8 * p + o] = end;
+    sum += cnt > 0 ? 1 : 0;
+  }
+  sum = wave_sum(sum);
+  if (lane_id() == 0) shw[threadIdx.x >> 6] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) a.blk[blockIdx.x] = shw[0] + shw[1] + shw[2] + shw[3];
+}
+
+__global__ __launch_bounds__(kBlock) void k_level_scan(LevelArgs a) {
+  __shared__ int sh[kLevelBlocks];
+  for (int k = threadIdx.x; k < kLevelBlocks; k += kBlock) sh[k] = a.blk[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int k = 0; k < kLevelBlocks; k++) {
+      const int v = sh[k];
+      sh[k] = run;
+      run += v;
+    }
+    DevCounters *dc = a.dc;
+    const int start = a.L == 1 ? 0 : dc->lvl_start[a.L - 1] + dc->lvl_count[a.L - 1];
+    int total = run;
+    if (start + total > a.cap_cells) {
+      dc->err |= 1;
+      total = 0;
+    }
+    dc->lvl_start[a.L] = start;
+    dc->lvl_count[a.L] = total;
+    if (total > 0) dc->nlevels = a.L;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < kLevelBlocks; k += kBlock) a.blk[k] = sh[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_level_emit(LevelArgs a) {
+  __shared__ int shw[4];
+  __shared__ int sh_nodes, sh_leaves;
+  DevCounters *dc = a.dc;
+  const int total = dc->lvl_count[a.L];
+  if (total == 0) return;  // nothing at this level (or capacity exceeded)
+  if (threadIdx.x == 0) { sh_nodes = 0; sh_leaves = 0; }
+  const int np = level_parent_count(a);
+  const int chunk = (np + kLevelBlocks - 1) / kLevelBlocks;
+  const int p0 = blockIdx.x * chunk, p1 = min(np, p0 + chunk);
+  const int o = threadIdx.x & 7;
+  const int gsub = (threadIdx.x & 63) >> 3;  // 8-lane group within the wave
+  const int start = dc->lvl_start[a.L];
+  const int pstart = a.L == 1 ? 0 : dc->lvl_start[a.L - 1];
+  int run = a.blk[blockIdx.x];
+  int nodes = 0, leaves = 0, capped = 0;
+  __syncthreads();
+  for (int pb = p0; pb < p1; pb += kBlock / 8) {
+    const int p = pb + (threadIdx.x >> 3);
+    const bool active = p < p1;
+    int b = 0, e = 0;
+    bool splits = false;
+    if (active) parent_range(a, p, b, e, splits);
+    int end = b, beg = b;
+    if (active && splits) {
+      end = a.bnd[8 * p + o];
+      beg = (o == 0) ? b : a.bnd[8 * p + o - 1];
+    }
+    const int cnt = (active && splits) ? end - beg : 0;
+    const bool is_child = cnt > 0;
+    const bool is_node = is_child && cnt > a.nleaf && a.L < kMaxLevel;
+    // 8-bit masks of this parent's children / node children
+    const uint64_t mc = ballot64(is_child), mn = ballot64(is_node);
+    const int sh = gsub * 8;
+    const uint32_t gc = (uint32_t)((mc >> sh) & 0xFFull), gn = (uint32_t)((mn >> sh) & 0xFFull);
+    const int nchild = __popc(gc), nnode = __popc(gn);
+    // exclusive scan of nchild over the parents of this block, in order
+    const int per_parent = (o == 0) ? nchild : 0;
+    const int incl = wave_incl_scan(per_parent);
+    if (lane_id() == 63) shw[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    int wpre = 0;
+    for (int k = 0; k < (threadIdx.x >> 6); k++) wpre += shw[k];
+    const int tot = shw[0] + shw[1] + shw[2] + shw[3];
+    __syncthreads();
+    // value at the group's lane 0 (its exclusive prefix)
+    const int excl_lane0 = __shfl(incl - per_parent, (threadIdx.x & 63) & ~7, kWave);
+    const int first = start + run + wpre + excl_lane0;
+    run += tot;
+    if (is_child) {
+      const uint32_t below = (1u << o) - 1u;
+      const int rank = is_node ? __popc(gn & below) : nnode + __popc((gc & ~gn) & below);
+      const int ci = first + rank;
+      const uint32_t parent_idx = a.L == 1 ? 0u : (uint32_t)(pstart + p);
+      uint4 cell;
+      cell.x = parent_idx | ((uint32_t)a.L << 27);
+      cell.y = is_node ? 0u : 0xFFFFFFFFu;  // node's first/nchild set one level down
+      cell.z = (uint32_t)beg;
+      cell.w = (uint32_t)end;
+      a.cells[ci] = cell;
+      if (is_node) nodes++; else leaves++;
+      if (cnt > a.nleaf && !is_node) capped = 1;
+    }
+    if (active && splits && o == 0) {
+      if (a.L == 1) {
+        a.root_cells[p] = make_int2(first, nchild);
+      } else {
+        a.cells[pstart + p].y = nchild > 0 ? cell_pack_y(first, nchild) : 0xFFFFFFFFu;
+      }
+    }
+  }
+  nodes = wave_sum(nodes);
+  leaves = wave_sum(leaves);
+  if (lane_id() == 0) {
+    atomicAdd(&sh_nodes, nodes);
+    atomicAdd(&sh_leaves, leaves);
+  }
+  if (capped) atomicOr(&dc->flags, (int)BT_FLAG_DEPTH_CAP);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (sh_nodes) atomicAdd(&dc->nnodes, sh_nodes);
+    if (sh_leaves) atomicAdd(&dc->nleaves, sh_leaves);
+  }
+}
+
+__global__ void k_levels_init(DevCounters *dc) {
+  if (threadIdx.x < kLevels + 2) {
+    dc->lvl_start[threadIdx.x] = 0;
+    dc->lvl_count[threadIdx.x] = 0;
+  }
+  if (threadIdx.x == 0) {
+    dc->nnodes = dc->nleaves = dc->nlevels = dc->ncells = 0;
+    dc->flags = 0;
+    dc->err &= ~1;
+  }
+}
+
+__global__ void k_levels_finish(DevCounters *dc, int2 *level_beg) {
+  const int L = threadIdx.x;
+  if (L < kLevels) {
+    const int cnt = (L >= 1 && L <= kMaxLevel) ? dc->lvl_count[L] : 0;
+    level_beg[L] = cnt > 0 ? make_int2(dc->lvl_start[L], dc->lvl_start[L] + cnt - 1)
+                           : make_int2(0, 0);
+  }
+  if (L == 0) dc->ncells = dc->nnodes + dc->nleaves;
+}
+
+// ---- bottom-up multipoles -------------------------------------------------
+
+struct MpArgs {
+  int L, nsys;
+  float inv_theta;
+  const int *off;
+  const float4 *box;
+  const uint64_t *keys;   // sorted geometric keys (tree order)
+  const float4 *pos;      // tree order
+  const uint4 *cells;
+  float4 *scr;            // 6 per cell: (g,M) (d,-) (Qxx,Qyy,Qzz,Qxy) (Qxz,Qyz,-,-) bmin bmax
+  float4 *size, *mono, *q0;
+  float2 *q1, *ls;
+  const DevCounters *dc;
+};
+
+template <typename R>
+struct Moments {
+  R M, sx, sy, sz, xx, yy, zz, xy, xz, yz;
+};
+
+__device__ void write_cell(const MpArgs &a, int c, float gx, float gy, float gz, double M,
+                           double dx, double dy, double dz, double qxx, double qyy, double qzz,
+                           double qxy, double qxz, double qyz, float3 mn, float3 mx) {
+#pragma clang fp contract(off)
+  const float comx = (float)((double)gx + dx), comy = (float)((double)gy + dy),
+              comz = (float)((double)gz + dz);
+  const float bcx = (mx.x + mn.x) * 0.5f, bcy = (mx.y + mn.y) * 0.5f, bcz = (mx.z + mn.z) * 0.5f;
+  const float hx = (mx.x - mn.x) * 0.5f, hy = (mx.y - mn.y) * 0.5f, hz = (mx.z - mn.z) * 0.5f;
+  const float ex = bcx - comx, ey = bcy - comy, ez = bcz - comz;
+  const float s = sqrtf(ex * ex + ey * ey + ez * ez);
+  const float l = fmaxf(2.0f * fmaxf(hx, fmaxf(hy, hz)), 1.0e-6f);
+  const float op = l * a.inv_theta + s;
+  a.size[c] = make_float4(comx, comy, comz, op * op);
+  a.mono[c] = make_float4(comx, comy, comz, (float)M);
+  a.q0[c] = make_float4((float)qxx, (float)qyy, (float)qzz, (float)qxy);
+  a.q1[c] = make_float2((float)qxz, (float)qyz);
+  a.ls[c] = make_float2(l, s);
+  float4 *s6 = a.scr + 6 * (size_t)c;
+  s6[0] = make_float4(gx, gy, gz, (float)M);
+  s6[1] = make_float4((float)dx, (float)dy, (float)dz, 0.f);
+  s6[2] = make_float4((float)qxx, (float)qyy, (float)qzz, (float)qxy);
+  s6[3] = make_float4((float)qxz, (float)qyz, 0.f, 0.f);
+  s6[4] = make_float4(mn.x, mn.y, mn.z, 0.f);
+  s6[5] = make_float4(mx.x, mx.y, mx.z, 0.f);
+}
+
+template <typename R>
+__device__ void finish_moments(const MpArgs &a, int c, float gx, float gy, float gz,
+                               const Moments<R> &m, float3 mn, float3 mx) {
+  if (!(m.M > R(0))) {
+    write_cell(a, c, gx, gy, gz, 0.0, 0, 0, 0, 0, 0, 0, 0, 0, 0, mn, mx);
+    return;
+  }
+  const R im = R(1) / m.M;
+  const R dx = m.sx * im, dy = m.sy * im, dz = m.sz * im;
+  write_cell(a, c, gx, gy, gz, (double)m.M, (double)dx, (double)dy, (double)dz,
+             (double)(m.xx * im - dx * dx), (double)(m.yy * im - dy * dy),
+             (double)(m.zz * im - dz * dz), (double)(m.xy * im - dx * dy),
+             (double)(m.xz * im - dx * dz), (double)(m.yz * im - dy * dz), mn, mx);
+}
+
+template <typename R>
+__device__ void node_moments(const MpArgs &a, int c, const uint4 cd, float gx, float gy,
+                             float gz) {
+  Moments<R> m{};
+  float3 mn = make_float3(1e10f, 1e10f, 1e10f), mx = make_float3(-1e10f, -1e10f, -1e10f);
+  const int f = cell_first(cd), nc = cell_nchild(cd);
+  for (int k = 0; k < nc; k++) {
+    const float4 *s6 = a.scr + 6 * (size_t)(f + k);
+    const float4 g = s6[0], d = s6[1], qa = s6[2], qb = s6[3], bmn = s6[4], bmx = s6[5];
+    // child com relative to this cell's centre: exact centre offset + child offset
+    const R ox = R(g.x - gx) + R(d.x), oy = R(g.y - gy) + R(d.y), oz = R(g.z - gz) + R(d.z);
+    const R Mc = R(g.w);
+    m.M += Mc;
+    m.sx += Mc * ox; m.sy += Mc * oy; m.sz += Mc * oz;
+    m.xx += Mc * (R(qa.x) + ox * ox); m.yy += Mc * (R(qa.y) + oy * oy);
+    m.zz += Mc * (R(qa.z) + oz * oz); m.xy += Mc * (R(qa.w) + ox * oy);
+    m.xz += Mc * (R(qb.x) + ox * oz); m.yz += Mc * (R(qb.y) + oy * oz);
+    mn.x = fminf(mn.x, bmn.x); mn.y = fminf(mn.y, bmn.y); mn.z = fminf(mn.z, bmn.z);
+    mx.x = fmaxf(mx.x, bmx.x); mx.y = fmaxf(mx.y, bmx.y); mx.z = fmaxf(mx.z, bmx.z);
+  }
+  finish_moments<R>(a, c, gx, gy, gz, m, mn, mx);
+}
+
+__global__ __launch_bounds__(kBlock) void k_multipole_level(MpArgs a) {
+  const int cnt = a.dc->lvl_count[a.L];
+  const int c0 = a.dc->lvl_start[a.L];
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < cnt; t += gridDim.x * kBlock) {
+    const int c = c0 + t;
+    const uint4 cd = a.cells[c];
+    const int pb = (int)cd.z, pe = (int)cd.w;
+    // geometric centre of the cell: replay the ChildBox steps of its key prefix
+    const int r = a.nsys > 1 ? system_of(pb, a.nsys, a.off) : 0;
+    const float4 bx = a.box[r];
+    float gx = bx.x, gy = bx.y, gz = bx.z, h = bx.w;
+    const uint64_t key = a.keys[pb];
+    for (int l = 1; l <= a.L; l++) child_step(gx, gy, gz, h, (int)((key >> (63 - 3 * l)) & 7ull));
+    if (cell_leaf(cd)) {
+      Moments<float> m{};
+      float3 mn = make_float3(1e10f, 1e10f, 1e10f), mx = make_float3(-1e10f, -1e10f, -1e10f);
+      for (int i = pb; i < pe; i++) {
+        const float4 p = a.pos[i];
+        mn.x = fminf(mn.x, p.x); mn.y = fminf(mn.y, p.y); mn.z = fminf(mn.z, p.z);
+        mx.x = fmaxf(mx.x, p.x); mx.y = fmaxf(mx.y, p.y); mx.z = fmaxf(mx.z, p.z);
+        const float dx = p.x - gx, dy = p.y - gy, dz = p.z - gz, w = p.w;
+        m.M += w;
+        m.sx += w * dx; m.sy += w * dy; m.sz += w * dz;
+        m.xx += w * dx * dx; m.yy += w * dy * dy; m.zz += w * dz * dz;
+        m.xy += w * dx * dy; m.xz += w * dx * dz; m.yz += w * dy * dz;
+      }
+      finish_moments<float>(a, c, gx, gy, gz, m, mn, mx);
+    } else if (a.L <= kFp64Levels) {
+      node_moments<double>(a, c, cd, gx, gy, gz);
+    } else {
+      node_moments<float>(a, c, cd, gx, gy, gz);
+    }
+  }
+}
+
+}  // namespace
+
+void launch_bbox(Ctx &c) {
+  const int nb = c.bbox_blocks;
+  hipLaunchKernelGGL(k_bbox_partial, dim3(nb, c.nsys), dim3(kBlock), 0, c.stream, c.pos, c.d_off,
+                     nb, c.bbox_part);
+  hipLaunchKernelGGL(k_bbox_final, dim3(c.nsys), dim3(kBlock), 0, c.stream, c.bbox_part, nb,
+                     c.box);
+}
+
+void launch_geo_keys(Ctx &c) {
+  hipLaunchKernelGGL(k_geo_keys, dim3(grid_for(c.n, kBlock, 8192)), dim3(kBlock), 0, c.stream,
+                     c.n, c.nsys, c.d_off, c.box, c.pos, c.geo_keys, c.valA);
+}
+
+void launch_gather_tree(Ctx &c) {
+  hipLaunchKernelGGL(k_gather3, dim3(grid_for(c.n, kBlock, 8192)), dim3(kBlock), 0, c.stream,
+                     c.n, c.valA, c.pos, c.vel, c.id, c.pos2, c.vel2, c.id2);
+  std::swap(c.pos, c.pos2);
+  std::swap(c.vel, c.vel2);
+  std::swap(c.id, c.id2);
+}
+
+void launch_levels(Ctx &c) {
+  hipLaunchKernelGGL(k_levels_init, dim3(1), dim3(64), 0, c.stream, c.dc);
+  LevelArgs a;
+  a.nsys = c.nsys;
+  a.nleaf = c.nleaf;
+  a.cap_cells = c.cap_cells;
+  a.n = c.n;
+  a.off = c.d_off;
+  a.keys = c.geo_keys;
+  a.cells = c.cells;
+  a.bnd = c.child_bnd;
+  a.blk = c.lvl_blk;
+  a.root_cells = c.root_cells;
+  a.dc = c.dc;
+  for (int L = 1; L <= kMaxLevel; L++) {
+    a.L = L;
+    hipLaunchKernelGGL(k_level_count, dim3(kLevelBlocks), dim3(kBlock), 0, c.stream, a);
+    hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(kBlock), 0, c.stream, a);
+    hipLaunchKernelGGL(k_level_emit, dim3(kLevelBlocks), dim3(kBlock), 0, c.stream, a);
+  }
+  hipLaunchKernelGGL(k_levels_finish, dim3(1), dim3(64), 0, c.stream, c.dc, c.level_beg);
+}
+
+void launch_multipoles(Ctx &c) {
+  MpArgs a;
+  a.nsys = c.nsys;
+  a.inv_theta = (float)(1.0 / (double)c.theta);
+  a.off = c.d_off;
+  a.box = c.box;
+  a.keys = c.geo_keys;
+  a.pos = c.pos;
+  a.cells = c.cells;
+  a.scr = c.cscr;
+  a.size = c.csize;
+  a.mono = c.cmono;
+  a.q0 = c.cq0;
+  a.q1 = c.cq1;
+  a.ls = c.cls;
+  a.dc = c.dc;
+  const int grid = grid_for(c.cap_cells, kBlock, 2048);
+  for (int L = kMaxLevel; L >= 1; L--) {
+    a.L = L;
+    hipLaunchKernelGGL(k_multipole_level, dim3(grid), dim3(kBlock), 0, c.stream, a);
+  }
+}
+
+}  // namespace bt

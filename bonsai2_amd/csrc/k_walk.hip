@@ -1,0 +1,424 @@
+// k_walk.hip -- computeForces on CDNA4 (reference code/computeForces.cu).
+//
+// One wave64 per group of <= 64 i-particles (one per lane), pulled from a
+// device-wide atomic queue by a persistent grid (the reference's
+// retired_groupCount loop, :514-522).  Per group:
+//   * the group box is the min/max of its particles (:539-555);
+//   * the walk starts at the group's level-1 cells (:259-263) and keeps a
+//     per-wave LIFO of cells in global memory; each step pops up to 64 cells,
+//     one per lane, and applies the reference's rules: open iff
+//     ds^2 < cellOp^2 (:30-54) or the cell holds < 3 particles (:284-285);
+//     opened nodes push their children, unopened cells join the P2C list,
+//     opened leaves put their particles on the P2P list (:293-430);
+//   * lists are kept in LDS, compacted with ballots and prefix sums; every 64
+//     entries the wave stages the j-data (positions relative to the group
+//     centre) in LDS and all 64 lanes sweep it with broadcast reads;
+//   * each batch is summed in FP32 and folded into Kahan-compensated
+//     accumulators.
+// The interaction set equals the reference's (the order of traversal does
+// not change which cells are opened); FP32 sums run in a different order.
+// P2P is code/computeForces.cu:58-78 (20 flops), P2C :83-128 (64 flops),
+// rearranged algebraically with the cell's half trace precomputed.
+#include "bt_device.h"
+#include "bt_internal.h"
+
+namespace bt {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kMaxWalkSteps = 1 << 24;
+
+struct WalkArgs {
+  int n, nsys;
+  float eps2;
+  const int2 *groups;
+  const int *group_sys;
+  const int2 *root_cells;
+  const float4 *pos_ph;
+  const float4 *pos_tree;
+  const uint4 *cells;
+  const float4 *csize;
+  const float4 *cmono;
+  const float4 *cq0;
+  const float2 *cq1;
+  float4 *acc;
+  int2 *gcounts;
+  double *gpot;
+  int *stack;
+  int stack_ints;
+  DevCounters *dc;
+};
+
+struct WaveShared {
+  float4 jA[kWave];   // P2C: (com - gc, M)         P2P: (x - gc, m)
+  float4 jB[kWave];   // P2C: (Qxx, Qyy, Qzz, Qxy)
+  float4 jC[kWave];   // P2C: (Qxz, Qyz, trQ/2, 0)
+  int alist[2 * kWave];
+  int dlist[2 * kWave];
+  int lpre[kWave];
+  int lbeg[kWave];
+};
+
+// Improved Barnes-Hut opening test, code/computeForces.cu:30-54 (same float ops).
+__device__ __forceinline__ bool open_cell(const float4 cs, float gcx, float gcy, float gcz,
+                                          float gsx, float gsy, float gsz) {
+#pragma clang fp contract(off)
+  float dx = fabsf(gcx - cs.x) - gsx;
+  float dy = fabsf(gcy - cs.y) - gsy;
+  float dz = fabsf(gcz - cs.z) - gsz;
+  dx += fabsf(dx); dx *= 0.5f;
+  dy += fabsf(dy); dy *= 0.5f;
+  dz += fabsf(dz); dz *= 0.5f;
+  const float ds2 = dx * dx + dy * dy + dz * dz;
+  return ds2 < fabsf(cs.w);
+}
+
+struct Acc4 {
+  float x, y, z, w;
+};
+
+template <bool KAHAN>
+__device__ __forceinline__ void fold(Acc4 &s, Acc4 &c, const Acc4 &b) {
+  if (KAHAN) {
+    float y, t;
+    y = b.x - c.x; t = s.x + y; c.x = (t - s.x) - y; s.x = t;
+    y = b.y - c.y; t = s.y + y; c.y = (t - s.y) - y; s.y = t;
+    y = b.z - c.z; t = s.z + y; c.z = (t - s.z) - y; s.z = t;
+    y = b.w - c.w; t = s.w + y; c.w = (t - s.w) - y; s.w = t;
+  } else {
+    s.x += b.x; s.y += b.y; s.z += b.z; s.w += b.w;
+  }
+}
+
+// Monopole + quadrupole of one cell (code/computeForces.cu:83-128):
+//   phi -= M/r - (trQ/2) M/r^3 + 1.5 (dr.Q.dr) M/r^5
+//   a   += (-M/r^3 + 1.5 trQ M/r^5 - 7.5 (dr.Q.dr) M/r^7) dr + 3 M/r^5 Q.dr
+__device__ __forceinline__ void p2c(Acc4 &b, float xi, float yi, float zi, const float4 A,
+                                    const float4 B, const float4 C, float eps2) {
+  const float dx = xi - A.x, dy = yi - A.y, dz = zi - A.z;
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  const float rinv2 = rinv * rinv;
+  const float mr1 = A.w * rinv;
+  const float mr3 = rinv2 * mr1;
+  const float mr5 = rinv2 * mr3;
+  const float mr7 = rinv2 * mr5;
+  const float qx = fmaf(B.x, dx, fmaf(B.w, dy, C.x * dz));
+  const float qy = fmaf(B.w, dx, fmaf(B.y, dy, C.y * dz));
+  const float qz = fmaf(C.x, dx, fmaf(C.y, dy, B.z * dz));
+  const float qrr = fmaf(qx, dx, fmaf(qy, dy, qz * dz));
+  const float hq = C.z;
+  b.w -= fmaf(1.5f * qrr, mr5, fmaf(-hq, mr3, mr1));
+  const float cc = fmaf(-7.5f * qrr, mr7, fmaf(3.0f * hq, mr5, -mr3));
+  const float d2 = 3.0f * mr5;
+  b.x = fmaf(cc, dx, fmaf(d2, qx, b.x));
+  b.y = fmaf(cc, dy, fmaf(d2, qy, b.y));
+  b.z = fmaf(cc, dz, fmaf(d2, qz, b.z));
+}
+
+// Particle-particle, code/computeForces.cu:58-78.
+__device__ __forceinline__ void p2p(Acc4 &b, float xi, float yi, float zi, const float4 A,
+                                    float eps2) {
+  const float dx = A.x - xi, dy = A.y - yi, dz = A.z - zi;
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  const float mr1 = A.w * rinv;
+  const float mr3 = mr1 * rinv * rinv;
+  b.w -= mr1;
+  b.x = fmaf(mr3, dx, b.x);
+  b.y = fmaf(mr3, dy, b.y);
+  b.z = fmaf(mr3, dz, b.z);
+}
+
+template <bool KAHAN>
+__device__ __forceinline__ void eval_p2c(const WalkArgs &a, WaveShared &S, int cnt, float gcx,
+                                         float gcy, float gcz, float xi, float yi, float zi,
+                                         Acc4 &s, Acc4 &c) {
+  const int lane = lane_id();
+  if (lane < cnt) {
+    const int ci = S.alist[lane];
+    const float4 m = a.cmono[ci];
+    const float4 q0 = a.cq0[ci];
+    const float2 q1 = a.cq1[ci];
+    S.jA[lane] = make_float4(m.x - gcx, m.y - gcy, m.z - gcz, m.w);
+    S.jB[lane] = q0;
+    S.jC[lane] = make_float4(q1.x, q1.y, 0.5f * (q0.x + q0.y + q0.z), 0.0f);
+  }
+  wave_lds_sync();
+  Acc4 b = {0.f, 0.f, 0.f, 0.f};
+  if (cnt == kWave) {
+#pragma unroll 8
+    for (int j = 0; j < kWave; j++) p2c(b, xi, yi, zi, S.jA[j], S.jB[j], S.jC[j], a.eps2);
+  } else {
+#pragma unroll 2
+    for (int j = 0; j < cnt; j++) p2c(b, xi, yi, zi, S.jA[j], S.jB[j], S.jC[j], a.eps2);
+  }
+  fold<KAHAN>(s, c, b);
+  wave_lds_sync();
+}
+
+template <bool KAHAN>
+__device__ __forceinline__ void eval_p2p(const WalkArgs &a, WaveShared &S, int cnt, float gcx,
+                                         float gcy, float gcz, float xi, float yi, float zi,
+                                         Acc4 &s, Acc4 &c) {
+  const int lane = lane_id();
+  if (lane < cnt) {
+    const float4 p = a.pos_tree[S.dlist[lane]];
+    S.jA[lane] = make_float4(p.x - gcx, p.y - gcy, p.z - gcz, p.w);
+  }
+  wave_lds_sync();
+  Acc4 b = {0.f, 0.f, 0.f, 0.f};
+  if (cnt == kWave) {
+#pragma unroll 8
+    for (int j = 0; j < kWave; j++) p2p(b, xi, yi, zi, S.jA[j], a.eps2);
+  } else {
+#pragma unroll 2
+    for (int j = 0; j < cnt; j++) p2p(b, xi, yi, zi, S.jA[j], a.eps2);
+  }
+  fold<KAHAN>(s, c, b);
+  wave_lds_sync();
+}
+
+template <bool KAHAN>
+__global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
+  __shared__ WaveShared sh[kWavesPerBlock];
+  const int lane = lane_id();
+  const int wv = threadIdx.x >> 6;
+  WaveShared &S = sh[wv];
+  int *stk = a.stack + (size_t)(blockIdx.x * kWavesPerBlock + wv) * (size_t)a.stack_ints;
+  const int ngroups = a.dc->ngroups;
+  // a failed build (cell or group capacity) leaves placeholder links: do not walk it
+  if (a.dc->err & 3) return;
+
+  for (;;) {
+    int g = 0;
+    if (lane == 0) g = atomicAdd(&a.dc->queue, 1);
+    g = wave_uniform(__shfl(g, 0, kWave));
+    if (g >= ngroups) break;
+    const int2 grp = a.groups[g];
+    const int pbeg = grp.x, np = max(grp.y, 1);
+    const int sys = a.nsys > 1 ? a.group_sys[g] : 0;
+    const bool valid = lane < np;
+    const float4 pi = a.pos_ph[pbeg + min(lane, np - 1)];
+
+    // group box (code/computeForces.cu:539-555)
+    const float mnx = wave_min(pi.x), mny = wave_min(pi.y), mnz = wave_min(pi.z);
+    const float mxx = wave_max(pi.x), mxy = wave_max(pi.y), mxz = wave_max(pi.z);
+    float gcx, gcy, gcz, gsx, gsy, gsz;
+    {
+#pragma clang fp contract(off)
+      gcx = 0.5f * (mxx + mnx); gcy = 0.5f * (mxy + mny); gcz = 0.5f * (mxz + mnz);
+      gsx = 0.5f * (mxx - mnx); gsy = 0.5f * (mxy - mny); gsz = 0.5f * (mxz - mnz);
+    }
+    // i-particle relative to the group centre
+    const float xi = pi.x - gcx, yi = pi.y - gcy, zi = pi.z - gcz;
+
+    Acc4 s = {0.f, 0.f, 0.f, 0.f}, cmp = {0.f, 0.f, 0.f, 0.f};
+    const int2 rc = a.root_cells[sys];
+    if (lane < rc.y) stk[lane] = rc.x + lane;
+    int top = rc.y;
+    int na = 0, nd = 0;          // pending list lengths
+    int n_app = 0, n_dir = 0;    // interactions evaluated (stats)
+    bool overflow = false;
+    int steps = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+
+    while (top > 0) {
+      // every cell is visited at most once per group; a longer walk means a
+      // corrupt tree, so stop rather than loop
+      if (++steps > kMaxWalkSteps) {
+        overflow = true;
+        break;
+      }
+      const int take = min(kWave, top);
+      top -= take;
+      const bool use = lane < take;
+      int ci = 0;
+      uint4 cd = make_uint4(0, 0, 0, 0);
+      float4 cs = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (use) {
+        ci = stk[top + lane];
+        cd = a.cells[ci];
+        cs = a.csize[ci];
+      }
+      bool split = false, leaf = false;
+      if (use) {
+        split = open_cell(cs, gcx, gcy, gcz, gsx, gsy, gsz) || ((int)cd.w - (int)cd.z < 3);
+        leaf = cell_leaf(cd);
+      }
+      const bool push = use && split && !leaf;
+      const bool approx = use && !split;
+      const bool direct = use && split && leaf;
+
+      // children of opened nodes go on the stack
+      const int nch = push ? cell_nchild(cd) : 0;
+      const int chi = wave_incl_scan(nch);
+      const int ctot = __shfl(chi, kWave - 1, kWave);
+      if (top + ctot > a.stack_ints) {
+        overflow = true;
+        break;
+      }
+      if (push) {
+        const int f = cell_first(cd);
+        const int base = top + chi - nch;
+        for (int k = 0; k < nch; k++) stk[base + k] = f + k;
+      }
+      top += ctot;
+
+      // approximate list (P2C)
+      const uint64_t am = ballot64(approx);
+      if (approx) S.alist[na + __popcll(am & lanemask_lt())] = ci;
+      na += __popcll(am);
+
+      // direct list (P2P): expand the opened leaves' particle ranges
+      const uint64_t dm = ballot64(direct);
+      const int ndl = __popcll(dm);
+      if (ndl > 0) {
+        const int lsz = direct ? (int)cd.w - (int)cd.z : 0;
+        const int li = wave_incl_scan(lsz);
+        const int ltot = __shfl(li, kWave - 1, kWave);
+        if (direct) {
+          const int r = __popcll(dm & lanemask_lt());
+          S.lpre[r] = li - lsz;
+          S.lbeg[r] = (int)cd.z;
+        }
+        wave_lds_sync();
+        for (int k0 = 0; k0 < ltot; k0 += kWave) {
+          const int k = k0 + lane;
+          if (k < ltot) {
+            int lo = 0, hi = ndl - 1;  // last leaf with lpre <= k
+            while (lo < hi) {
+              const int mid = (lo + hi + 1) >> 1;
+              if (S.lpre[mid] <= k) lo = mid; else hi = mid - 1;
+            }
+            S.dlist[nd + lane] = S.lbeg[lo] + (k - S.lpre[lo]);
+          }
+          nd += min(kWave, ltot - k0);
+          wave_lds_sync();
+          if (nd >= kWave) {
+            eval_p2p<KAHAN>(a, S, kWave, gcx, gcy, gcz, xi, yi, zi, s, cmp);
+            n_dir += kWave;
+            nd -= kWave;
+            if (lane < nd) S.dlist[lane] = S.dlist[kWave + lane];
+            wave_lds_sync();
+          }
+        }
+      }
+      wave_lds_sync();
+      if (na >= kWave) {
+        eval_p2c<KAHAN>(a, S, kWave, gcx, gcy, gcz, xi, yi, zi, s, cmp);
+        n_app += kWave;
+        na -= kWave;
+        if (lane < na) S.alist[lane] = S.alist[kWave + lane];
+        wave_lds_sync();
+      }
+      // make this step's stack pushes visible to the next step's pops
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    if (overflow) {
+      if (lane == 0) atomicOr(&a.dc->err, 4);
+      na = 0;
+      nd = 0;
+    }
+    if (na > 0) {
+      eval_p2c<KAHAN>(a, S, na, gcx, gcy, gcz, xi, yi, zi, s, cmp);
+      n_app += na;
+    }
+    if (nd > 0) {
+      eval_p2p<KAHAN>(a, S, nd, gcx, gcy, gcz, xi, yi, zi, s, cmp);
+      n_dir += nd;
+    }
+    if (valid) a.acc[pbeg + lane] = make_float4(s.x, s.y, s.z, s.w);
+    const double gp = wave_sum(valid ? (double)pi.w * (double)s.w : 0.0);
+    if (lane == 0) {
+      a.gcounts[g] = overflow ? make_int2(-1, -1) : make_int2(n_app, n_dir);
+      a.gpot[g] = gp;
+    }
+  }
+}
+
+// Interaction statistics (code/computeForces.cu:567-609, :701-715): per valid
+// i-particle its group's counts; {direct_sum, direct_max, approx_sum,
+// approx_max} and 1/2 sum m phi.  One block, deterministic order.
+__global__ __launch_bounds__(1024) void k_walk_stats(const int2 *__restrict__ groups,
+                                                     const int2 *__restrict__ gcounts,
+                                                     const double *__restrict__ gpot,
+                                                     const DevCounters *dc,
+                                                     unsigned long long *out, double *pot) {
+  __shared__ unsigned long long s_ds[16], s_as[16];
+  __shared__ unsigned int s_dm[16], s_am[16];
+  __shared__ double s_p[16];
+  const int ng = dc->ngroups;
+  unsigned long long ds = 0, as = 0;
+  unsigned int dm = 0, am = 0;
+  double p = 0.0;
+  for (int g = threadIdx.x; g < ng; g += blockDim.x) {
+    const int2 c = gcounts[g];
+    const int np = groups[g].y;
+    ds += (unsigned long long)c.y * (unsigned long long)np;
+    as += (unsigned long long)c.x * (unsigned long long)np;
+    dm = max(dm, (unsigned int)c.y);
+    am = max(am, (unsigned int)c.x);
+    p += gpot[g];
+  }
+  ds = wave_sum(ds); as = wave_sum(as); p = wave_sum(p);
+  for (int m = 32; m >= 1; m >>= 1) {
+    dm = max(dm, (unsigned int)__shfl_xor((int)dm, m, kWave));
+    am = max(am, (unsigned int)__shfl_xor((int)am, m, kWave));
+  }
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 0) { s_ds[w] = ds; s_as[w] = as; s_dm[w] = dm; s_am[w] = am; s_p[w] = p; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
+      ds += s_ds[k]; as += s_as[k]; dm = max(dm, s_dm[k]); am = max(am, s_am[k]); p += s_p[k];
+    }
+    out[0] = ds; out[1] = dm; out[2] = as; out[3] = am;
+    *pot = 0.5 * p;
+  }
+}
+
+__global__ void k_walk_reset(DevCounters *dc) {
+  if (threadIdx.x == 0) {
+    dc->queue = 0;
+    dc->err &= ~4;
+  }
+}
+
+}  // namespace
+
+void launch_walk(Ctx &c) {
+  WalkArgs a;
+  a.n = c.n;
+  a.nsys = c.nsys;
+  a.eps2 = c.eps * c.eps;
+  a.groups = c.groups;
+  a.group_sys = c.group_sys;
+  a.root_cells = c.root_cells;
+  a.pos_ph = c.pos_ph;
+  a.pos_tree = c.pos;
+  a.cells = c.cells;
+  a.csize = c.csize;
+  a.cmono = c.cmono;
+  a.cq0 = c.cq0;
+  a.cq1 = c.cq1;
+  a.acc = c.acc;
+  a.gcounts = c.gcounts;
+  a.gpot = c.gpot;
+  a.stack = c.stack;
+  a.stack_ints = c.stack_ints;
+  a.dc = c.dc;
+  hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, c.stream, c.dc);
+  if (c.kahan)
+    hipLaunchKernelGGL(k_walk<true>, dim3(c.walk_blocks), dim3(kBlock), 0, c.stream, a);
+  else
+    hipLaunchKernelGGL(k_walk<false>, dim3(c.walk_blocks), dim3(kBlock), 0, c.stream, a);
+  hipLaunchKernelGGL(k_walk_stats, dim3(1), dim3(1024), 0, c.stream, c.groups, c.gcounts, c.gpot,
+                     c.dc, c.d_wstats, c.d_wpot);
+}
+
+}  // namespace bt

@@ -1,0 +1,181 @@
+/* bonsai2_amd.h -- C ABI of the MI355X Barnes-Hut treecode (libbonsai2_amd.so).
+ *
+ * The reference (treecode/Bonsai2) has no C ABI: its boundary is the C++
+ * class template Treecode<real_t> (code/Treecode.h:123-223), instantiated for
+ * float only (code/TreecodeInstances.h:3).  Each entry point below replaces
+ * one member of that class; include/Treecode.h rebuilds the class on top of
+ * these functions so that reference callers compile unchanged.
+ *
+ * Conventions
+ *   - Every call returns 0 on success or a negative BT_ERR_* code; the text
+ *     of the last error is available from bt_last_error().
+ *   - All state lives in the context (no process-global device symbols, unlike
+ *     code/buildTree.cu:23-36 and code/computeForces.cu:465-473).  One context
+ *     is used by one host thread at a time.
+ *   - Particle arrays are 4 floats per particle: pos = (x, y, z, mass),
+ *     vel = (vx, vy, vz, mass) as the reference driver packs them
+ *     (code/treecode.cu:73-86).  The caller owns host buffers; the context
+ *     owns device buffers.
+ *   - Orders: BT_ORDER_INPUT = the order given to bt_set_particles;
+ *     BT_ORDER_TREE = octree order after bt_build_tree (reference d_ptclPos);
+ *     BT_ORDER_PH = Peano-Hilbert order after bt_make_groups (reference
+ *     d_ptclPos_tmp / d_ptclAcc).
+ *   - Work is queued on the context's stream; calls that return host data
+ *     synchronise that stream.
+ */
+#ifndef BONSAI2_AMD_H
+#define BONSAI2_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bt_ctx bt_ctx;
+
+enum {
+  BT_OK = 0,
+  BT_ERR_ARG = -1,            /* bad argument (NLEAF, NCRIT, sizes, order) */
+  BT_ERR_HIP = -2,            /* HIP runtime error */
+  BT_ERR_STATE = -3,          /* call order violated (e.g. forces before groups) */
+  BT_ERR_CAPACITY = -4,       /* cell or group capacity exceeded */
+  BT_ERR_STACK = -5,          /* walk stack overflow (reference: :302-303, :565) */
+  BT_ERR_NODEVICE = -6,       /* no HIP device */
+  BT_ERR_IO = -7              /* tipsy file error */
+};
+
+enum { BT_ORDER_INPUT = 0, BT_ORDER_TREE = 1, BT_ORDER_PH = 2 };
+
+/* Build flags returned by bt_get_counts()[7]. */
+enum { BT_FLAG_DEPTH_CAP = 1 };
+
+/* ---- lifetime: Treecode::Treecode / ~Treecode / alloc ---------------- */
+
+/* Treecode(eps, theta, ncrit), code/Treecode.h:164-173. */
+int bt_create(bt_ctx **ctx, float eps, float theta, int ncrit);
+/* ~Treecode (RAII cuda_mem frees, code/cudamem.h:12). */
+int bt_destroy(bt_ctx *ctx);
+/* Treecode::alloc(n), code/Treecode.h:175-198.  nsys >= 1 independent
+ * systems ("forest"); offsets[0..nsys] partition [0, n) (null when nsys=1).
+ * A later call with a larger n or different systems reallocates. */
+int bt_alloc(bt_ctx *ctx, int n, int nsys, const int *offsets);
+/* Queue all work on this stream (default: the context's own stream). */
+int bt_set_stream(bt_ctx *ctx, void *hip_stream);
+/* eps/theta are public members in the reference (code/Treecode.h:133). */
+int bt_set_params(bt_ctx *ctx, float eps, float theta);
+const char *bt_last_error(const bt_ctx *ctx);
+
+/* ---- particles: Treecode::ptcl_h2d / ptcl_d2h --------------------------- */
+
+/* ptcl_h2d(), code/Treecode.h:207-214: copies n particles (input order). */
+int bt_set_particles(bt_ctx *ctx, const float *pos4, const float *vel4);
+/* Same, from device pointers (no host staging). */
+int bt_set_particles_device(bt_ctx *ctx, const void *d_pos4, const void *d_vel4);
+/* ptcl_d2h(), code/Treecode.h:200-204, in the requested order. */
+int bt_get_particles(bt_ctx *ctx, float *pos4, float *vel4, int order);
+
+/* ---- the gravity step -------------------------------------------------- */
+
+/* buildTree(nLeaf), code/buildTree.cu:845-975.  nleaf in {16,24,32,48,64}
+ * (:849).  Leaves the particles in tree order. */
+int bt_build_tree(bt_ctx *ctx, int nleaf);
+/* computeMultipoles(), code/computeMultipoles.cu:194-234. */
+int bt_compute_multipoles(bt_ctx *ctx);
+/* makeGroups(levelSplit, nCrit), code/makeGroups.cu:236-341.
+ * 1 <= level_split <= 7, 1 <= ncrit <= 64. */
+int bt_make_groups(bt_ctx *ctx, int level_split, int ncrit);
+/* computeForces(INTCOUNT), code/computeForces.cu:622-727.  When stats is not
+ * null it receives {direct_sum/N, direct_max, approx_sum/N, approx_max}
+ * (:701-715); grav_potential (:711) is returned by bt_get_potential(). */
+int bt_compute_forces(bt_ctx *ctx, int intcount, double stats[4]);
+/* moveParticles(), code/moveParticles.cu:3-7 (a stub in the reference):
+ * half kick with the current accelerations, then drift: v += a*dt/2,
+ * x += v*dt. */
+int bt_move_particles(bt_ctx *ctx, float dt);
+/* Half kick only (closes a kick-drift-kick step after new forces). */
+int bt_kick(bt_ctx *ctx, float dt_half);
+/* computeEnergies(), code/computeEnergies.cu:3-7 (a stub in the reference):
+ * FP64 reductions K = 1/2 sum m v^2 and W = 1/2 sum m (phi + m/eps), per
+ * system (arrays of nsys) when ekin/epot are not null. */
+int bt_compute_energies(bt_ctx *ctx, double *ekin, double *epot);
+/* One full kick-drift-kick step: forces are (re)computed as needed. */
+int bt_step(bt_ctx *ctx, float dt, int nleaf, int level_split, int ncrit);
+
+/* ---- results ----------------------------------------------------------- */
+
+/* d_ptclAcc (code/Treecode.h:144): (ax, ay, az, phi) per particle. */
+int bt_get_acc(bt_ctx *ctx, float *acc4, int order);
+/* Interaction statistics of the last walk, as bt_compute_forces returns them:
+ * {direct_sum/N, direct_max, approx_sum/N, approx_max} (code/computeForces.cu
+ * :701-715), plus the raw integer totals {direct_sum, approx_sum} in totals2
+ * (may be null). */
+int bt_get_walk_stats(bt_ctx *ctx, double stats[4], double totals2[2]);
+/* grav_potential (code/Treecode.h:162): 1/2 sum m phi incl. self term. */
+int bt_get_potential(bt_ctx *ctx, double *pot);
+/* Counters: {n, ncells, nnodes, nleaves, nlevels, ngroups, nsys, flags}
+ * (reference private members nCells/nNodes/nLeaves/nLevels/nGroups,
+ * code/Treecode.h:135). */
+int bt_get_counts(bt_ctx *ctx, int counts[8]);
+/* Copies of the device tree in the reference layouts:
+ * cells = CellData uint4 (code/Treecode.h:45-89), level_range = int2[32]
+ * (d_level_begIdx), size/mono/quad0 = float4, quad1 = float2 (:158-160).
+ * Any pointer may be null.  Arrays must hold ncells entries. */
+int bt_get_tree(bt_ctx *ctx, uint32_t *cells4, int *level_range64, float *size4,
+                float *mono4, float *quad0_4, float *quad1_2);
+/* (l, s) per cell: bbox size and bbox-centre to com distance
+ * (code/computeMultipoles.cu:171-175). */
+int bt_get_cell_ls(bt_ctx *ctx, float *ls2);
+/* Group list (code/Treecode.h:29-43): (pbeg, np) in PH order. */
+int bt_get_groups(bt_ctx *ctx, int *groups2);
+/* Per-group interaction counts (n_approx, n_direct) of the last walk. */
+int bt_get_group_counts(bt_ctx *ctx, int *counts2);
+/* Sorted PH keys and permutation (d_key/d_value, code/makeGroups.cu:270-280):
+ * perm[k] = tree index of PH index k.  Either pointer may be null. */
+int bt_get_ph(bt_ctx *ctx, uint64_t *keys, int *perm);
+/* Original (input) index of each particle in the given order. */
+int bt_get_ids(bt_ctx *ctx, int *ids, int order);
+/* Root box per system {cx, cy, cz, hsize} (d_domain, code/buildTree.cu:161). */
+int bt_get_boxes(bt_ctx *ctx, float *boxes4);
+
+/* ---- checks and measurement ------------------------------------------- */
+
+/* FP64 direct sum (the error oracle of the walk) for targets given as PH
+ * indices; sources = all particles of the target's system.  out = ns x
+ * (ax, ay, az, phi) doubles, self term included in phi. */
+int bt_direct_fp64(bt_ctx *ctx, int ns, const int *ph_idx, double *out4);
+/* Milliseconds of the phases of the last step, measured with HIP events on
+ * the context stream: {bbox+keys, sort, tree levels, multipoles, ph keys,
+ * ph sort, groups, walk, integrate, energies}. */
+int bt_get_timings(bt_ctx *ctx, float ms[10]);
+/* Walk options: kahan = compensated accumulators (default 1);
+ * stack_ints = per-wave traversal stack capacity (default 16384);
+ * blocks_per_cu = persistent grid size (default 0 = automatic). */
+int bt_set_walk_options(bt_ctx *ctx, int kahan, int stack_ints, int blocks_per_cu);
+
+/* ---- initial conditions and I/O (host only, no device needed) ---------- */
+
+/* Plummer sphere (Aarseth-Henon-Wielen, code/plummer.h:37-117 algorithm)
+ * with a counter-based RNG: radius truncated at rmax (structural units),
+ * Henon units, m = 1/n, centre of mass removed in FP64.  Fills pos4/vel4. */
+int bt_ic_plummer(int n, uint64_t seed, double rmax, float *pos4, float *vel4);
+/* nclust Plummer subclusters: centres U[-box,box]^3, masses log-uniform in
+ * [mmin, 1] normalised to 1, scale radius a0 * m^(1/3), particle count
+ * proportional to mass (equal particle masses 1/n). */
+int bt_ic_clustered(int n, int nclust, uint64_t seed, double box, double mmin, double a0,
+                    float *pos4, float *vel4);
+/* Plummer halo (mass mh) + exponential disk (mass 1-mh, scale length rd,
+ * sech^2 scale height z0) on circular orbits. */
+int bt_ic_halo_disk(int n, uint64_t seed, double mh, double rd, double z0, float *pos4,
+                    float *vel4);
+/* Tipsy snapshot reader (code/read_tipsy.h:84-168, tipsydefs.h:22-51).
+ * With pos4 == null only *n is returned. */
+int bt_read_tipsy(const char *path, int *n, float *pos4, float *vel4, int cap);
+/* Tipsy snapshot writer (dark particles; eps written as given). */
+int bt_write_tipsy(const char *path, int n, const float *pos4, const float *vel4, double time,
+                   float eps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BONSAI2_AMD_H */

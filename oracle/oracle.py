@@ -219,7 +219,8 @@ def direct_fp64(pos, idx, eps, longdouble=False):
     idx = np.ascontiguousarray(idx, np.int32)
     out = np.zeros((len(idx), 4), np.float64)
     fn = lib().or_direct_ld if longdouble else lib().or_direct_fp64
-    fn(len(pos), _ptr(pos), len(idx), _ptr(idx), ctypes.c_double(float(eps) ** 2), _ptr(out))
+    e = float(np.float32(eps))  # the softening the float32 step uses
+    fn(len(pos), _ptr(pos), len(idx), _ptr(idx), ctypes.c_double(e * e), _ptr(out))
     return out
 
 
@@ -227,7 +228,7 @@ def energies(pos, vel, acc, eps):
     ek = np.zeros(1, np.float64)
     ep = np.zeros(1, np.float64)
     lib().or_energies(len(pos), _ptr(_f4(pos)), _ptr(_f4(vel)), _ptr(_f4(acc)),
-                      ctypes.c_double(float(eps)), _ptr(ek), _ptr(ep))
+                      ctypes.c_double(float(np.float32(eps))), _ptr(ek), _ptr(ep))
     return float(ek[0]), float(ep[0])
 
 
