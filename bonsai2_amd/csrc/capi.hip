@@ -325,7 +325,7 @@ int bt_alloc(bt_ctx *h, int n, int nsys, const int *offsets) {
     rc |= dev_alloc(c, &c.valB, n); rc |= dev_alloc(c, &c.geo_keys, n);
     rc |= dev_alloc(c, &c.pos_ph, n); rc |= dev_alloc(c, &c.perm, n);
     rc |= dev_alloc(c, &c.id_ph, n); rc |= dev_alloc(c, &c.ph_keys, n);
-    rc |= dev_alloc(c, &c.acc, n);
+    rc |= dev_alloc(c, &c.acc, (size_t)n + 64);  // + spare slots for idle walk lanes
     c.cap_groups = n;
     rc |= dev_alloc(c, &c.groups, n); rc |= dev_alloc(c, &c.group_sys, n);
     rc |= dev_alloc(c, &c.gtmp0, n); rc |= dev_alloc(c, &c.gtmp1, n);

@@ -28,7 +28,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kMaxWalkSteps = 1 << 24;
+constexpr int kMaxWalkSteps = 1 << 20;
 
 struct WalkArgs {
   int n, nsys;
@@ -192,10 +192,14 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
   // a failed build (cell or group capacity) leaves placeholder links: do not walk it
   if (a.dc->err & 3) return;
 
+  // The persistent loop has no lane-dependent branches at its ends: two
+  // "if (lane == 0)" blocks around the back edge (queue fetch, stats store) are
+  // merged by the compiler into a region that only lane 0 re-enters, leaving
+  // the other lanes to loop on a stale group index.  Every lane therefore
+  // joins the queue atomic (lane 0 adds 1, the others 0) and the per-group
+  // stores below are branch-free.
   for (;;) {
-    int g = 0;
-    if (lane == 0) g = atomicAdd(&a.dc->queue, 1);
-    g = wave_uniform(__shfl(g, 0, kWave));
+    const int g = wave_uniform(atomicAdd(&a.dc->queue, lane == 0 ? 1 : 0));
     if (g >= ngroups) break;
     const int2 grp = a.groups[g];
     const int pbeg = grp.x, np = max(grp.y, 1);
@@ -223,8 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
     int n_app = 0, n_dir = 0;    // interactions evaluated (stats)
     bool overflow = false;
     int steps = 0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
 
     while (top > 0) {
       // every cell is visited at most once per group; a longer walk means a
@@ -315,12 +318,12 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
         if (lane < na) S.alist[lane] = S.alist[kWave + lane];
         wave_lds_sync();
       }
-      // make this step's stack pushes visible to the next step's pops
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      // stack pushes of this step (stores by some lanes) must have completed
+      // before the next step's pops (loads by other lanes)
+      __builtin_amdgcn_s_waitcnt(0);
     }
     if (overflow) {
-      if (lane == 0) atomicOr(&a.dc->err, 4);
+      atomicOr(&a.dc->err, 4);  // every lane, same bit
       na = 0;
       nd = 0;
     }
@@ -332,12 +335,11 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
       eval_p2p<KAHAN>(a, S, nd, gcx, gcy, gcz, xi, yi, zi, s, cmp);
       n_dir += nd;
     }
-    if (valid) a.acc[pbeg + lane] = make_float4(s.x, s.y, s.z, s.w);
+    // lanes past the end of the group write to the kWave spare slots after acc[n)
+    a.acc[valid ? pbeg + lane : a.n + lane] = make_float4(s.x, s.y, s.z, s.w);
     const double gp = wave_sum(valid ? (double)pi.w * (double)s.w : 0.0);
-    if (lane == 0) {
-      a.gcounts[g] = overflow ? make_int2(-1, -1) : make_int2(n_app, n_dir);
-      a.gpot[g] = gp;
-    }
+    a.gcounts[g] = overflow ? make_int2(-1, -1) : make_int2(n_app, n_dir);
+    a.gpot[g] = gp;
   }
 }
 
