@@ -22,9 +22,11 @@
 // Multipoles (code/computeMultipoles.cu:81-190) are computed bottom-up, one
 // launch per level from the deepest: a leaf sums its particles, a node
 // combines its children with the parallel-axis theorem.  Moments are carried
-// relative to each cell's exact geometric centre in FP32 (so the sums do not
-// cancel for small cells far from the origin); levels <= kFp64Levels combine
-// in FP64.  Outputs keep the reference layout (:180-188).
+// relative to a reference point inside the cell's particle bbox (a leaf's
+// first particle; a node's first child's reference), so every offset is at
+// most the particle extent l and the FP32 sums keep their digits for small
+// cells far from the origin and for sparse cells; levels <= kFp64Levels
+// combine in FP64.  Outputs keep the reference layout (:180-188).
 #include "bt_device.h"
 #include "bt_internal.h"
 
@@ -369,14 +371,11 @@ __global__ void k_levels_finish(DevCounters *dc, int2 *level_beg) {
 // ---- bottom-up multipoles -------------------------------------------------
 
 struct MpArgs {
-  int L, nsys;
+  int L;
   float inv_theta;
-  const int *off;
-  const float4 *box;
-  const uint64_t *keys;   // sorted geometric keys (tree order)
   const float4 *pos;      // tree order
   const uint4 *cells;
-  float4 *scr;            // 6 per cell: (g,M) (d,-) (Qxx,Qyy,Qzz,Qxy) (Qxz,Qyz,-,-) bmin bmax
+  float4 *scr;            // 6 per cell: (ref,M) (com-ref,-) (Qxx,Qyy,Qzz,Qxy) (Qxz,Qyz,-,-) bmin bmax
   float4 *size, *mono, *q0;
   float2 *q1, *ls;
   const DevCounters *dc;
@@ -387,12 +386,12 @@ struct Moments {
   R M, sx, sy, sz, xx, yy, zz, xy, xz, yz;
 };
 
-__device__ void write_cell(const MpArgs &a, int c, float gx, float gy, float gz, double M,
+__device__ void write_cell(const MpArgs &a, int c, float rx, float ry, float rz, double M,
                            double dx, double dy, double dz, double qxx, double qyy, double qzz,
                            double qxy, double qxz, double qyz, float3 mn, float3 mx) {
 #pragma clang fp contract(off)
-  const float comx = (float)((double)gx + dx), comy = (float)((double)gy + dy),
-              comz = (float)((double)gz + dz);
+  const float comx = (float)((double)rx + dx), comy = (float)((double)ry + dy),
+              comz = (float)((double)rz + dz);
   const float bcx = (mx.x + mn.x) * 0.5f, bcy = (mx.y + mn.y) * 0.5f, bcz = (mx.z + mn.z) * 0.5f;
   const float hx = (mx.x - mn.x) * 0.5f, hy = (mx.y - mn.y) * 0.5f, hz = (mx.z - mn.z) * 0.5f;
   const float ex = bcx - comx, ey = bcy - comy, ez = bcz - comz;
@@ -405,7 +404,7 @@ __device__ void write_cell(const MpArgs &a, int c, float gx, float gy, float gz,
   a.q1[c] = make_float2((float)qxz, (float)qyz);
   a.ls[c] = make_float2(l, s);
   float4 *s6 = a.scr + 6 * (size_t)c;
-  s6[0] = make_float4(gx, gy, gz, (float)M);
+  s6[0] = make_float4(rx, ry, rz, (float)M);
   s6[1] = make_float4((float)dx, (float)dy, (float)dz, 0.f);
   s6[2] = make_float4((float)qxx, (float)qyy, (float)qzz, (float)qxy);
   s6[3] = make_float4((float)qxz, (float)qyz, 0.f, 0.f);
@@ -414,31 +413,33 @@ __device__ void write_cell(const MpArgs &a, int c, float gx, float gy, float gz,
 }
 
 template <typename R>
-__device__ void finish_moments(const MpArgs &a, int c, float gx, float gy, float gz,
+__device__ void finish_moments(const MpArgs &a, int c, float rx, float ry, float rz,
                                const Moments<R> &m, float3 mn, float3 mx) {
   if (!(m.M > R(0))) {
-    write_cell(a, c, gx, gy, gz, 0.0, 0, 0, 0, 0, 0, 0, 0, 0, 0, mn, mx);
+    write_cell(a, c, rx, ry, rz, 0.0, 0, 0, 0, 0, 0, 0, 0, 0, 0, mn, mx);
     return;
   }
   const R im = R(1) / m.M;
   const R dx = m.sx * im, dy = m.sy * im, dz = m.sz * im;
-  write_cell(a, c, gx, gy, gz, (double)m.M, (double)dx, (double)dy, (double)dz,
+  write_cell(a, c, rx, ry, rz, (double)m.M, (double)dx, (double)dy, (double)dz,
              (double)(m.xx * im - dx * dx), (double)(m.yy * im - dy * dy),
              (double)(m.zz * im - dz * dz), (double)(m.xy * im - dx * dy),
              (double)(m.xz * im - dx * dz), (double)(m.yz * im - dy * dz), mn, mx);
 }
 
 template <typename R>
-__device__ void node_moments(const MpArgs &a, int c, const uint4 cd, float gx, float gy,
-                             float gz) {
+__device__ void node_moments(const MpArgs &a, int c, const uint4 cd) {
   Moments<R> m{};
   float3 mn = make_float3(1e10f, 1e10f, 1e10f), mx = make_float3(-1e10f, -1e10f, -1e10f);
   const int f = cell_first(cd), nc = cell_nchild(cd);
+  const float4 r0 = a.scr[6 * (size_t)f];  // this node's reference = first child's
   for (int k = 0; k < nc; k++) {
     const float4 *s6 = a.scr + 6 * (size_t)(f + k);
     const float4 g = s6[0], d = s6[1], qa = s6[2], qb = s6[3], bmn = s6[4], bmx = s6[5];
-    // child com relative to this cell's centre: exact centre offset + child offset
-    const R ox = R(g.x - gx) + R(d.x), oy = R(g.y - gy) + R(d.y), oz = R(g.z - gz) + R(d.z);
+    // child com relative to this node's reference: both references lie in the
+    // node's bbox, so their difference is at most l and (near-)exact in float
+    const R ox = R(g.x - r0.x) + R(d.x), oy = R(g.y - r0.y) + R(d.y),
+            oz = R(g.z - r0.z) + R(d.z);
     const R Mc = R(g.w);
     m.M += Mc;
     m.sx += Mc * ox; m.sy += Mc * oy; m.sz += Mc * oz;
@@ -448,7 +449,7 @@ __device__ void node_moments(const MpArgs &a, int c, const uint4 cd, float gx, f
     mn.x = fminf(mn.x, bmn.x); mn.y = fminf(mn.y, bmn.y); mn.z = fminf(mn.z, bmn.z);
     mx.x = fmaxf(mx.x, bmx.x); mx.y = fmaxf(mx.y, bmx.y); mx.z = fmaxf(mx.z, bmx.z);
   }
-  finish_moments<R>(a, c, gx, gy, gz, m, mn, mx);
+  finish_moments<R>(a, c, r0.x, r0.y, r0.z, m, mn, mx);
 }
 
 __global__ __launch_bounds__(kBlock) void k_multipole_level(MpArgs a) {
@@ -458,30 +459,25 @@ __global__ __launch_bounds__(kBlock) void k_multipole_level(MpArgs a) {
     const int c = c0 + t;
     const uint4 cd = a.cells[c];
     const int pb = (int)cd.z, pe = (int)cd.w;
-    // geometric centre of the cell: replay the ChildBox steps of its key prefix
-    const int r = a.nsys > 1 ? system_of(pb, a.nsys, a.off) : 0;
-    const float4 bx = a.box[r];
-    float gx = bx.x, gy = bx.y, gz = bx.z, h = bx.w;
-    const uint64_t key = a.keys[pb];
-    for (int l = 1; l <= a.L; l++) child_step(gx, gy, gz, h, (int)((key >> (63 - 3 * l)) & 7ull));
     if (cell_leaf(cd)) {
       Moments<float> m{};
       float3 mn = make_float3(1e10f, 1e10f, 1e10f), mx = make_float3(-1e10f, -1e10f, -1e10f);
+      const float4 r0 = a.pos[pb];  // reference point: the first particle
       for (int i = pb; i < pe; i++) {
         const float4 p = a.pos[i];
         mn.x = fminf(mn.x, p.x); mn.y = fminf(mn.y, p.y); mn.z = fminf(mn.z, p.z);
         mx.x = fmaxf(mx.x, p.x); mx.y = fmaxf(mx.y, p.y); mx.z = fmaxf(mx.z, p.z);
-        const float dx = p.x - gx, dy = p.y - gy, dz = p.z - gz, w = p.w;
+        const float dx = p.x - r0.x, dy = p.y - r0.y, dz = p.z - r0.z, w = p.w;
         m.M += w;
         m.sx += w * dx; m.sy += w * dy; m.sz += w * dz;
         m.xx += w * dx * dx; m.yy += w * dy * dy; m.zz += w * dz * dz;
         m.xy += w * dx * dy; m.xz += w * dx * dz; m.yz += w * dy * dz;
       }
-      finish_moments<float>(a, c, gx, gy, gz, m, mn, mx);
+      finish_moments<float>(a, c, r0.x, r0.y, r0.z, m, mn, mx);
     } else if (a.L <= kFp64Levels) {
-      node_moments<double>(a, c, cd, gx, gy, gz);
+      node_moments<double>(a, c, cd);
     } else {
-      node_moments<float>(a, c, cd, gx, gy, gz);
+      node_moments<float>(a, c, cd);
     }
   }
 }
@@ -534,11 +530,7 @@ void launch_levels(Ctx &c) {
 
 void launch_multipoles(Ctx &c) {
   MpArgs a;
-  a.nsys = c.nsys;
   a.inv_theta = (float)(1.0 / (double)c.theta);
-  a.off = c.d_off;
-  a.box = c.box;
-  a.keys = c.geo_keys;
   a.pos = c.pos;
   a.cells = c.cells;
   a.scr = c.cscr;

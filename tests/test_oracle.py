@@ -1,0 +1,304 @@
+"""CPU tests of the oracle (the checker), pinned against things it does not
+compute itself: the reference's own Plummer sampler and published tree
+counts, closed forms, brute force, and structural invariants."""
+import itertools
+
+import numpy as np
+import pytest
+
+
+# ---- reference sampler and published known answer ----------------------------
+
+def test_plummer_restatement_matches_reference_sampler(oracle):
+    """or_plummer_ref is bit-exact against code/plummer.h compiled into oracle/_ref."""
+    ref = oracle.ref_host()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    for n in (64, 1000, 4096):
+        m1, p1, v1 = oracle.plummer_ref(n)
+        m2, p2, v2 = np.zeros(n), np.zeros(3 * n), np.zeros(3 * n)
+        assert ref.ref_plummer(n, 19810614, oracle._ptr(m2), oracle._ptr(p2), oracle._ptr(v2)) == 0
+        assert np.array_equal(m1, m2)
+        assert np.array_equal(p1.ravel(), p2)
+        assert np.array_equal(v1.ravel(), v2)
+
+
+def test_tree_counts_match_published_known_answer(oracle):
+    """treebuild/readme:32-40: Plummer N=2e7, seed 19810614, NLEAF=16 ->
+    nb_leaf 20000000, nnodes 596463, nleaves 4041848, ncells 4638311, nlevels 15."""
+    pos, _ = oracle.plummer_ref_f4(20_000_000, 19810614, 100.0)
+    t = oracle.build_tree(pos, 16)
+    assert t.nb_leaf == 20_000_000
+    assert t.nnodes == 596463
+    assert t.nleaves == 4041848
+    assert t.ncells == 4638311
+    assert t.nlevels == 15
+    assert t.capped == 0
+
+
+# ---- root box ------------------------------------------------------------------
+
+def test_bounding_box_hand_case(oracle):
+    # extent (0,0,0)..(3,1,1): centre (1.5,.5,.5), half extent 1.5 -> hsize 2,
+    # centre already a multiple of 2/2^20
+    pos = np.array([[0, 0, 0, 1], [3, 1, 1, 1]], np.float32)
+    box, mn, mx = oracle.bounding_box(pos)
+    assert list(box) == [1.5, 0.5, 0.5, 2.0]
+    assert list(mn) == [0, 0, 0] and list(mx) == [3, 1, 1]
+
+
+def test_bounding_box_properties(oracle):
+    rng = np.random.default_rng(3)
+    for scale, shift in ((1.0, 0.0), (1e-3, 5.0), (40.0, -7.0)):
+        pos = np.zeros((1000, 4), np.float32)
+        pos[:, :3] = rng.normal(shift, scale, (1000, 3))
+        box, mn, mx = oracle.bounding_box(pos)
+        c, h = box[:3].astype(np.float64), float(box[3])
+        assert np.array_equal(mn, pos[:, :3].min(0)) and np.array_equal(mx, pos[:, :3].max(0))
+        half = float(((mx.astype(np.float64) - mn) / 2).max())
+        assert h >= half > h / 2                       # smallest power of two >= extent
+        assert np.log2(h) == np.round(np.log2(h))
+        q = h / 2 ** 20                                 # centre quantised to hsize/2^20
+        assert np.all(np.abs(c / q - np.round(c / q)) == 0)
+        # within one quantum of the float32 midpoint (code/buildTree.cu:145 is float)
+        mid = (mx.astype(np.float64) + mn) / 2
+        assert np.all(np.abs(c - mid) < q * 1.0001 + np.spacing(np.abs(mid).astype(np.float32)))
+
+
+# ---- octree ----------------------------------------------------------------------
+
+def test_recursive_partition_equals_geometric_key_sort(oracle, bt):
+    """Two routes to the tree order: recursive stable partition by ChildBox
+    octants vs a stable sort of each particle's own octant-digit key."""
+    pos, _ = bt.plummer(20000, seed=5)
+    box, _, _ = oracle.bounding_box(pos)
+    t = oracle.build_tree(pos, 16, box)
+    keys = oracle.geo_keys(pos, box)
+    assert np.array_equal(t.order, np.argsort(keys, kind="stable"))
+
+
+@pytest.mark.parametrize("nleaf", [16, 24, 32, 48, 64])
+def test_tree_invariants(oracle, bt, nleaf):
+    pos, _ = bt.clustered(20000, seed=9)
+    box, _, _ = oracle.bounding_box(pos)
+    t = oracle.build_tree(pos, nleaf, box)
+    keys = oracle.geo_keys(pos, box)[t.order]
+    cells = t.cells.astype(np.int64)
+    level = cells[:, 0] >> 27
+    leaf = cells[:, 1] == 0xFFFFFFFF
+    b, e = cells[:, 2], cells[:, 3]
+    np_ = e - b
+    assert np_[leaf].sum() == len(pos)                  # SWtest/treebuild.cu:1477
+    assert (np_[leaf] <= nleaf).all() and (np_[~leaf] > nleaf).all()
+    assert (np.diff(level) >= 0).all()                  # level order
+    for L in range(1, t.nlevels + 1):
+        lo, hi = t.level_range[L]
+        assert (level[lo:hi + 1] == L).all()
+    first = cells[:, 1] & 0x1FFFFFFF
+    nch = (cells[:, 1] >> 29) + 1
+    for c in np.where(~leaf)[0]:
+        kids = np.arange(first[c], first[c] + nch[c])
+        assert (level[kids] == level[c] + 1).all()
+        # children partition the parent's range
+        rng_ = sorted(zip(b[kids], e[kids]))
+        assert rng_[0][0] == b[c] and rng_[-1][1] == e[c]
+        assert all(rng_[i][1] == rng_[i + 1][0] for i in range(len(rng_) - 1))
+        # nodes first, then leaves (code/buildTree.cu:393-400)
+        lk = leaf[kids]
+        assert not (lk[:-1] & ~lk[1:]).any()
+    # every particle of a cell shares the cell's octant-digit prefix
+    for c in range(0, len(cells), 97):
+        sh = np.uint64(63 - 3 * int(level[c]))
+        pre = keys[b[c]:e[c]] >> sh
+        assert (pre == pre[0]).all()
+
+
+def test_depth_cap_on_coincident_particles(oracle):
+    pos = np.zeros((40, 4), np.float32)
+    pos[:, 3] = 1.0 / 40
+    pos[:20, :3] = 0.25  # 20 coincident particles > NLEAF: no split can separate them
+    pos[20:, 0] = np.linspace(-1, 1, 20)
+    t = oracle.build_tree(pos, 16)
+    assert t.capped == 1 and t.nb_leaf == 40 and t.nlevels == 21
+
+
+# ---- Peano-Hilbert keys -----------------------------------------------------------
+
+@pytest.mark.parametrize("nbits", [1, 2, 3])
+def test_hilbert_key_bijective_and_face_adjacent(oracle, nbits):
+    """code/makeGroups.cu:19-106 is a Hilbert curve: a bijection on the
+    (2^nbits)^3 cells whose consecutive keys are face neighbours."""
+    keys = {}
+    for x, y, z in itertools.product(range(2 ** nbits), repeat=3):
+        keys[oracle.hilbert_key(x, y, z, nbits)] = (x, y, z)
+    assert sorted(keys) == list(range(8 ** nbits))
+    walk = [keys[k] for k in range(8 ** nbits)]
+    assert all(sum(abs(a - b) for a, b in zip(walk[i], walk[i + 1])) == 1
+               for i in range(len(walk) - 1))
+
+
+def test_hilbert_key_golden(oracle):
+    # top digit only (lower bits zero): output digit table of makeGroups.cu:28
+    top = 1 << 20
+    table = [0, 1, 7, 6, 3, 2, 4, 5]
+    for idx in range(8):
+        x, y, z = (idx >> 2) & 1, (idx >> 1) & 1, idx & 1
+        k = oracle.hilbert_key(x * top, y * top, z * top, 21)
+        assert k >> 60 == table[idx]
+    assert oracle.hilbert_key(0, 0, 0, 21) == 0
+
+
+# ---- multipoles ---------------------------------------------------------------------
+
+def _cell(pos):
+    return np.array([[1 << 27, 0xFFFFFFFF, 0, len(pos)]], np.uint32)
+
+
+def test_multipole_closed_forms(oracle):
+    # one particle: com = x, Q = 0, l = 1e-6 floor, s = 0
+    p = np.array([[0.3, -0.2, 0.1, 0.5]], np.float32)
+    size, mono, q0, q1, ls = oracle.multipoles(_cell(p), p, 0.5)
+    assert np.array_equal(mono[0], p[0]) and not q0.any() and not q1.any()
+    assert ls[0, 0] == np.float32(1e-6) and ls[0, 1] == 0
+    # two equal masses at x = +-a: com 0, Q_xx = a^2, l = 2a, cellOp = 2a/theta
+    a = 0.25
+    p = np.array([[a, 0, 0, 1], [-a, 0, 0, 1]], np.float32)
+    size, mono, q0, q1, ls = oracle.multipoles(_cell(p), p, 0.5)
+    assert mono[0, 3] == 2 and np.allclose(mono[0, :3], 0)
+    assert q0[0, 0] == np.float32(a * a) and not q0[0, 1:].any() and not q1.any()
+    assert ls[0, 0] == 2 * a and size[0, 3] == np.float32((2 * a / 0.5) ** 2)
+
+
+def test_multipole_parallel_axis_and_psd(oracle, bt):
+    """Q of a node equals the mass-weighted children Q plus the spread of the
+    children's centres (parallel-axis theorem); Q is positive semi-definite."""
+    pos, _ = bt.plummer(5000, seed=4)
+    t = oracle.build_tree(pos, 16)
+    pt = pos[t.order]
+    size, mono, q0, q1, ls = oracle.multipoles(t.cells, pt, 0.5)
+    Q = np.zeros((len(t.cells), 3, 3))
+    Q[:, 0, 0], Q[:, 1, 1], Q[:, 2, 2] = q0[:, 0], q0[:, 1], q0[:, 2]
+    Q[:, 0, 1] = Q[:, 1, 0] = q0[:, 3]
+    Q[:, 0, 2] = Q[:, 2, 0] = q1[:, 0]
+    Q[:, 1, 2] = Q[:, 2, 1] = q1[:, 1]
+    assert (np.linalg.eigvalsh(Q).min(axis=1) >= -1e-7 * ls[:, 0] ** 2).all()
+    cells = t.cells.astype(np.int64)
+    for c in np.where(cells[:, 1] != 0xFFFFFFFF)[0][:200]:
+        f, n = cells[c, 1] & 0x1FFFFFFF, (cells[c, 1] >> 29) + 1
+        kids = np.arange(f, f + n)
+        m = mono[kids, 3].astype(np.float64)
+        x = mono[kids, :3].astype(np.float64)
+        M = m.sum()
+        com = (m[:, None] * x).sum(0) / M
+        d = x - com
+        Qp = ((m[:, None, None] * (Q[kids] + d[:, :, None] * d[:, None, :])).sum(0)) / M
+        assert np.allclose(com, mono[c, :3], atol=1e-6 * ls[c, 0] + 1e-7)
+        assert np.allclose(Qp, Q[c], atol=2e-6 * ls[c, 0] ** 2)
+
+
+def test_reference_fp32_moment_arithmetic_loses_digits(oracle):
+    """The reference's FP32 products (code/computeMultipoles.cu:113-158) lose
+    most digits of Q for a small cell far from the origin; the FP64 mode does
+    not.  This is why the oracle's default sums moments in FP64."""
+    rng = np.random.default_rng(1)
+    p = np.zeros((16, 4), np.float32)
+    p[:, :3] = 10.0 + 1e-3 * rng.random((16, 3))
+    p[:, 3] = 1.0 / 16
+    exact = np.cov(p[:, :3].astype(np.float64).T, bias=True)[0, 0]
+    q64 = oracle.multipoles(_cell(p), p, 0.5, mode=0)[2][0, 0]
+    q32 = oracle.multipoles(_cell(p), p, 0.5, mode=1)[2][0, 0]
+    assert abs(q64 - exact) < 1e-6 * exact
+    assert abs(q32 - exact) > 1e-2 * exact
+
+
+# ---- groups ---------------------------------------------------------------------------
+
+def test_groups_closed_form(oracle):
+    # masked level-1 digits: runs of length 5, 3, 130 -> chop by 64
+    keys = np.array([0] * 5 + [1 << 60] * 3 + [3 << 60] * 130, np.uint64)
+    g = oracle.make_groups(keys, 1, 64)
+    assert g.tolist() == [[0, 5], [5, 3], [8, 64], [72, 64], [136, 2]]
+    # lower digits do not split at level_split 1 but do at 2
+    keys = np.array([0, 1 << 57, 2 << 57], np.uint64)
+    assert oracle.make_groups(keys, 1, 64).tolist() == [[0, 3]]
+    assert oracle.make_groups(keys, 2, 64).tolist() == [[0, 1], [1, 1], [2, 1]]
+
+
+# ---- walk ------------------------------------------------------------------------------
+
+def _pipeline(oracle, pos, theta, eps, nleaf=16, ncrit=64, acc_type=0):
+    return oracle.reference_pipeline(pos, nleaf, ncrit, theta, eps, 5, acc_type)
+
+
+def test_walk_theta_to_zero_is_direct_sum(oracle, bt):
+    pos, _ = bt.plummer(2000, seed=2)
+    r = _pipeline(oracle, pos, 1e-4, 0.01, acc_type=1)
+    assert r["stats"][2] == 0  # no approximations left
+    d = oracle.direct_fp64(r["pos_ph"], np.arange(len(pos), dtype=np.int32), 0.01)
+    rel = np.linalg.norm(r["acc"][:, :3] - d[:, :3], axis=1) / np.linalg.norm(d[:, :3], axis=1)
+    assert rel.max() < 1e-5
+    assert np.allclose(r["acc"][:, 3], d[:, 3], rtol=1e-5)
+
+
+def test_walk_two_body_closed_form(oracle):
+    m1, m2, r, eps = 0.7, 0.3, 0.4, 0.05
+    pos = np.array([[0, 0, 0, m1], [r, 0, 0, m2]], np.float32)
+    res = _pipeline(oracle, pos, 0.5, eps, acc_type=1)
+    s = (r * r + eps * eps) ** -1.5
+    acc = {round(float(p[0]), 3): a for p, a in zip(res["pos_ph"], res["acc"])}
+    assert np.isclose(acc[0.0][0], m2 * r * s, rtol=1e-6)
+    assert np.isclose(acc[0.4][0], -m1 * r * s, rtol=1e-6)
+    # potential includes the self term -m_i/eps (code/computeForces.cu:64-72)
+    assert np.isclose(acc[0.0][3], -m1 / eps - m2 / np.sqrt(r * r + eps * eps), rtol=1e-6)
+
+
+def test_quadrupole_error_scales_as_third_power(oracle):
+    """P2C of a 4-particle cell vs the exact sum: the truncation error
+    of a monopole+quadrupole expansion falls as (a/d)^3 (next term: octupole)."""
+    a = 0.05
+    errs = []
+    for d in (1.0, 2.0, 4.0):
+        # 4 particles: cells with < 3 particles are always opened (:284-285)
+        pts = np.array([[a, a / 3, 0, 0.3], [-a, -a / 3, 0, 0.3], [a / 2, -a, a / 4, 0.2],
+                        [-a / 2, a, -a / 4, 0.2]], np.float64)
+        cell = _cell(pts.astype(np.float32))
+        size, mono, q0, q1, ls = oracle.multipoles(cell, pts.astype(np.float32), 0.5)
+        target = np.array([[d, 0.3 * d, 0.1 * d, 1e-9]], np.float32)
+        lr = np.zeros(64, np.int32)
+        lr[2:4] = [0, 0]
+        acc, _, _ = oracle.walk(np.array([[0, 1]], np.int32), lr, target, pts.astype(np.float32),
+                                cell, size * 0, mono, q0, q1, 0.0, acc_type=1)
+        x = target[0, :3].astype(np.float64)
+        ex = sum(p[3] * (p[:3] - x) / np.linalg.norm(p[:3] - x) ** 3 for p in pts)
+        errs.append(np.linalg.norm(acc[0, :3] - ex) / np.linalg.norm(ex))
+    assert errs[0] / errs[1] > 6 and errs[1] / errs[2] > 6  # ~8 per doubling
+
+
+def test_direct_fp64_against_long_double(oracle, bt):
+    pos, _ = bt.plummer(300, seed=8)
+    idx = np.arange(300, dtype=np.int32)
+    a = oracle.direct_fp64(pos, idx, 0.01)
+    b = oracle.direct_fp64(pos, idx, 0.01, longdouble=True)
+    assert np.allclose(a, b, rtol=1e-12, atol=0)
+
+
+def test_energies_two_body(oracle):
+    m1, m2, r, eps = 0.6, 0.4, 0.5, 0.05
+    pos = np.array([[0, 0, 0, m1], [r, 0, 0, m2]], np.float32)
+    vel = np.array([[0, 0.3, 0, m1], [0, -0.45, 0, m2]], np.float32)
+    acc = oracle.direct_fp64(pos, np.arange(2, dtype=np.int32), eps).astype(np.float32)
+    ek, ep = oracle.energies(pos, vel, acc, eps)
+    assert np.isclose(ek, 0.5 * m1 * 0.3 ** 2 + 0.5 * m2 * 0.45 ** 2, rtol=1e-6)
+    assert np.isclose(ep, -m1 * m2 / np.sqrt(r * r + eps * eps), rtol=1e-5)
+
+
+def test_walk_stats_are_list_lengths(oracle, bt):
+    pos, _ = bt.plummer(3000, seed=6)
+    r = _pipeline(oracle, pos, 0.6, 0.01)
+    acc, st, pot, cnt = oracle.walk(r["groups"], r["tree"].level_range, r["pos_ph"],
+                                    r["pos_tree"], r["tree"].cells, r["size"], r["mono"],
+                                    r["q0"], r["q1"], 0.01, counts=True)
+    npg = r["groups"][:, 1].astype(np.int64)
+    assert st[0] == (cnt[:, 1] * npg).sum() and st[2] == (cnt[:, 0] * npg).sum()
+    assert st[1] == cnt[:, 1].max() and st[3] == cnt[:, 0].max()
+    assert np.isclose(pot, 0.5 * (r["pos_ph"][:, 3].astype(np.float64) * acc[:, 3]).sum())
