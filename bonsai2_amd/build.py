@@ -59,6 +59,16 @@ def build(verbose=False):
     if rc != 0:
         raise RuntimeError("link failed")
     os.replace(tmp, OUT)
+    # the command-line driver, linked against the library next to it
+    exe = os.path.join(HERE, "treecode")
+    rc, cmd, out = _run(["g++"] + HOST_FLAGS + [os.path.join(CSRC, "treecode_main.cpp"), "-o",
+                                                 exe + ".tmp", "-L" + HERE, "-lbonsai2_amd",
+                                                 "-Wl,-rpath,$ORIGIN"])
+    if verbose or rc != 0:
+        sys.stderr.write(cmd + "\n" + out)
+    if rc != 0:
+        raise RuntimeError("driver link failed")
+    os.replace(exe + ".tmp", exe)
     return OUT
 
 

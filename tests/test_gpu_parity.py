@@ -3,7 +3,7 @@
 Bars: bit-exact for the root box, geometric tree, PH keys, permutation and
 groups (integer/index work); multipoles within FP32 rounding of the oracle's
 FP64 per-cell sums; forces within 1e-5 relative RMS of the oracle walk with
-the same theta/NLEAF/NCRIT (BASELINE tolerance), interaction counts equal
+the same theta/NLEAF/NCRIT, interaction counts equal
 (to 1e-4 relative: an opening decision on the exact float boundary may flip
 when the GPU's moments differ from the oracle's in the last bit).
 """
@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 CASES = [
     # (name, n, theta, eps, nleaf, ncrit, level_split, ic)
-    ("c1_plummer_2p16", 1 << 16, 0.4, 0.01, 16, 64, 5, "plummer"),
+    ("plummer_2p16_theta0.4", 1 << 16, 0.4, 0.01, 16, 64, 5, "plummer"),
     ("plummer_nleaf32_ncrit32", 20000, 0.5, 0.01, 32, 32, 5, "plummer"),
     ("clustered_small", 1 << 15, 0.5, 0.002, 16, 64, 5, "clustered"),
     ("halo_disk_small", 1 << 15, 0.4, 0.002, 24, 48, 4, "halo_disk"),
