@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--cpu-groups", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kahan", action="store_true")
+    ap.add_argument("--walk-variant", type=int, default=1, choices=(0, 1),
+                    help="1: j-data by scalar loads, 0: j-data staged in LDS")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -110,6 +112,7 @@ def main():
     t = bt.Treecode(w["eps"], w["theta"], args.ncrit)
     if args.no_kahan:
         t.set_walk_options(kahan=0)
+    t.set_walk_variant(args.walk_variant)
     t.alloc(n)
     t.ptcl_h2d(pos, vel)
     # first force evaluation (not timed), then warmup steps
@@ -164,7 +167,8 @@ def main():
         "data": "synthetic (seeded %s, resident in HBM)" % w["ic"],
         "config": {"workload": args.workload, "n": n, "theta": w["theta"], "eps": w["eps"],
                    "nleaf": args.nleaf, "ncrit": args.ncrit, "level_split": args.level_split,
-                   "dt": args.dt, "opening": "fixed theta", "kahan": not args.no_kahan},
+                   "dt": args.dt, "opening": "fixed theta", "kahan": not args.no_kahan,
+                   "walk_jdata": "scalar" if args.walk_variant == 1 else "lds"},
         "interactions_per_s": float((inter / (walk_ms * 1e-3)).mean()),
         "walk_ms": float(walk_ms.mean()),
         "force_evals_per_s_walk": float(n / (walk_ms.mean() * 1e-3)),

@@ -58,6 +58,7 @@ _SIGS = {
     "bt_direct_fp64": [P, I, P, P],
     "bt_get_timings": [P, P],
     "bt_set_walk_options": [P, I, I, I],
+    "bt_set_walk_variant": [P, I],
     "bt_ic_plummer": [I, ctypes.c_uint64, D, P, P],
     "bt_ic_clustered": [I, I, ctypes.c_uint64, D, D, D, P, P],
     "bt_ic_halo_disk": [I, ctypes.c_uint64, D, D, D, P, P],
@@ -161,6 +162,10 @@ class Treecode:
     def set_walk_options(self, kahan=1, stack_ints=16384, blocks_per_cu=0):
         self._ck(self._L.bt_set_walk_options(self._h, int(kahan), int(stack_ints),
                                              int(blocks_per_cu)), "set_walk_options")
+
+    def set_walk_variant(self, variant):
+        """1 = scalar-load j-data (default), 0 = LDS-staged j-data."""
+        self._ck(self._L.bt_set_walk_variant(self._h, int(variant)), "set_walk_variant")
 
     # -- particles ---------------------------------------------------------
     def ptcl_h2d(self, pos, vel):

@@ -68,6 +68,7 @@ struct Ctx {
   float4 *csize = nullptr, *cmono = nullptr, *cq0 = nullptr;
   float2 *cq1 = nullptr, *cls = nullptr;
   float4 *cscr = nullptr;      // 6 float4 per cell (bottom-up moments)
+  float4 *cwalk = nullptr;     // 3 float4 per cell: walk record (com,M) (3Q) (3Qxz,3Qyz,trQ/2,1.5trQ)
   int *child_bnd = nullptr;    // 8 ints per parent of the current level
   int *lvl_blk = nullptr;      // block sums / offsets of the level scan
   int2 *level_beg = nullptr;   // int2[32] (reference d_level_begIdx)
@@ -95,6 +96,7 @@ struct Ctx {
   int walk_blocks = 0;         // persistent grid (blocks of 256)
   int blocks_per_cu = 0;
   int kahan = 1;
+  int walk_variant = 1;        // 0: j-data staged in LDS, 1: j-data through scalar loads
   int num_cus = 256;
   unsigned long long *d_wstats = nullptr;  // direct_sum, direct_max, approx_sum, approx_max
   double *d_wpot = nullptr;

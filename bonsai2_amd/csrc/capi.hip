@@ -79,6 +79,7 @@ void free_particle_arrays(Ctx &c) {
 void free_cell_arrays(Ctx &c) {
   dev_free(&c.cells); dev_free(&c.csize); dev_free(&c.cmono); dev_free(&c.cq0);
   dev_free(&c.cq1); dev_free(&c.cls); dev_free(&c.cscr); dev_free(&c.child_bnd);
+  dev_free(&c.cwalk);
 }
 
 int alloc_cells(Ctx &c, int cap) {
@@ -91,6 +92,7 @@ int alloc_cells(Ctx &c, int cap) {
   rc |= dev_alloc(c, &c.cq1, cap);
   rc |= dev_alloc(c, &c.cls, cap);
   rc |= dev_alloc(c, &c.cscr, (size_t)cap * 6);
+  rc |= dev_alloc(c, &c.cwalk, (size_t)cap * 3);
   rc |= dev_alloc(c, &c.child_bnd, (size_t)cap * 8 + 8 * (size_t)c.nsys);
   return rc ? BT_ERR_HIP : 0;
 }
@@ -287,6 +289,13 @@ int bt_set_walk_options(bt_ctx *h, int kahan, int stack_ints, int blocks_per_cu)
   c.kahan = kahan ? 1 : 0;
   c.stack_ints = stack_ints;
   c.blocks_per_cu = blocks_per_cu;
+  return 0;
+}
+
+int bt_set_walk_variant(bt_ctx *h, int variant) {
+  if (!h) return BT_ERR_ARG;
+  if (variant < 0 || variant > 1) return fail(h->c, BT_ERR_ARG, "walk variant must be 0 or 1");
+  h->c.walk_variant = variant;
   return 0;
 }
 

@@ -378,6 +378,7 @@ struct MpArgs {
   float4 *scr;            // 6 per cell: (ref,M) (com-ref,-) (Qxx,Qyy,Qzz,Qxy) (Qxz,Qyz,-,-) bmin bmax
   float4 *size, *mono, *q0;
   float2 *q1, *ls;
+  float4 *wrec;           // 3 per cell, the walk's P2C record
   const DevCounters *dc;
 };
 
@@ -403,6 +404,13 @@ __device__ void write_cell(const MpArgs &a, int c, float rx, float ry, float rz,
   a.q0[c] = make_float4((float)qxx, (float)qyy, (float)qzz, (float)qxy);
   a.q1[c] = make_float2((float)qxz, (float)qyz);
   a.ls[c] = make_float2(l, s);
+  // walk record: quadrupole pre-multiplied by 3, trace terms precomputed
+  const float fqxx = (float)qxx, fqyy = (float)qyy, fqzz = (float)qzz;
+  const float tr = fqxx + fqyy + fqzz;
+  a.wrec[3 * (size_t)c] = make_float4(comx, comy, comz, (float)M);
+  a.wrec[3 * (size_t)c + 1] = make_float4(3.0f * fqxx, 3.0f * fqyy, 3.0f * fqzz, 3.0f * (float)qxy);
+  a.wrec[3 * (size_t)c + 2] =
+      make_float4(3.0f * (float)qxz, 3.0f * (float)qyz, 0.5f * tr, 1.5f * tr);
   float4 *s6 = a.scr + 6 * (size_t)c;
   s6[0] = make_float4(rx, ry, rz, (float)M);
   s6[1] = make_float4((float)dx, (float)dy, (float)dz, 0.f);
@@ -539,6 +547,7 @@ void launch_multipoles(Ctx &c) {
   a.q0 = c.cq0;
   a.q1 = c.cq1;
   a.ls = c.cls;
+  a.wrec = c.cwalk;
   a.dc = c.dc;
   const int grid = grid_for(c.cap_cells, kBlock, 2048);
   for (int L = kMaxLevel; L >= 1; L--) {

@@ -11,8 +11,12 @@
 //     opened nodes push their children, unopened cells join the P2C list,
 //     opened leaves put their particles on the P2P list (:293-430);
 //   * lists are kept in LDS, compacted with ballots and prefix sums; every 64
-//     entries the wave stages the j-data (positions relative to the group
-//     centre) in LDS and all 64 lanes sweep it with broadcast reads;
+//     entries the wave evaluates the batch.  Two ways to feed the j-data:
+//     scalar (default) -- the batch index is moved to an SGPR with readlane
+//     and the 48-byte cell record (or particle) is fetched by s_load through
+//     the scalar cache, so each interaction costs VALU work only; LDS --
+//     the wave stages the batch (relative to the group centre) in LDS and
+//     sweeps it with broadcast ds_read_b128;
 //   * each batch is summed in FP32 and folded into Kahan-compensated
 //     accumulators.
 // The interaction set equals the reference's (the order of traversal does
@@ -43,6 +47,7 @@ struct WalkArgs {
   const float4 *cmono;
   const float4 *cq0;
   const float2 *cq1;
+  const float4 *cwalk;   // 3 per cell: (com, M) (3Qxx,3Qyy,3Qzz,3Qxy) (3Qxz,3Qyz,trQ/2,1.5trQ)
   float4 *acc;
   int2 *gcounts;
   double *gpot;
@@ -181,7 +186,102 @@ __device__ __forceinline__ void eval_p2p(const WalkArgs &a, WaveShared &S, int c
   wave_lds_sync();
 }
 
+// Cell records and particles do not change during the walk: reading them
+// through the constant address space lets the compiler fetch a wave-uniform
+// record with s_load (scalar cache) instead of a vector load.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(4))) const f4v cf4v;
+
+__device__ __forceinline__ float4 ld_uniform(const cf4v *p, int i) {
+  const f4v v = p[i];
+  return make_float4(v[0], v[1], v[2], v[3]);
+}
+
+// P2C from a cell record with the quadrupole pre-scaled by 3 (R1 = 3Q diag +
+// 3Qxy, R2 = 3Qxz, 3Qyz, trQ/2, 1.5 trQ); dr = x_i - com and u = dr.3Q.dr/r^2:
+//   phi -= M/r - (trQ/2) M/r^3 + (u/2) M/r^3
+//   a   += (M/r^5 (1.5 trQ - 2.5 u) - M/r^3) dr + M/r^5 (3Q).dr
+// the same terms as the LDS path's p2c.
+__device__ __forceinline__ void p2c_rec(Acc4 &b, float xi, float yi, float zi, const float4 A,
+                                        const float4 B, const float4 C, float eps2) {
+  const float dx = xi - A.x, dy = yi - A.y, dz = zi - A.z;
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  const float rinv2 = rinv * rinv;
+  const float m1 = A.w * rinv;
+  const float m3 = rinv2 * m1;
+  const float m5 = rinv2 * m3;
+  const float qx = fmaf(B.x, dx, fmaf(B.w, dy, C.x * dz));
+  const float qy = fmaf(B.w, dx, fmaf(B.y, dy, C.y * dz));
+  const float qz = fmaf(C.x, dx, fmaf(C.y, dy, B.z * dz));
+  const float u = fmaf(qx, dx, fmaf(qy, dy, qz * dz)) * rinv2;
+  b.w -= m1;
+  b.w = fmaf(-m3, fmaf(0.5f, u, -C.z), b.w);
+  const float cc = fmaf(m5, fmaf(-2.5f, u, C.w), -m3);
+  b.x = fmaf(cc, dx, fmaf(m5, qx, b.x));
+  b.y = fmaf(cc, dy, fmaf(m5, qy, b.y));
+  b.z = fmaf(cc, dz, fmaf(m5, qz, b.z));
+}
+
 template <bool KAHAN>
+__device__ __forceinline__ void eval_p2c_s(const WalkArgs &a, WaveShared &S, int cnt, float xi,
+                                           float yi, float zi, Acc4 &s, Acc4 &c) {
+  const cf4v *rec = (const cf4v *)a.cwalk;
+  const int my = S.alist[lane_id()];
+  Acc4 b = {0.f, 0.f, 0.f, 0.f};
+  if (cnt == kWave) {
+#pragma unroll 8
+    for (int j = 0; j < kWave; j++) {
+      const int r = 3 * __builtin_amdgcn_readlane(my, j);
+      p2c_rec(b, xi, yi, zi, ld_uniform(rec, r), ld_uniform(rec, r + 1), ld_uniform(rec, r + 2),
+              a.eps2);
+    }
+  } else {
+#pragma unroll 2
+    for (int j = 0; j < cnt; j++) {
+      const int r = 3 * __builtin_amdgcn_readlane(my, j);
+      p2c_rec(b, xi, yi, zi, ld_uniform(rec, r), ld_uniform(rec, r + 1), ld_uniform(rec, r + 2),
+              a.eps2);
+    }
+  }
+  fold<KAHAN>(s, c, b);
+}
+
+template <bool KAHAN>
+__device__ __forceinline__ void eval_p2p_s(const WalkArgs &a, WaveShared &S, int cnt, float xi,
+                                           float yi, float zi, Acc4 &s, Acc4 &c) {
+  const cf4v *pj = (const cf4v *)a.pos_tree;
+  const int my = S.dlist[lane_id()];
+  Acc4 b = {0.f, 0.f, 0.f, 0.f};
+  if (cnt == kWave) {
+#pragma unroll 8
+    for (int j = 0; j < kWave; j++) p2p(b, xi, yi, zi, ld_uniform(pj, __builtin_amdgcn_readlane(my, j)), a.eps2);
+  } else {
+#pragma unroll 2
+    for (int j = 0; j < cnt; j++) p2p(b, xi, yi, zi, ld_uniform(pj, __builtin_amdgcn_readlane(my, j)), a.eps2);
+  }
+  fold<KAHAN>(s, c, b);
+}
+
+// One batch of cnt P2C (list in S.alist) or P2P (S.dlist) interactions.  The
+// LDS path takes i relative to the group centre, the scalar path absolute i.
+template <bool KAHAN, bool SJ>
+__device__ __forceinline__ void batch_p2c(const WalkArgs &a, WaveShared &S, int cnt, float gcx,
+                                          float gcy, float gcz, float xi, float yi, float zi,
+                                          Acc4 &s, Acc4 &c) {
+  if (SJ) eval_p2c_s<KAHAN>(a, S, cnt, xi, yi, zi, s, c);
+  else eval_p2c<KAHAN>(a, S, cnt, gcx, gcy, gcz, xi, yi, zi, s, c);
+}
+
+template <bool KAHAN, bool SJ>
+__device__ __forceinline__ void batch_p2p(const WalkArgs &a, WaveShared &S, int cnt, float gcx,
+                                          float gcy, float gcz, float xi, float yi, float zi,
+                                          Acc4 &s, Acc4 &c) {
+  if (SJ) eval_p2p_s<KAHAN>(a, S, cnt, xi, yi, zi, s, c);
+  else eval_p2p<KAHAN>(a, S, cnt, gcx, gcy, gcz, xi, yi, zi, s, c);
+}
+
+template <bool KAHAN, bool SJ>
 __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
   __shared__ WaveShared sh[kWavesPerBlock];
   const int lane = lane_id();
@@ -216,8 +316,9 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
       gcx = 0.5f * (mxx + mnx); gcy = 0.5f * (mxy + mny); gcz = 0.5f * (mxz + mnz);
       gsx = 0.5f * (mxx - mnx); gsy = 0.5f * (mxy - mny); gsz = 0.5f * (mxz - mnz);
     }
-    // i-particle relative to the group centre
-    const float xi = pi.x - gcx, yi = pi.y - gcy, zi = pi.z - gcz;
+    // i-particle: relative to the group centre (LDS path) or absolute (scalar path)
+    const float xi = SJ ? pi.x : pi.x - gcx, yi = SJ ? pi.y : pi.y - gcy,
+                zi = SJ ? pi.z : pi.z - gcz;
 
     Acc4 s = {0.f, 0.f, 0.f, 0.f}, cmp = {0.f, 0.f, 0.f, 0.f};
     const int2 rc = a.root_cells[sys];
@@ -302,7 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
           nd += min(kWave, ltot - k0);
           wave_lds_sync();
           if (nd >= kWave) {
-            eval_p2p<KAHAN>(a, S, kWave, gcx, gcy, gcz, xi, yi, zi, s, cmp);
+            batch_p2p<KAHAN, SJ>(a, S, kWave, gcx, gcy, gcz, xi, yi, zi, s, cmp);
             n_dir += kWave;
             nd -= kWave;
             if (lane < nd) S.dlist[lane] = S.dlist[kWave + lane];
@@ -312,7 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
       }
       wave_lds_sync();
       if (na >= kWave) {
-        eval_p2c<KAHAN>(a, S, kWave, gcx, gcy, gcz, xi, yi, zi, s, cmp);
+        batch_p2c<KAHAN, SJ>(a, S, kWave, gcx, gcy, gcz, xi, yi, zi, s, cmp);
         n_app += kWave;
         na -= kWave;
         if (lane < na) S.alist[lane] = S.alist[kWave + lane];
@@ -328,11 +429,11 @@ __global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
       nd = 0;
     }
     if (na > 0) {
-      eval_p2c<KAHAN>(a, S, na, gcx, gcy, gcz, xi, yi, zi, s, cmp);
+      batch_p2c<KAHAN, SJ>(a, S, na, gcx, gcy, gcz, xi, yi, zi, s, cmp);
       n_app += na;
     }
     if (nd > 0) {
-      eval_p2p<KAHAN>(a, S, nd, gcx, gcy, gcz, xi, yi, zi, s, cmp);
+      batch_p2p<KAHAN, SJ>(a, S, nd, gcx, gcy, gcz, xi, yi, zi, s, cmp);
       n_dir += nd;
     }
     // lanes past the end of the group write to the kWave spare slots after acc[n)
@@ -408,6 +509,7 @@ void launch_walk(Ctx &c) {
   a.cmono = c.cmono;
   a.cq0 = c.cq0;
   a.cq1 = c.cq1;
+  a.cwalk = c.cwalk;
   a.acc = c.acc;
   a.gcounts = c.gcounts;
   a.gpot = c.gpot;
@@ -415,10 +517,10 @@ void launch_walk(Ctx &c) {
   a.stack_ints = c.stack_ints;
   a.dc = c.dc;
   hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, c.stream, c.dc);
-  if (c.kahan)
-    hipLaunchKernelGGL(k_walk<true>, dim3(c.walk_blocks), dim3(kBlock), 0, c.stream, a);
-  else
-    hipLaunchKernelGGL(k_walk<false>, dim3(c.walk_blocks), dim3(kBlock), 0, c.stream, a);
+  void (*kern)(WalkArgs);
+  if (c.walk_variant == 1) kern = c.kahan ? k_walk<true, true> : k_walk<false, true>;
+  else kern = c.kahan ? k_walk<true, false> : k_walk<false, false>;
+  hipLaunchKernelGGL(kern, dim3(c.walk_blocks), dim3(kBlock), 0, c.stream, a);
   hipLaunchKernelGGL(k_walk_stats, dim3(1), dim3(1024), 0, c.stream, c.groups, c.gcounts, c.gpot,
                      c.dc, c.d_wstats, c.d_wpot);
 }
