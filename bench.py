@@ -92,8 +92,8 @@ def main():
     ap.add_argument("--cpu-groups", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kahan", action="store_true")
-    ap.add_argument("--walk-variant", type=int, default=1, choices=(0, 1),
-                    help="1: j-data by scalar loads, 0: j-data staged in LDS")
+    ap.add_argument("--walk-variant", type=int, default=0, choices=(0, 1),
+                    help="0: j-data staged in LDS, 1: j-data by scalar loads")
     args = ap.parse_args()
 
     if args.gpus != 1:

@@ -164,7 +164,7 @@ class Treecode:
                                              int(blocks_per_cu)), "set_walk_options")
 
     def set_walk_variant(self, variant):
-        """1 = scalar-load j-data (default), 0 = LDS-staged j-data."""
+        """0 = LDS-staged j-data (default), 1 = scalar-load j-data."""
         self._ck(self._L.bt_set_walk_variant(self._h, int(variant)), "set_walk_variant")
 
     # -- particles ---------------------------------------------------------

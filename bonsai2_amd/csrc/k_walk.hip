@@ -12,11 +12,12 @@
 //     opened leaves put their particles on the P2P list (:293-430);
 //   * lists are kept in LDS, compacted with ballots and prefix sums; every 64
 //     entries the wave evaluates the batch.  Two ways to feed the j-data:
-//     scalar (default) -- the batch index is moved to an SGPR with readlane
-//     and the 48-byte cell record (or particle) is fetched by s_load through
-//     the scalar cache, so each interaction costs VALU work only; LDS --
-//     the wave stages the batch (relative to the group centre) in LDS and
-//     sweeps it with broadcast ds_read_b128;
+//     LDS (default) -- the wave stages the batch (relative to the group
+//     centre) in LDS and sweeps it with broadcast ds_read_b128; scalar --
+//     the batch index is moved to an SGPR with readlane and the 48-byte cell
+//     record (or particle) is fetched by s_load through the scalar cache.
+//     Both run at the same rate on MI355X: the FP32 VALU issue rate bounds
+//     the walk, not LDS or the scalar cache;
 //   * each batch is summed in FP32 and folded into Kahan-compensated
 //     accumulators.
 // The interaction set equals the reference's (the order of traversal does
@@ -33,6 +34,17 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kMaxWalkSteps = 1 << 20;
+
+// Unroll of the full-batch j-loops and the occupancy target of the walk.
+#ifndef BT_UNROLL_LDS
+#define BT_UNROLL_LDS 8
+#endif
+#ifndef BT_UNROLL_S
+#define BT_UNROLL_S 8
+#endif
+#ifndef BT_WALK_WAVES_PER_EU
+#define BT_WALK_WAVES_PER_EU 1
+#endif
 
 struct WalkArgs {
   int n, nsys;
@@ -154,7 +166,7 @@ __device__ __forceinline__ void eval_p2c(const WalkArgs &a, WaveShared &S, int c
   wave_lds_sync();
   Acc4 b = {0.f, 0.f, 0.f, 0.f};
   if (cnt == kWave) {
-#pragma unroll 8
+#pragma unroll BT_UNROLL_LDS
     for (int j = 0; j < kWave; j++) p2c(b, xi, yi, zi, S.jA[j], S.jB[j], S.jC[j], a.eps2);
   } else {
 #pragma unroll 2
@@ -176,7 +188,7 @@ __device__ __forceinline__ void eval_p2p(const WalkArgs &a, WaveShared &S, int c
   wave_lds_sync();
   Acc4 b = {0.f, 0.f, 0.f, 0.f};
   if (cnt == kWave) {
-#pragma unroll 8
+#pragma unroll BT_UNROLL_LDS
     for (int j = 0; j < kWave; j++) p2p(b, xi, yi, zi, S.jA[j], a.eps2);
   } else {
 #pragma unroll 2
@@ -230,7 +242,7 @@ __device__ __forceinline__ void eval_p2c_s(const WalkArgs &a, WaveShared &S, int
   const int my = S.alist[lane_id()];
   Acc4 b = {0.f, 0.f, 0.f, 0.f};
   if (cnt == kWave) {
-#pragma unroll 8
+#pragma unroll BT_UNROLL_S
     for (int j = 0; j < kWave; j++) {
       const int r = 3 * __builtin_amdgcn_readlane(my, j);
       p2c_rec(b, xi, yi, zi, ld_uniform(rec, r), ld_uniform(rec, r + 1), ld_uniform(rec, r + 2),
@@ -254,7 +266,7 @@ __device__ __forceinline__ void eval_p2p_s(const WalkArgs &a, WaveShared &S, int
   const int my = S.dlist[lane_id()];
   Acc4 b = {0.f, 0.f, 0.f, 0.f};
   if (cnt == kWave) {
-#pragma unroll 8
+#pragma unroll BT_UNROLL_S
     for (int j = 0; j < kWave; j++) p2p(b, xi, yi, zi, ld_uniform(pj, __builtin_amdgcn_readlane(my, j)), a.eps2);
   } else {
 #pragma unroll 2
@@ -282,7 +294,8 @@ __device__ __forceinline__ void batch_p2p(const WalkArgs &a, WaveShared &S, int 
 }
 
 template <bool KAHAN, bool SJ>
-__global__ __launch_bounds__(kBlock) void k_walk(WalkArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BT_WALK_WAVES_PER_EU)))
+void k_walk(WalkArgs a) {
   __shared__ WaveShared sh[kWavesPerBlock];
   const int lane = lane_id();
   const int wv = threadIdx.x >> 6;

@@ -152,10 +152,11 @@ int bt_get_timings(bt_ctx *ctx, float ms[10]);
  * stack_ints = per-wave traversal stack capacity (default 16384);
  * blocks_per_cu = persistent grid size (default 0 = automatic). */
 int bt_set_walk_options(bt_ctx *ctx, int kahan, int stack_ints, int blocks_per_cu);
-/* How the walk feeds interaction data to the lanes: 1 (default) = each
- * batch entry is read by scalar loads straight from the immutable cell /
- * particle arrays and used from SGPRs; 0 = the wave stages each batch in LDS
- * and sweeps it with broadcast reads.  Results agree to FP32 rounding. */
+/* How the walk feeds interaction data to the lanes: 0 (default) = the wave
+ * stages each batch of 64 cells / particles in LDS and sweeps it with
+ * broadcast reads; 1 = each batch entry is read by scalar loads straight from
+ * the immutable cell / particle arrays and used from SGPRs.  Both evaluate the
+ * same interaction lists; forces agree to FP32 rounding. */
 int bt_set_walk_variant(bt_ctx *ctx, int variant);
 
 /* ---- initial conditions and I/O (host only, no device needed) ---------- */

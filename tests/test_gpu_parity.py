@@ -129,23 +129,23 @@ def test_interaction_counts(run):
         assert abs(a - b) <= 1e-4 * max(1.0, abs(b)), (stats, exp)
 
 
-def test_lds_staged_walk_matches(run):
-    """The LDS-staged j-data walk (variant 0) evaluates the same interaction
-    lists as the default scalar-load walk: counts equal, forces within FP32
+def test_scalar_load_walk_matches(run):
+    """The scalar-load j-data walk (variant 1) evaluates the same interaction
+    lists as the default LDS-staged walk: counts equal, forces within FP32
     rounding of each other and within the oracle bar."""
     t, ref = run["t"], run["ref"]
-    a1 = t.acc(2)
-    t.set_walk_variant(0)
+    a0 = t.acc(2)
+    t.set_walk_variant(1)
     try:
-        st0 = t.computeForces()
-        a0 = t.acc(2)
+        st1 = t.computeForces()
+        a1 = t.acc(2)
     finally:
-        t.set_walk_variant(1)
+        t.set_walk_variant(0)
         t.computeForces()
-    assert tuple(st0) == tuple(run["stats"])
-    assert rel_rms(a0[:, :3], ref["acc"][:, :3]) <= 1e-5
-    assert rel_rms(a0[:, :3], a1[:, :3]) <= 1e-5
-    assert np.array_equal(t.acc(2), a1)  # the default walk is deterministic
+    assert tuple(st1) == tuple(run["stats"])
+    assert rel_rms(a1[:, :3], ref["acc"][:, :3]) <= 1e-5
+    assert rel_rms(a1[:, :3], a0[:, :3]) <= 1e-5
+    assert np.array_equal(t.acc(2), a0)  # the default walk is deterministic
 
 
 def test_potential_matches(run):
