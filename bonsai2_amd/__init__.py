@@ -59,6 +59,16 @@ _SIGS = {
     "bt_get_timings": [P, P],
     "bt_set_walk_options": [P, I, I, I],
     "bt_set_walk_variant": [P, I],
+    "bt_mlp_init": [P, ctypes.c_uint64, F],
+    "bt_mlp_set_params": [P, P],
+    "bt_mlp_get_params": [P, P],
+    "bt_mlp_forward": [P, I, P, P],
+    "bt_mlp_grad": [P, I, P, P, P, P],
+    "bt_mlp_train": [P, I, P, P, I, I, F, ctypes.c_uint64, P],
+    "bt_learn_collect": [P, I, P, F, F, I, P, P, P, P, P, P],
+    "bt_set_opening": [P, I, F, F, F],
+    "bt_set_opening_check": [P, I, F, ctypes.c_uint64],
+    "bt_get_opening_check": [P, P],
     "bt_ic_plummer": [I, ctypes.c_uint64, D, P, P],
     "bt_ic_clustered": [I, I, ctypes.c_uint64, D, D, D, P, P],
     "bt_ic_halo_disk": [I, ctypes.c_uint64, D, D, D, P, P],
@@ -166,6 +176,79 @@ class Treecode:
     def set_walk_variant(self, variant):
         """0 = LDS-staged j-data (default), 1 = scalar-load j-data."""
         self._ck(self._L.bt_set_walk_variant(self._h, int(variant)), "set_walk_variant")
+
+    # -- learned opening angle (no reference counterpart) --------------------
+    MLP_PARAMS = 1377
+
+    def mlp_init(self, seed=1, b3=0.0):
+        self._ck(self._L.bt_mlp_init(self._h, int(seed), float(b3)), "mlp_init")
+
+    def set_mlp_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.shape != (self.MLP_PARAMS,):
+            raise ValueError("expected 1377 parameters")
+        self._ck(self._L.bt_mlp_set_params(self._h, _p(p)), "mlp_set_params")
+
+    def mlp_params(self):
+        p = np.empty(self.MLP_PARAMS, np.float32)
+        self._ck(self._L.bt_mlp_get_params(self._h, _p(p)), "mlp_get_params")
+        return p
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+        y = np.empty(len(x), np.float32)
+        self._ck(self._L.bt_mlp_forward(self._h, len(x), _p(x), _p(y)), "mlp_forward")
+        return y
+
+    def mlp_grad(self, x, t):
+        """(loss, grad[1377]) of the mean squared error on (x, t)."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.empty(self.MLP_PARAMS, np.float64)
+        loss = np.zeros(1, np.float64)
+        self._ck(self._L.bt_mlp_grad(self._h, len(x), _p(x), _p(t), _p(g), _p(loss)), "mlp_grad")
+        return float(loss[0]), g
+
+    def mlp_train(self, x, t, steps=2000, batch=8192, lr=3e-3, seed=1):
+        """Adam on the GPU; returns the per-step minibatch loss."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+        t = np.ascontiguousarray(t, np.float32)
+        curve = np.empty(int(steps), np.float32)
+        self._ck(self._L.bt_mlp_train(self._h, len(x), _p(x), _p(t), int(steps), int(batch),
+                                      float(lr), int(seed), _p(curve)), "mlp_train")
+        return curve
+
+    def learn_collect(self, gsel, theta_lo=0.25, theta_hi=1.0, max_pairs=1 << 22):
+        """Training pairs of the groups gsel -> dict(n, feat, target, err, theta_need, gc)."""
+        gsel = np.ascontiguousarray(gsel, np.int32)
+        feat = np.empty((max_pairs, 8), np.float32)
+        tg = np.empty(max_pairs, np.float32)
+        er = np.empty(max_pairs, np.float32)
+        tn = np.empty(max_pairs, np.float32)
+        gc = np.empty((max_pairs, 2), np.int32)
+        n = np.zeros(1, np.int32)
+        self._ck(self._L.bt_learn_collect(self._h, len(gsel), _p(gsel), float(theta_lo),
+                                          float(theta_hi), int(max_pairs), _p(feat), _p(tg),
+                                          _p(er), _p(tn), _p(gc), _p(n)), "learn_collect")
+        m = min(int(n[0]), max_pairs)
+        return dict(n=int(n[0]), feat=feat[:m], target=tg[:m], err=er[:m], theta_need=tn[:m],
+                    gc=gc[:m])
+
+    def set_opening(self, mode, budget=1e-4, theta_lo=0.25, theta_hi=1.0):
+        """mode 0: fixed theta; mode 1: the network's per-pair angle."""
+        self._ck(self._L.bt_set_opening(self._h, int(mode), float(budget), float(theta_lo),
+                                        float(theta_hi)), "set_opening")
+
+    def set_opening_check(self, nsamples=256, p99_max=1e-3, seed=1):
+        self._ck(self._L.bt_set_opening_check(self._h, int(nsamples), float(p99_max), int(seed)),
+                 "set_opening_check")
+
+    def opening_check(self):
+        """{p50, p99, max, fell_back, fallbacks, checks} of the last re-check."""
+        o = np.zeros(6, np.float64)
+        self._ck(self._L.bt_get_opening_check(self._h, _p(o)), "opening_check")
+        return dict(p50=o[0], p99=o[1], max=o[2], fell_back=bool(o[3]), fallbacks=int(o[4]),
+                    checks=int(o[5]))
 
     # -- particles ---------------------------------------------------------
     def ptcl_h2d(self, pos, vel):

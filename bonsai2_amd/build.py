@@ -15,7 +15,7 @@ OUT = os.path.join(HERE, "libbonsai2_amd.so")
 BUILD = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["k_sort.hip", "k_tree.hip", "k_groups.hip", "k_walk.hip", "k_misc.hip",
-               "capi.hip"]
+               "k_learn.hip", "capi.hip"]
 HOST_SOURCES = ["host_io.cpp"]
 
 ARCH = os.environ.get("BONSAI_ARCH", "gfx950")

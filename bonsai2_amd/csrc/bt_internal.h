@@ -17,6 +17,22 @@ constexpr int kLevels = 32;   // level table slots (reference int2[32], code/Tre
 constexpr int kLevelBlocks = BT_LEVEL_BLOCKS;  // fixed grid of the per-level tree kernels
 constexpr int kFp64Levels = 4;     // cells at levels <= this combine moments in FP64
 
+struct MlpDev;   // matrix-core operands of the opening-angle network (bt_learn.h)
+
+// Device buffers of one training-pair collection (bt_learn_collect).
+struct LearnBuffers {
+  int nsel = 0;
+  int *gsel = nullptr;
+  float theta_lo = 0.25f, theta_hi = 1.0f;
+  int *count = nullptr;
+  int max_pairs = 0;
+  int2 *pairs = nullptr;
+  double *tneed = nullptr;
+  double *amag = nullptr;
+  float *feat = nullptr, *target = nullptr, *err = nullptr, *tneed_f = nullptr;
+  float *sink = nullptr;
+};
+
 // Device-resident counters of one context (one allocation).
 struct DevCounters {
   int lvl_start[kLevels + 2];
@@ -101,6 +117,24 @@ struct Ctx {
   unsigned long long *d_wstats = nullptr;  // direct_sum, direct_max, approx_sum, approx_max
   double *d_wpot = nullptr;
 
+  // learned opening angle
+  float2 *sys_scale = nullptr;   // per system (root half size, total mass)
+  float *mlp_params = nullptr;   // 1377 FP32 master parameters
+  float *mlp_m = nullptr, *mlp_v = nullptr;  // Adam moments
+  MlpDev *mlp_dev = nullptr;     // FP16 per-lane operands for inference
+  bool have_mlp = false;
+  int opening = 0;               // 0 fixed theta, 1 learned per-pair theta
+  float learn_shift = 0.f, learn_theta_lo = 0.25f, learn_theta_hi = 1.0f;
+  int check_ns = 0;              // FP64 samples re-checked after each learned walk
+  float check_p99 = 1e-3f;       // above this p99 the step falls back to fixed theta
+  unsigned long long check_seed = 1;
+  unsigned long long check_calls = 0;
+  int *d_check_idx = nullptr;
+  double *d_check = nullptr;     // direct results (4 * ns) + {p50, p99, max}
+  double last_check[3] = {0, 0, 0};
+  int last_fallback = 0;
+  long long fallbacks = 0, checks = 0;
+
   // energies / direct sum scratch
   double *d_energy = nullptr;  // 2 * nsys
   double *d_direct = nullptr;
@@ -140,6 +174,18 @@ void launch_gather_ph(Ctx &c);
 void launch_groups(Ctx &c);
 
 void launch_walk(Ctx &c);
+
+void launch_mlp_prepare(Ctx &c);
+void launch_mlp_forward(Ctx &c, int n, const float *x, float *y);
+void launch_sys_scale(Ctx &c);
+void launch_learn_pairs(Ctx &c, LearnBuffers &L);
+void launch_learn_errors(Ctx &c, LearnBuffers &L, int npairs);
+void launch_learn_amag(Ctx &c, int ns, const int *d_idx, const double *d_direct, double *amag);
+int mlp_grad_blocks();
+void launch_mlp_grad(Ctx &c, const float *params, int n, const float *feat, const float *target,
+                     int batch, unsigned long long seed, int step, float *partial, double *grad);
+void launch_mlp_adam(Ctx &c, float *p, float *m, float *v, const double *grad, float lr, int step);
+void launch_learn_check(Ctx &c, int ns, const int *d_idx, const double *d_direct, double *out);
 
 void launch_kick(Ctx &c, float dt_half);
 void launch_drift(Ctx &c, float dt);

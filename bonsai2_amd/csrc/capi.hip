@@ -5,11 +5,13 @@
 // unlike the reference (process-global __device__ symbols, exit() on error,
 // code/cutil.h:772-780) all state is per context and errors are returned.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <vector>
 
 #include "bt_internal.h"
+#include "bt_learn.h"
 
 using namespace bt;
 
@@ -73,7 +75,7 @@ void free_particle_arrays(Ctx &c) {
   dev_free(&c.acc); dev_free(&c.groups); dev_free(&c.group_sys); dev_free(&c.gtmp0);
   dev_free(&c.gtmp1); dev_free(&c.gcounts); dev_free(&c.gpot);
   dev_free(&c.box); dev_free(&c.bbox_part); dev_free(&c.root_cells); dev_free(&c.d_off);
-  dev_free(&c.d_energy);
+  dev_free(&c.d_energy); dev_free(&c.sys_scale);
 }
 
 void free_cell_arrays(Ctx &c) {
@@ -136,6 +138,7 @@ int phase_build(Ctx &c) {
 
 int phase_multipoles(Ctx &c) {
   launch_multipoles(c);
+  launch_sys_scale(c);
   record(c, 4);
   BT_LAUNCH_CK();
   return 0;
@@ -158,6 +161,44 @@ int phase_forces(Ctx &c) {
   launch_walk(c);
   record(c, 8);
   BT_LAUNCH_CK();
+  return 0;
+}
+
+uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// After a learned-angle walk: relative force errors of check_ns seeded
+// samples against the FP64 direct sum; if their p99 exceeds check_p99 the
+// forces are recomputed with the fixed angle (the fallback).  Synchronises.
+int learned_check(Ctx &c) {
+  c.last_fallback = 0;
+  if (c.opening != 1 || !c.have_mlp || c.check_ns <= 0) return 0;
+  const int ns = std::min(c.check_ns, c.n);
+  std::vector<int> idx(ns);
+  for (int k = 0; k < ns; k++)
+    idx[k] = (int)(mix64(c.check_seed ^ (c.check_calls << 32) ^ (uint64_t)k) % (uint64_t)c.n);
+  c.check_calls++;
+  BT_CK(hipMemcpyAsync(c.d_check_idx, idx.data(), sizeof(int) * ns, hipMemcpyHostToDevice,
+                       c.stream));
+  double *res = c.d_check + (size_t)ns * 65 * 4;
+  launch_direct(c, ns, c.d_check_idx, c.d_check);
+  launch_learn_check(c, ns, c.d_check_idx, c.d_check, res);
+  BT_LAUNCH_CK();
+  BT_CK(hipMemcpyAsync(c.last_check, res, sizeof(double) * 3, hipMemcpyDeviceToHost, c.stream));
+  BT_CK(hipStreamSynchronize(c.stream));
+  c.checks++;
+  if (c.last_check[1] > (double)c.check_p99) {
+    c.opening = 0;
+    const int rc = phase_forces(c);
+    c.opening = 1;
+    if (rc) return rc;
+    c.last_fallback = 1;
+    c.fallbacks++;
+  }
   return 0;
 }
 
@@ -227,7 +268,9 @@ int bt_create(bt_ctx **out, float eps, float theta, int ncrit) {
   for (int k = 0; k < 12; k++) (void)hipEventCreate(&c.ev[k]);
   if (dev_alloc(c, &c.dc, 1) || dev_alloc(c, &c.level_beg, kLevels) ||
       dev_alloc(c, &c.d_wstats, 4) || dev_alloc(c, &c.d_wpot, 1) ||
-      dev_alloc(c, &c.lvl_blk, kLevelBlocks)) {
+      dev_alloc(c, &c.lvl_blk, kLevelBlocks) || dev_alloc(c, &c.mlp_params, kMlpParams) ||
+      dev_alloc(c, &c.mlp_m, kMlpParams) || dev_alloc(c, &c.mlp_v, kMlpParams) ||
+      dev_alloc(c, &c.mlp_dev, 1)) {
     delete h;
     return BT_ERR_HIP;
   }
@@ -254,6 +297,8 @@ int bt_destroy(bt_ctx *h) {
   dev_free(&c.sort_hist);
   dev_free(&c.scan_blk);
   dev_free(&c.d_direct);
+  dev_free(&c.mlp_params); dev_free(&c.mlp_m); dev_free(&c.mlp_v); dev_free(&c.mlp_dev);
+  dev_free(&c.d_check); dev_free(&c.d_check_idx);
   for (int k = 0; k < 12; k++)
     if (c.ev[k]) (void)hipEventDestroy(c.ev[k]);
   if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
@@ -340,6 +385,7 @@ int bt_alloc(bt_ctx *h, int n, int nsys, const int *offsets) {
     rc |= dev_alloc(c, &c.gtmp0, n); rc |= dev_alloc(c, &c.gtmp1, n);
     rc |= dev_alloc(c, &c.gcounts, n); rc |= dev_alloc(c, &c.gpot, n);
     rc |= dev_alloc(c, &c.box, nsys); rc |= dev_alloc(c, &c.root_cells, nsys);
+    rc |= dev_alloc(c, &c.sys_scale, nsys);
     rc |= dev_alloc(c, &c.d_off, nsys + 1);
     rc |= dev_alloc(c, &c.d_energy, 2 * (size_t)nsys * 65);
     if (rc) return BT_ERR_HIP;
@@ -438,6 +484,7 @@ int bt_compute_forces(bt_ctx *h, int intcount, double stats[4]) {
   if (phase_forces(c)) return BT_ERR_HIP;
   if (read_counters(c)) return BT_ERR_HIP;
   if (c.hc.err & 4) return fail(c, BT_ERR_STACK, "walk stack overflow");
+  if (learned_check(c)) return BT_ERR_HIP;
   c.have_acc = true;
   if (intcount && stats) {
     unsigned long long w[4];
@@ -514,6 +561,10 @@ int bt_step(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit) {
   launch_drift(c, dt);
   if (phase_build(c) || phase_multipoles(c) || phase_groups(c) || phase_forces(c))
     return BT_ERR_HIP;
+  if (c.opening == 1 && c.check_ns > 0) {
+    if (read_counters(c)) return BT_ERR_HIP;
+    if (!(c.hc.err & 7) && learned_check(c)) return BT_ERR_HIP;
+  }
   launch_kick(c, 0.5f * dt);
   record(c, 9);
   BT_LAUNCH_CK();
@@ -733,6 +784,293 @@ int bt_direct_fp64(bt_ctx *h, int ns, const int *ph_idx, double *out4) {
 int bt_get_timings(bt_ctx *h, float ms[10]) {
   if (!h || !ms) return BT_ERR_ARG;
   for (int k = 0; k < 10; k++) ms[k] = h->c.ms[k];
+  return 0;
+}
+
+
+// ---- learned opening angle ------------------------------------------------
+
+int bt_mlp_set_params(bt_ctx *h, const float *params) {
+  if (!h || !params) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  for (int i = 0; i < kMlpParams; i++)
+    if (!std::isfinite(params[i])) return fail(c, BT_ERR_ARG, "non-finite network parameter");
+  BT_CK(hipMemcpyAsync(c.mlp_params, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice,
+                       c.stream));
+  launch_mlp_prepare(c);
+  BT_LAUNCH_CK();
+  BT_CK(hipStreamSynchronize(c.stream));
+  c.have_mlp = true;
+  return 0;
+}
+
+int bt_mlp_get_params(bt_ctx *h, float *params) {
+  if (!h || !params) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  if (!c.have_mlp) return fail(c, BT_ERR_STATE, "no network parameters");
+  BT_CK(hipMemcpyAsync(params, c.mlp_params, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost,
+                       c.stream));
+  BT_CK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int bt_mlp_init(bt_ctx *h, uint64_t seed, float b3) {
+  if (!h) return BT_ERR_ARG;
+  std::vector<float> p(kMlpParams, 0.0f);
+  uint64_t st = seed;
+  auto uni = [&st]() {  // uniform in [-1, 1)
+    st = mix64(st);
+    return (float)((double)(st >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0);
+  };
+  const float g1 = std::sqrt(6.0f / (kMlpIn + kMlpH)), g2 = std::sqrt(6.0f / (2 * kMlpH)),
+              g3 = std::sqrt(6.0f / (kMlpH + 1));
+  for (int i = 0; i < kMlpIn * kMlpH; i++) p[kOffW1 + i] = g1 * uni();
+  for (int i = 0; i < kMlpH * kMlpH; i++) p[kOffW2 + i] = g2 * uni();
+  for (int i = 0; i < kMlpH; i++) p[kOffW3 + i] = g3 * uni();
+  p[kOffB3] = b3;
+  return bt_mlp_set_params(h, p.data());
+}
+
+int bt_mlp_forward(bt_ctx *h, int n, const float *x, float *y) {
+  if (!h || n <= 0 || !x || !y) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  if (!c.have_mlp) return fail(c, BT_ERR_STATE, "no network parameters");
+  float *dx = nullptr, *dy = nullptr;
+  if (dev_alloc(c, &dx, (size_t)n * 8) || dev_alloc(c, &dy, (size_t)n)) {
+    dev_free(&dx);
+    return BT_ERR_HIP;
+  }
+  int rc = 0;
+  if (hipMemcpyAsync(dx, x, sizeof(float) * 8 * (size_t)n, hipMemcpyHostToDevice, c.stream) !=
+      hipSuccess)
+    rc = BT_ERR_HIP;
+  if (!rc) {
+    launch_mlp_forward(c, n, dx, dy);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(y, dy, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c.stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(c.stream) != hipSuccess)
+      rc = BT_ERR_HIP;
+  }
+  dev_free(&dx);
+  dev_free(&dy);
+  return rc ? fail(c, rc, "network forward failed") : 0;
+}
+
+namespace {
+
+struct TrainBufs {
+  float *x = nullptr, *t = nullptr, *partial = nullptr;
+  double *grad = nullptr, *losses = nullptr;
+  void release() {
+    dev_free(&x); dev_free(&t); dev_free(&partial); dev_free(&grad); dev_free(&losses);
+  }
+};
+
+int upload_samples(Ctx &c, TrainBufs &b, int n, const float *x, const float *t, int nloss) {
+  if (dev_alloc(c, &b.x, (size_t)n * 8) || dev_alloc(c, &b.t, (size_t)n) ||
+      dev_alloc(c, &b.partial, (size_t)mlp_grad_blocks() * (kMlpParams + 1)) ||
+      dev_alloc(c, &b.grad, (size_t)kMlpParams + 1) ||
+      dev_alloc(c, &b.losses, (size_t)std::max(nloss, 1)))
+    return BT_ERR_HIP;
+  BT_CK(hipMemcpyAsync(b.x, x, sizeof(float) * 8 * (size_t)n, hipMemcpyHostToDevice, c.stream));
+  BT_CK(hipMemcpyAsync(b.t, t, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c.stream));
+  return 0;
+}
+
+}  // namespace
+
+int bt_mlp_grad(bt_ctx *h, int n, const float *x, const float *t, double *grad, double *loss) {
+  if (!h || n <= 0 || !x || !t || !grad) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  if (!c.have_mlp) return fail(c, BT_ERR_STATE, "no network parameters");
+  TrainBufs b;
+  int rc = upload_samples(c, b, n, x, t, 1);
+  if (!rc) {
+    launch_mlp_grad(c, c.mlp_params, n, b.x, b.t, n, 0ull, 0, b.partial, b.grad);
+    std::vector<double> g(kMlpParams + 1);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(g.data(), b.grad, sizeof(double) * (kMlpParams + 1), hipMemcpyDeviceToHost,
+                       c.stream) != hipSuccess ||
+        hipStreamSynchronize(c.stream) != hipSuccess) {
+      rc = BT_ERR_HIP;
+    } else {
+      std::memcpy(grad, g.data(), sizeof(double) * kMlpParams);
+      if (loss) *loss = g[kMlpParams];
+    }
+  }
+  b.release();
+  return rc ? fail(c, rc, "network gradient failed") : 0;
+}
+
+int bt_mlp_train(bt_ctx *h, int n, const float *x, const float *t, int steps, int batch, float lr,
+                 uint64_t seed, float *loss_curve) {
+  if (!h || n <= 0 || !x || !t || steps <= 0 || batch <= 0 || !(lr > 0.f)) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  if (!c.have_mlp) return fail(c, BT_ERR_STATE, "initialise or set the network first");
+  batch = std::min(batch, n);
+  TrainBufs b;
+  int rc = upload_samples(c, b, n, x, t, steps);
+  if (!rc) {
+    if (hipMemsetAsync(c.mlp_m, 0, sizeof(float) * kMlpParams, c.stream) != hipSuccess ||
+        hipMemsetAsync(c.mlp_v, 0, sizeof(float) * kMlpParams, c.stream) != hipSuccess)
+      rc = BT_ERR_HIP;
+    for (int k = 0; k < steps && !rc; k++) {
+      // cosine decay to lr / 20
+      const float lr_k =
+          lr * (0.05f + 0.95f * 0.5f * (1.0f + std::cos(3.14159265f * (float)k / (float)steps)));
+      launch_mlp_grad(c, c.mlp_params, n, b.x, b.t, batch, seed, k, b.partial, b.grad);
+      launch_mlp_adam(c, c.mlp_params, c.mlp_m, c.mlp_v, b.grad, lr_k, k + 1);
+      if (hipMemcpyAsync(b.losses + k, b.grad + kMlpParams, sizeof(double),
+                         hipMemcpyDeviceToDevice, c.stream) != hipSuccess)
+        rc = BT_ERR_HIP;
+    }
+    if (!rc) {
+      launch_mlp_prepare(c);
+      std::vector<double> l(steps);
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(l.data(), b.losses, sizeof(double) * steps, hipMemcpyDeviceToHost,
+                         c.stream) != hipSuccess ||
+          hipStreamSynchronize(c.stream) != hipSuccess)
+        rc = BT_ERR_HIP;
+      else if (loss_curve)
+        for (int k = 0; k < steps; k++) loss_curve[k] = (float)l[k];
+    }
+  }
+  b.release();
+  return rc ? fail(c, rc, "network training failed") : 0;
+}
+
+int bt_learn_collect(bt_ctx *h, int nsel, const int *gsel, float theta_lo, float theta_hi,
+                     int max_pairs, float *feat, float *target, float *err, float *theta_need,
+                     int *pair_gc, int *npairs) {
+  if (!h || nsel <= 0 || !gsel || !npairs || max_pairs <= 0) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  c.err.clear();
+  if (!(theta_lo > 0.f) || !(theta_hi > theta_lo)) return fail(c, BT_ERR_ARG, "bad theta band");
+  if (!c.have_mp || !c.have_groups) return fail(c, BT_ERR_STATE, "computeMultipoles and makeGroups first");
+  if (read_counters(c)) return BT_ERR_HIP;
+  const int ng = c.hc.ngroups;
+  std::vector<int2> grp(ng);
+  BT_CK(hipMemcpy(grp.data(), c.groups, sizeof(int2) * ng, hipMemcpyDeviceToHost));
+  std::vector<int> pidx;
+  for (int k = 0; k < nsel; k++) {
+    if (gsel[k] < 0 || gsel[k] >= ng) return fail(c, BT_ERR_ARG, "group index out of range");
+    for (int i = 0; i < grp[gsel[k]].y; i++) pidx.push_back(grp[gsel[k]].x + i);
+  }
+  if (ensure_stack(c)) return BT_ERR_HIP;
+  LearnBuffers L;
+  L.nsel = nsel;
+  L.theta_lo = theta_lo;
+  L.theta_hi = theta_hi;
+  L.max_pairs = max_pairs;
+  const int ns = (int)pidx.size();
+  double *dd = nullptr;
+  int *didx = nullptr;
+  int rc = 0;
+  rc |= dev_alloc(c, &L.gsel, nsel);
+  rc |= dev_alloc(c, &L.count, 1);
+  rc |= dev_alloc(c, &L.pairs, max_pairs);
+  rc |= dev_alloc(c, &L.tneed, max_pairs);
+  rc |= dev_alloc(c, &L.amag, (size_t)c.n + 64);
+  rc |= dev_alloc(c, &L.feat, (size_t)max_pairs * 8);
+  rc |= dev_alloc(c, &L.target, max_pairs);
+  rc |= dev_alloc(c, &L.err, max_pairs);
+  rc |= dev_alloc(c, &L.tneed_f, max_pairs);
+  rc |= dev_alloc(c, &L.sink, (size_t)c.walk_blocks * 4 * 64);
+  rc |= dev_alloc(c, &dd, (size_t)ns * 65 * 4);
+  rc |= dev_alloc(c, &didx, ns);
+  int found = 0;
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(L.gsel, gsel, sizeof(int) * nsel, hipMemcpyHostToDevice, c.stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(didx, pidx.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c.stream);
+    if (e == hipSuccess) e = hipMemsetAsync(L.count, 0, sizeof(int), c.stream);
+    if (e == hipSuccess) e = hipMemsetAsync(L.amag, 0, sizeof(double) * ((size_t)c.n + 64), c.stream);
+    if (e == hipSuccess) {
+      launch_direct(c, ns, didx, dd);
+      launch_learn_amag(c, ns, didx, dd, L.amag);
+      launch_learn_pairs(c, L);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&found, L.count, sizeof(int), hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    const int np = std::min(found, max_pairs);
+    if (e == hipSuccess && np > 0) {
+      launch_learn_errors(c, L, np);
+      e = hipGetLastError();
+      std::vector<int2> pg(np);
+      if (e == hipSuccess && feat)
+        e = hipMemcpyAsync(feat, L.feat, sizeof(float) * 8 * np, hipMemcpyDeviceToHost, c.stream);
+      if (e == hipSuccess && target)
+        e = hipMemcpyAsync(target, L.target, sizeof(float) * np, hipMemcpyDeviceToHost, c.stream);
+      if (e == hipSuccess && err)
+        e = hipMemcpyAsync(err, L.err, sizeof(float) * np, hipMemcpyDeviceToHost, c.stream);
+      if (e == hipSuccess && theta_need)
+        e = hipMemcpyAsync(theta_need, L.tneed_f, sizeof(float) * np, hipMemcpyDeviceToHost,
+                           c.stream);
+      if (e == hipSuccess && pair_gc)
+        e = hipMemcpyAsync(pair_gc, L.pairs, sizeof(int2) * np, hipMemcpyDeviceToHost, c.stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    }
+    if (e != hipSuccess) {
+      c.err = std::string("learn_collect: ") + hipGetErrorString(e);
+      rc = BT_ERR_HIP;
+    }
+  }
+  dev_free(&L.gsel); dev_free(&L.count); dev_free(&L.pairs); dev_free(&L.tneed);
+  dev_free(&L.amag); dev_free(&L.feat); dev_free(&L.target); dev_free(&L.err);
+  dev_free(&L.tneed_f); dev_free(&L.sink); dev_free(&dd); dev_free(&didx);
+  if (rc) return rc;
+  if (read_counters(c)) return BT_ERR_HIP;
+  if (c.hc.err & 4) return fail(c, BT_ERR_STACK, "traversal stack overflow");
+  *npairs = found;
+  return 0;
+}
+
+int bt_set_opening(bt_ctx *h, int mode, float budget, float theta_lo, float theta_hi) {
+  if (!h) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  if (mode != 0 && mode != 1) return fail(c, BT_ERR_ARG, "opening mode must be 0 or 1");
+  if (mode == 1) {
+    if (!c.have_mlp) return fail(c, BT_ERR_STATE, "no network parameters");
+    if (!(budget > 0.f) || !(theta_lo > 0.f) || !(theta_hi > theta_lo))
+      return fail(c, BT_ERR_ARG, "bad budget or theta band");
+    c.learn_shift = std::log2(budget) / 3.0f;
+    c.learn_theta_lo = theta_lo;
+    c.learn_theta_hi = theta_hi;
+  }
+  c.opening = mode;
+  return 0;
+}
+
+int bt_set_opening_check(bt_ctx *h, int nsamples, float p99_max, uint64_t seed) {
+  if (!h) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  if (nsamples < 0 || nsamples > 1024 || !(p99_max > 0.f))
+    return fail(c, BT_ERR_ARG, "samples must be in [0, 1024], p99_max > 0");
+  if (nsamples > 0) {
+    if (dev_alloc(c, &c.d_check, (size_t)nsamples * 65 * 4 + 4) ||
+        dev_alloc(c, &c.d_check_idx, nsamples))
+      return BT_ERR_HIP;
+  }
+  c.check_ns = nsamples;
+  c.check_p99 = p99_max;
+  c.check_seed = seed;
+  c.check_calls = 0;
+  c.fallbacks = c.checks = 0;
+  return 0;
+}
+
+int bt_get_opening_check(bt_ctx *h, double out[6]) {
+  if (!h || !out) return BT_ERR_ARG;
+  const Ctx &c = h->c;
+  out[0] = c.last_check[0];
+  out[1] = c.last_check[1];
+  out[2] = c.last_check[2];
+  out[3] = c.last_fallback;
+  out[4] = (double)c.fallbacks;
+  out[5] = (double)c.checks;
   return 0;
 }
 

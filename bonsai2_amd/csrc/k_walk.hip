@@ -26,6 +26,7 @@
 // rearranged algebraically with the cell's half trace precomputed.
 #include "bt_device.h"
 #include "bt_internal.h"
+#include "bt_learn.h"
 
 namespace bt {
 
@@ -60,6 +61,11 @@ struct WalkArgs {
   const float4 *cq0;
   const float2 *cq1;
   const float4 *cwalk;   // 3 per cell: (com, M) (3Qxx,3Qyy,3Qzz,3Qxy) (3Qxz,3Qyz,trQ/2,1.5trQ)
+  // learned opening angle
+  const float2 *cls;     // (l, s) per cell
+  const float2 *sys_scale;
+  const MlpDev *mlp;
+  float shift, it_min, it_max;
   float4 *acc;
   int2 *gcounts;
   double *gpot;
@@ -293,14 +299,29 @@ __device__ __forceinline__ void batch_p2p(const WalkArgs &a, WaveShared &S, int 
   else eval_p2p<KAHAN>(a, S, cnt, gcx, gcy, gcz, xi, yi, zi, s, c);
 }
 
-template <bool KAHAN, bool SJ>
+// LEARN: each popped cell gets its own opening angle from the network
+// (bt_learn.h), evaluated by the wave on the matrix cores for up to 64 cells.
+template <bool KAHAN, bool SJ, bool LEARN>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BT_WALK_WAVES_PER_EU)))
 void k_walk(WalkArgs a) {
   __shared__ WaveShared sh[kWavesPerBlock];
+  __shared__ float s_mlp[LEARN ? 64 : 1];
   const int lane = lane_id();
   const int wv = threadIdx.x >> 6;
   WaveShared &S = sh[wv];
   int *stk = a.stack + (size_t)(blockIdx.x * kWavesPerBlock + wv) * (size_t)a.stack_ints;
+  half8_t ma1, ma2_0, ma2_1;
+  float mb3 = 0.f;
+  if (LEARN) {
+    if (threadIdx.x < 32) s_mlp[threadIdx.x] = a.mlp->c2p[threadIdx.x >> 4][threadIdx.x & 15];
+    else if (threadIdx.x < 64)
+      s_mlp[threadIdx.x] = a.mlp->w3p[(threadIdx.x - 32) >> 4][threadIdx.x & 15];
+    ma1 = a.mlp->a1[lane];
+    ma2_0 = a.mlp->a2[0][lane];
+    ma2_1 = a.mlp->a2[1][lane];
+    mb3 = a.mlp->b3;
+    __syncthreads();
+  }
   const int ngroups = a.dc->ngroups;
   // a failed build (cell or group capacity) leaves placeholder links: do not walk it
   if (a.dc->err & 3) return;
@@ -329,6 +350,8 @@ void k_walk(WalkArgs a) {
       gcx = 0.5f * (mxx + mnx); gcy = 0.5f * (mxy + mny); gcz = 0.5f * (mxz + mnz);
       gsx = 0.5f * (mxx - mnx); gsy = 0.5f * (mxy - mny); gsz = 0.5f * (mxz - mnz);
     }
+    GroupGeom geom;
+    if (LEARN) geom = make_group_geom(gcx, gcy, gcz, gsx, gsy, gsz, np, a.sys_scale[sys]);
     // i-particle: relative to the group centre (LDS path) or absolute (scalar path)
     const float xi = SJ ? pi.x : pi.x - gcx, yi = SJ ? pi.y : pi.y - gcy,
                 zi = SJ ? pi.z : pi.z - gcz;
@@ -362,7 +385,32 @@ void k_walk(WalkArgs a) {
         cs = a.csize[ci];
       }
       bool split = false, leaf = false;
-      if (use) {
+      if (LEARN) {
+        // every lane evaluates the network (idle lanes on a zero cell)
+        float4 m = make_float4(0.f, 0.f, 0.f, 0.f), q0 = m;
+        float2 q1 = make_float2(0.f, 0.f), lsv = q1;
+        if (use) {
+          m = a.cmono[ci];
+          q0 = a.cq0[ci];
+          q1 = a.cq1[ci];
+          lsv = a.cls[ci];
+        }
+        float x[8];
+        pair_features(geom, m, lsv.x, q0, q1, x);
+        const float y = mlp_wave(ma1, ma2_0, ma2_1, s_mlp, mb3, pack_features(x), take > 32);
+        const float it = learned_inv_theta(y, a.shift, a.it_min, a.it_max);
+        float op;
+        {
+#pragma clang fp contract(off)
+          op = lsv.x * it + lsv.y;
+          op = op * op;
+        }
+        if (use) {
+          split = open_cell(make_float4(cs.x, cs.y, cs.z, op), gcx, gcy, gcz, gsx, gsy, gsz) ||
+                  ((int)cd.w - (int)cd.z < 3);
+          leaf = cell_leaf(cd);
+        }
+      } else if (use) {
         split = open_cell(cs, gcx, gcy, gcz, gsx, gsy, gsz) || ((int)cd.w - (int)cd.z < 3);
         leaf = cell_leaf(cd);
       }
@@ -523,6 +571,12 @@ void launch_walk(Ctx &c) {
   a.cq0 = c.cq0;
   a.cq1 = c.cq1;
   a.cwalk = c.cwalk;
+  a.cls = c.cls;
+  a.sys_scale = c.sys_scale;
+  a.mlp = c.mlp_dev;
+  a.shift = c.learn_shift;
+  a.it_min = 1.0f / c.learn_theta_hi;
+  a.it_max = 1.0f / c.learn_theta_lo;
   a.acc = c.acc;
   a.gcounts = c.gcounts;
   a.gpot = c.gpot;
@@ -531,8 +585,12 @@ void launch_walk(Ctx &c) {
   a.dc = c.dc;
   hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, c.stream, c.dc);
   void (*kern)(WalkArgs);
-  if (c.walk_variant == 1) kern = c.kahan ? k_walk<true, true> : k_walk<false, true>;
-  else kern = c.kahan ? k_walk<true, false> : k_walk<false, false>;
+  if (c.opening == 1 && c.have_mlp)
+    kern = c.kahan ? k_walk<true, false, true> : k_walk<false, false, true>;
+  else if (c.walk_variant == 1)
+    kern = c.kahan ? k_walk<true, true, false> : k_walk<false, true, false>;
+  else
+    kern = c.kahan ? k_walk<true, false, false> : k_walk<false, false, false>;
   hipLaunchKernelGGL(kern, dim3(c.walk_blocks), dim3(kBlock), 0, c.stream, a);
   hipLaunchKernelGGL(k_walk_stats, dim3(1), dim3(1024), 0, c.stream, c.groups, c.gcounts, c.gpot,
                      c.dc, c.d_wstats, c.d_wpot);

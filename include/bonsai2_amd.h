@@ -159,6 +159,51 @@ int bt_set_walk_options(bt_ctx *ctx, int kahan, int stack_ints, int blocks_per_c
  * same interaction lists; forces agree to FP32 rounding. */
 int bt_set_walk_variant(bt_ctx *ctx, int variant);
 
+/* ---- learned opening angle (no reference counterpart) --------------------
+ * The reference opens every cell with one angle theta (code/computeForces.cu
+ * :30-54).  With opening mode 1 a network picks the angle of each
+ * (group, candidate cell) pair from 8 geometric features:
+ *   theta = clamp(exp2(y + log2(budget) / 3), theta_lo, theta_hi),
+ * and the cell is opened iff ds^2 < (l / theta + s)^2 (or it holds < 3
+ * particles).  The network is 8 -> 32 (SiLU) -> 32 (SiLU) -> 1 with FP16
+ * parameters and FP32 accumulation; its 1377 parameters are, in order,
+ * W1[32][8], b1[32], W2[32][32], b2[32], w3[32], b3.  It is trained on the
+ * GPU on pairs the method generates: target = log2(theta_need) -
+ * log2(e) / 3, where theta_need = l / (ds - s) is the pair's acceptance
+ * boundary and e the pair's largest relative P2C error against the FP64
+ * direct force of the cell's particles; accepting a pair iff its predicted
+ * e <= budget is then the same as the angle rule above. */
+int bt_mlp_init(bt_ctx *ctx, uint64_t seed, float b3);  /* Xavier-uniform weights, b3 */
+int bt_mlp_set_params(bt_ctx *ctx, const float *params);  /* 1377 floats, stored FP16 */
+int bt_mlp_get_params(bt_ctx *ctx, float *params);        /* the FP32 master copy */
+/* y[k] = network(x[8k .. 8k+7]) by the walk's matrix-core routine. */
+int bt_mlp_forward(bt_ctx *ctx, int n, const float *x, float *y);
+/* mean squared error of the network on (x, t) and its gradient (FP64 out). */
+int bt_mlp_grad(bt_ctx *ctx, int n, const float *x, const float *t, double *grad,
+                double *loss);
+/* Adam on (x, t): steps minibatches of batch samples, cosine-decayed lr;
+ * loss_curve (may be NULL) gets the minibatch loss of every step. */
+int bt_mlp_train(bt_ctx *ctx, int n, const float *x, const float *t, int steps, int batch,
+                 float lr, uint64_t seed, float *loss_curve);
+/* Training pairs of the groups gsel[0..nsel) (indices into bt_get_groups)
+ * from the current tree: every cell the walk visits at theta_lo whose
+ * theta_need lies in [theta_lo, theta_hi] and that holds >= 3 particles.
+ * Writes up to max_pairs rows of feat[8], target, err, theta_need and
+ * pair_gc[2] = (group, cell) (any output may be NULL); *npairs = pairs
+ * found (may exceed max_pairs). */
+int bt_learn_collect(bt_ctx *ctx, int nsel, const int *gsel, float theta_lo, float theta_hi,
+                     int max_pairs, float *feat, float *target, float *err, float *theta_need,
+                     int *pair_gc, int *npairs);
+/* mode 0: fixed theta (default); mode 1: the network's per-pair angle. */
+int bt_set_opening(bt_ctx *ctx, int mode, float budget, float theta_lo, float theta_hi);
+/* After each learned-angle walk, nsamples (<= 1024; 0 = off) seeded particles
+ * are re-checked against the FP64 direct sum; if their p99 relative force
+ * error exceeds p99_max the forces are recomputed with the fixed angle. */
+int bt_set_opening_check(bt_ctx *ctx, int nsamples, float p99_max, uint64_t seed);
+/* {p50, p99, max of the last check, last call fell back (0/1), fallbacks,
+ * checks} */
+int bt_get_opening_check(bt_ctx *ctx, double out[6]);
+
 /* ---- initial conditions and I/O (host only, no device needed) ---------- */
 
 /* Plummer sphere (Aarseth-Henon-Wielen, code/plummer.h:37-117 algorithm)

@@ -21,6 +21,14 @@
 //   or_plummer_ref       bit-exact against the reference's own plummer.h
 //                        compiled into oracle/_ref (when that builds)
 //   or_energies          closed forms (two-body)
+//   or_f16_round         numpy float16 conversion (round to nearest even)
+//   or_pair_features     closed forms on hand-built pairs (tests)
+//   or_mlp_forward/grad  finite differences of the forward pass in FP64;
+//                        a hand-computed 8-32-32-1 case with one-hot weights
+//   or_walk (learned)    a network that outputs a constant log2(theta)
+//                        reproduces the fixed-theta walk exactly
+//   or_learn_pairs       brute-force pair error on a 2-cell toy; theta_need
+//                        is the exact acceptance boundary of the walk's test
 // Every function has at least one such check; none is "parity unpinned".
 #include <algorithm>
 #include <cmath>
@@ -492,6 +500,149 @@ int or_make_groups(int n, const uint64_t *sorted_keys, const int *seg_id,
 // ---------------------------------------------------------------------------
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// Learned opening angle (no reference counterpart).  A per-(group, cell)
+// network y = MLP(x) picks the opening angle of each candidate cell:
+// theta = exp2(y + shift), clamped to [theta_lo, theta_hi]; the cell is then
+// opened iff ds^2 < (l / theta + s)^2, the improved Barnes-Hut test of
+// code/computeForces.cu:30-54 with a per-pair angle (or if it holds < 3
+// particles, :284-285).
+//
+// Features (8, all in [0, 1], IEEE +,-,*,/,sqrt only so they are exact on
+// any IEEE machine): with o = com - group centre, d = |o| (>= 1e-30):
+//   x0 = rg / (rg + d)         rg = |group half-size vector|   group radius
+//   x1 = np / 64                                              group count
+//   x2 = l / (l + d)           l = cell bbox size              cell size
+//   x3 = d / (d + R)           R = root box half size          distance
+//   x4 = min(M / M_sys, 1)                                     mass fraction
+//   x5 = |Q|_F / (|Q|_F + M l^2 + 1e-30)                       quadrupole ratio
+//   x6 = |o_z| / d, x7 = |o_k| / d (k = longest group axis)    direction cosines
+// Network: 8 -> 32 (SiLU) -> 32 (SiLU) -> 1, weights and biases stored in
+// FP16, FP32 accumulation; the input and the first hidden layer are rounded
+// to FP16 (they feed FP16 matrix products), the second hidden layer is used
+// in FP32 by the output dot product.  Parameter order (1377 floats):
+//   W1[32][8], b1[32], W2[32][32], b2[32], w3[32], b3.
+// ---------------------------------------------------------------------------
+constexpr int kMlpIn = 8, kMlpH = 32;
+constexpr int kOffW1 = 0, kOffB1 = 256, kOffW2 = 288, kOffB2 = 1312, kOffW3 = 1344,
+              kOffB3 = 1376, kMlpParams = 1377;
+
+inline uint16_t f32_to_f16_bits(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  uint32_t mant = x & 0x7FFFFFu;
+  const int exp = (int)((x >> 23) & 0xFF);
+  if (exp == 0xFF) return (uint16_t)(sign | 0x7C00u | (mant ? 0x200u : 0u));
+  const int e = exp - 127 + 15;
+  if (e >= 31) return (uint16_t)(sign | 0x7C00u);
+  if (e <= 0) {
+    if (e < -10) return (uint16_t)sign;
+    mant |= 0x800000u;
+    const int shift = 14 - e;
+    uint32_t hm = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1);
+    if (rem > half || (rem == half && (hm & 1u))) hm++;
+    return (uint16_t)(sign | hm);
+  }
+  uint32_t h = sign | ((uint32_t)e << 10) | (mant >> 13);
+  const uint32_t rem = mant & 0x1FFFu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+  return (uint16_t)h;
+}
+
+inline float f16_bits_to_f32(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  const int e = (h >> 10) & 0x1F;
+  uint32_t m = h & 0x3FFu, x;
+  if (e == 0) {
+    if (m == 0) {
+      x = sign;
+    } else {  // subnormal: normalise
+      int ee = -1;
+      do { m <<= 1; ee++; } while (!(m & 0x400u));
+      x = sign | ((uint32_t)(127 - 15 - ee) << 23) | ((m & 0x3FFu) << 13);
+    }
+  } else if (e == 31) {
+    x = sign | 0x7F800000u | (m << 13);
+  } else {
+    x = sign | ((uint32_t)(e - 15 + 127) << 23) | (m << 13);
+  }
+  float f;
+  std::memcpy(&f, &x, 4);
+  return f;
+}
+
+inline float f16r(float x) { return f16_bits_to_f32(f32_to_f16_bits(x)); }
+
+struct PairIn {
+  float gcx, gcy, gcz, gsx, gsy, gsz;
+  int np;
+  float cx, cy, cz, M, l;
+  float qxx, qyy, qzz, qxy, qxz, qyz;
+  float R, Msys;
+};
+
+inline void pair_features(const PairIn &p, float x[8]) {
+  const float ox = p.cx - p.gcx, oy = p.cy - p.gcy, oz = p.cz - p.gcz;
+  const float d = std::fmax(std::sqrt(ox * ox + oy * oy + oz * oz), 1e-30f);
+  const float rg = std::sqrt(p.gsx * p.gsx + p.gsy * p.gsy + p.gsz * p.gsz);
+  x[0] = rg / (rg + d);
+  x[1] = (float)p.np * 0.015625f;
+  x[2] = p.l / (p.l + d);
+  x[3] = d / (d + p.R);
+  x[4] = std::fmin(p.M / p.Msys, 1.0f);
+  const float qn = std::sqrt(p.qxx * p.qxx + p.qyy * p.qyy + p.qzz * p.qzz +
+                             2.0f * (p.qxy * p.qxy + p.qxz * p.qxz + p.qyz * p.qyz));
+  x[5] = qn / (qn + p.M * p.l * p.l + 1e-30f);
+  x[6] = std::fabs(oz) / d;
+  const float ok = (p.gsx >= p.gsy && p.gsx >= p.gsz) ? ox : (p.gsy >= p.gsz ? oy : oz);
+  x[7] = std::fabs(ok) / d;
+}
+
+inline double silu_d(double z) { return z / (1.0 + std::exp(-z)); }
+
+// Forward pass on FP16-rounded parameters pr[] (see above).  Optional caches
+// for the backward pass: xh[8], z1[32], a1h[32], z2[32], a2[32].
+inline double mlp_forward_one(const float *pr, const float *x, float *xh_c, double *z1_c,
+                              float *a1h_c, double *z2_c, float *a2_c, bool round16) {
+  float xh[kMlpIn], a1h[kMlpH], a2[kMlpH];
+  for (int k = 0; k < kMlpIn; k++) xh[k] = round16 ? f16r(x[k]) : x[k];
+  for (int h = 0; h < kMlpH; h++) {
+    double z = pr[kOffB1 + h];
+    for (int k = 0; k < kMlpIn; k++) z += (double)pr[kOffW1 + h * kMlpIn + k] * xh[k];
+    const float a = (float)silu_d((double)(float)z);
+    a1h[h] = round16 ? f16r(a) : a;
+    if (z1_c) z1_c[h] = (double)(float)z;
+  }
+  for (int h = 0; h < kMlpH; h++) {
+    double z = pr[kOffB2 + h];
+    for (int k = 0; k < kMlpH; k++) z += (double)pr[kOffW2 + h * kMlpH + k] * a1h[k];
+    a2[h] = (float)silu_d((double)(float)z);
+    if (z2_c) z2_c[h] = (double)(float)z;
+  }
+  double y = pr[kOffB3];
+  for (int h = 0; h < kMlpH; h++) y += (double)pr[kOffW3 + h] * a2[h];
+  if (xh_c) for (int k = 0; k < kMlpIn; k++) xh_c[k] = xh[k];
+  if (a1h_c) for (int h = 0; h < kMlpH; h++) a1h_c[h] = a1h[h];
+  if (a2_c) for (int h = 0; h < kMlpH; h++) a2_c[h] = a2[h];
+  return y;
+}
+
+struct LearnCfg {
+  const float *pr;     // FP16-rounded parameters
+  float shift;         // log2(theta) offset (error budget)
+  float theta_lo, theta_hi;
+  const F2 *group_scale;  // per group (R, M_sys)
+};
+
+// 1 / theta for one pair, clamped to [1/theta_hi, 1/theta_lo]
+inline float learned_inv_theta(const LearnCfg &lc, const float x[8]) {
+  const double y = mlp_forward_one(lc.pr, x, nullptr, nullptr, nullptr, nullptr, nullptr, true);
+  const float it = std::exp2(-((float)y + lc.shift));
+  return std::fmin(std::fmax(it, 1.0f / lc.theta_hi), 1.0f / lc.theta_lo);
+}
+
 template <typename A>
 static void p2p(A acc[4], float px, float py, float pz, const F4 &j, float eps2) {
   const float dx = j.x - px, dy = j.y - py, dz = j.z - pz;
@@ -551,7 +702,8 @@ static void walk_impl(int ngroups, const I2 *groups, const int *group_cells,
                       const U4 *cells, const F4 *csize, const F4 *mono, const F4 *quad0,
                       const F2 *quad1, float eps2, const F2 *cell_ls,
                       const float *group_inv_theta, F4 *acc_out,
-                      long long *stats_out, double *pot_out, int *counts_out) {
+                      long long *stats_out, double *pot_out, int *counts_out,
+                      const LearnCfg *lc = nullptr) {
   long long dsum = 0, asum = 0, dmax = 0, amax = 0;
   double pot = 0;
 #pragma omp parallel reduction(+ : dsum, asum, pot) reduction(max : dmax, amax)
@@ -582,9 +734,23 @@ static void walk_impl(int ngroups, const I2 *groups, const int *group_cells,
         nxt.clear();
         for (int c : cur) {
           const U4 cd = cells[c];
-          const bool split = open_test(csize[c], gcx, gcy, gcz, gsx, gsy, gsz,
-                                       lsp ? lsp + c : nullptr, it) ||
-                             ((int)cd.w - (int)cd.z < 3);
+          bool split;
+          if (lc) {
+            const F4 &m = mono[c];
+            const PairIn pin{gcx, gcy, gcz, gsx, gsy, gsz, np, m.x, m.y, m.z, m.w,
+                             cell_ls[c].x, quad0[c].x, quad0[c].y, quad0[c].z, quad0[c].w,
+                             quad1[c].x, quad1[c].y, lc->group_scale[g].x,
+                             lc->group_scale[g].y};
+            float xf[8];
+            pair_features(pin, xf);
+            split = open_test(csize[c], gcx, gcy, gcz, gsx, gsy, gsz, cell_ls + c,
+                              learned_inv_theta(*lc, xf)) ||
+                    ((int)cd.w - (int)cd.z < 3);
+          } else {
+            split = open_test(csize[c], gcx, gcy, gcz, gsx, gsy, gsz,
+                              lsp ? lsp + c : nullptr, it) ||
+                    ((int)cd.w - (int)cd.z < 3);
+          }
           const bool leaf = cell_is_leaf(cd);
           if (!split) {
             alist.push_back(c);
@@ -735,6 +901,200 @@ void or_plummer_ref(long n, unsigned seed, double rmax, double *mass, double *po
   for (int d = 0; d < 3; d++) { xc[d] /= mc; vc[d] /= mc; }
   for (long k = 0; k < n; k++)
     for (int d = 0; d < 3; d++) { pos[3 * k + d] -= xc[d]; vel[3 * k + d] -= vc[d]; }
+}
+
+
+// ---------------------------------------------------------------------------
+// Learned opening angle: primitives exported for the tests (see the block
+// above walk_impl for the definitions).
+// ---------------------------------------------------------------------------
+void or_f16_round(int n, const float *in, float *out) {
+  for (int i = 0; i < n; i++) out[i] = f16r(in[i]);
+}
+
+// pin[20*k ..]: gc.xyz, gs.xyz, np, com.xyz, M, l, Qxx, Qyy, Qzz, Qxy, Qxz, Qyz, R, M_sys
+void or_pair_features(int n, const float *pin, float *feat) {
+  for (int k = 0; k < n; k++) {
+    const float *q = pin + 20 * k;
+    const PairIn p{q[0], q[1], q[2], q[3], q[4], q[5], (int)q[6], q[7], q[8], q[9], q[10],
+                   q[11], q[12], q[13], q[14], q[15], q[16], q[17], q[18], q[19]};
+    pair_features(p, feat + 8 * k);
+  }
+}
+
+// y[k] = MLP(x[8k..]) with FP16-rounded parameters (round16 = 1) or raw
+// FP32 parameters and no activation rounding (round16 = 0).
+void or_mlp_forward(int n, const float *params, const float *x, int round16, float *y) {
+  std::vector<float> pr(params, params + kMlpParams);
+  if (round16) for (auto &v : pr) v = f16r(v);
+  for (int k = 0; k < n; k++)
+    y[k] = (float)mlp_forward_one(pr.data(), x + 8 * k, nullptr, nullptr, nullptr, nullptr,
+                                  nullptr, round16 != 0);
+}
+
+// Mean squared error L = 1/n sum (y - t)^2 and dL/dparams (FP64), the
+// rounding to FP16 treated as the identity in the backward pass.
+void or_mlp_grad(int n, const float *params, const float *x, const float *t, int round16,
+                 double *grad, double *loss) {
+  std::vector<float> pr(params, params + kMlpParams);
+  if (round16) for (auto &v : pr) v = f16r(v);
+  for (int i = 0; i < kMlpParams; i++) grad[i] = 0.0;
+  double L = 0.0;
+  for (int k = 0; k < n; k++) {
+    float xh[kMlpIn], a1h[kMlpH], a2[kMlpH];
+    double z1[kMlpH], z2[kMlpH];
+    const double y = mlp_forward_one(pr.data(), x + 8 * k, xh, z1, a1h, z2, a2, round16 != 0);
+    const double r = y - (double)t[k];
+    L += r * r;
+    const double g = 2.0 * r / n;
+    grad[kOffB3] += g;
+    double dz2[kMlpH];
+    for (int h = 0; h < kMlpH; h++) {
+      grad[kOffW3 + h] += g * a2[h];
+      const double sg = 1.0 / (1.0 + std::exp(-z2[h]));
+      dz2[h] = g * pr[kOffW3 + h] * sg * (1.0 + z2[h] * (1.0 - sg));
+      grad[kOffB2 + h] += dz2[h];
+      for (int j = 0; j < kMlpH; j++) grad[kOffW2 + h * kMlpH + j] += dz2[h] * a1h[j];
+    }
+    for (int j = 0; j < kMlpH; j++) {
+      double da = 0.0;
+      for (int h = 0; h < kMlpH; h++) da += dz2[h] * pr[kOffW2 + h * kMlpH + j];
+      const double sg = 1.0 / (1.0 + std::exp(-z1[j]));
+      const double dz1 = da * sg * (1.0 + z1[j] * (1.0 - sg));
+      grad[kOffB1 + j] += dz1;
+      for (int f = 0; f < kMlpIn; f++) grad[kOffW1 + j * kMlpIn + f] += dz1 * xh[f];
+    }
+  }
+  *loss = L / n;
+}
+
+// The walk with the learned per-pair opening angle (FP32 accumulation).
+// params: raw FP32 (rounded to FP16 here); group_scale[g] = (R, M_sys).
+void or_walk_learned(int ngroups, const I2 *groups, const int *group_cells,
+                     const int *level_range, const F4 *pos_ph, const F4 *pos_tree,
+                     const U4 *cells, const F4 *csize, const F4 *mono, const F4 *quad0,
+                     const F2 *quad1, const F2 *cell_ls, float eps2, const float *params,
+                     float shift, float theta_lo, float theta_hi, const F2 *group_scale,
+                     F4 *acc_out, long long *stats_out, double *pot_out, int *counts_out) {
+  std::vector<float> pr(params, params + kMlpParams);
+  for (auto &v : pr) v = f16r(v);
+  const LearnCfg lc{pr.data(), shift, theta_lo, theta_hi, group_scale};
+  walk_impl<float>(ngroups, groups, group_cells, level_range, pos_ph, pos_tree, cells, csize,
+                   mono, quad0, quad1, eps2, cell_ls, nullptr, acc_out, stats_out, pot_out,
+                   counts_out, &lc);
+}
+
+// Training pairs for the groups gsel[0..nsel): every cell the walk visits at
+// theta_lo (a superset of what any angle in [theta_lo, theta_hi] visits)
+// whose acceptance boundary theta_need = l / (ds - s) lies in
+// [theta_lo, theta_hi] and that holds >= 3 particles.  For each: the
+// features, theta_need, the pair error
+//   e = max_i |a_P2C(i) - a_direct(i)| / amag[i]     (FP64, softened)
+// over the group's particles i (amag = |a_i| of the FP64 direct sum, PH
+// order), and the target log2(theta_need) - log2(max(e, 1e-12)) / 3.
+// Groups in gsel order, cells breadth-first.  Returns the pair count (only
+// the first max_pairs are written).
+int or_learn_pairs(int nsel, const int *gsel, const I2 *groups, const int *group_cells,
+                   const int *level_range, const F4 *pos_ph, const F4 *pos_tree,
+                   const U4 *cells, const F4 *csize, const F4 *mono, const F4 *quad0,
+                   const F2 *quad1, const F2 *cell_ls, const F2 *group_scale, float theta_lo,
+                   float theta_hi, float eps2, const double *amag, int max_pairs, int *pair_gc,
+                   float *feat, float *theta_need, float *target, float *err) {
+  int np_out = 0;
+  const float it_lo = 1.0f / theta_lo;
+  std::vector<int> cur, nxt;
+  for (int si = 0; si < nsel; si++) {
+    const int g = gsel[si];
+    const int pbeg = groups[g].x, np = groups[g].y;
+    float mn[3] = {pos_ph[pbeg].x, pos_ph[pbeg].y, pos_ph[pbeg].z};
+    float mx[3] = {mn[0], mn[1], mn[2]};
+    for (int i = pbeg; i < pbeg + np; i++) {
+      mn[0] = std::min(mn[0], pos_ph[i].x); mx[0] = std::max(mx[0], pos_ph[i].x);
+      mn[1] = std::min(mn[1], pos_ph[i].y); mx[1] = std::max(mx[1], pos_ph[i].y);
+      mn[2] = std::min(mn[2], pos_ph[i].z); mx[2] = std::max(mx[2], pos_ph[i].z);
+    }
+    const float gcx = 0.5f * (mx[0] + mn[0]), gcy = 0.5f * (mx[1] + mn[1]),
+                gcz = 0.5f * (mx[2] + mn[2]);
+    const float gsx = 0.5f * (mx[0] - mn[0]), gsy = 0.5f * (mx[1] - mn[1]),
+                gsz = 0.5f * (mx[2] - mn[2]);
+    int c0, nc;
+    if (group_cells) { c0 = group_cells[2 * g]; nc = group_cells[2 * g + 1]; }
+    else { c0 = level_range[2]; nc = level_range[3] - level_range[2] + 1; }
+    cur.clear();
+    for (int c = c0; c < c0 + nc; c++) cur.push_back(c);
+    while (!cur.empty()) {
+      nxt.clear();
+      for (int c : cur) {
+        const U4 cd = cells[c];
+        const int nc_p = (int)cd.w - (int)cd.z;
+        const F4 &cs = csize[c];
+        float dx = std::fabs(gcx - cs.x) - gsx, dy = std::fabs(gcy - cs.y) - gsy,
+              dz = std::fabs(gcz - cs.z) - gsz;
+        dx += std::fabs(dx); dx *= 0.5f;
+        dy += std::fabs(dy); dy *= 0.5f;
+        dz += std::fabs(dz); dz *= 0.5f;
+        const float ds2 = dx * dx + dy * dy + dz * dz;
+        const float l = cell_ls[c].x, sc = cell_ls[c].y;
+        const double ds = std::sqrt((double)ds2);
+        if (nc_p >= 3 && ds > (double)sc) {
+          const double tn = (double)l / (ds - (double)sc);
+          if (tn >= (double)theta_lo && tn <= (double)theta_hi) {
+            if (np_out < max_pairs) {
+              const F4 &m = mono[c];
+              const PairIn pin{gcx, gcy, gcz, gsx, gsy, gsz, np, m.x, m.y, m.z, m.w, l,
+                               quad0[c].x, quad0[c].y, quad0[c].z, quad0[c].w, quad1[c].x,
+                               quad1[c].y, group_scale[g].x, group_scale[g].y};
+              pair_features(pin, feat + 8 * np_out);
+              double e = 0.0;
+              for (int i = pbeg; i < pbeg + np; i++) {
+                const double px = pos_ph[i].x, py = pos_ph[i].y, pz = pos_ph[i].z;
+                double ax = 0, ay = 0, az = 0;
+                for (int j = (int)cd.z; j < (int)cd.w; j++) {
+                  const double rx = pos_tree[j].x - px, ry = pos_tree[j].y - py,
+                               rz = pos_tree[j].z - pz;
+                  const double r2 = rx * rx + ry * ry + rz * rz + (double)eps2;
+                  const double ri = 1.0 / std::sqrt(r2);
+                  const double w = pos_tree[j].w * ri * ri * ri;
+                  ax += w * rx; ay += w * ry; az += w * rz;
+                }
+                // quadrupole P2C in FP64 (same terms as the walk's p2c)
+                const double qx_ = px - m.x, qy_ = py - m.y, qz_ = pz - m.z;
+                const double r2 = qx_ * qx_ + qy_ * qy_ + qz_ * qz_ + (double)eps2;
+                const double ri = 1.0 / std::sqrt(r2), ri2 = ri * ri;
+                const double m1 = m.w * ri, m3 = m1 * ri2, m5 = m3 * ri2, m7 = m5 * ri2;
+                const double Qxx = quad0[c].x, Qyy = quad0[c].y, Qzz = quad0[c].z,
+                             Qxy = quad0[c].w, Qxz = quad1[c].x, Qyz = quad1[c].y;
+                const double tq = Qxx + Qyy + Qzz;
+                const double Qx = Qxx * qx_ + Qxy * qy_ + Qxz * qz_;
+                const double Qy = Qxy * qx_ + Qyy * qy_ + Qyz * qz_;
+                const double Qz = Qxz * qx_ + Qyz * qy_ + Qzz * qz_;
+                const double qrr = Qx * qx_ + Qy * qy_ + Qz * qz_;
+                const double C = -m3 + 1.5 * tq * m5 - 7.5 * qrr * m7;
+                const double bx = C * qx_ + 3.0 * m5 * Qx, by = C * qy_ + 3.0 * m5 * Qy,
+                             bz = C * qz_ + 3.0 * m5 * Qz;
+                const double ex = bx - ax, ey = by - ay, ez = bz - az;
+                const double ei = std::sqrt(ex * ex + ey * ey + ez * ez) / amag[i];
+                e = std::max(e, ei);
+              }
+              pair_gc[2 * np_out] = g;
+              pair_gc[2 * np_out + 1] = c;
+              theta_need[np_out] = (float)tn;
+              err[np_out] = (float)e;
+              target[np_out] = (float)(std::log2(tn) - std::log2(std::max(e, 1e-12)) / 3.0);
+            }
+            np_out++;
+          }
+        }
+        const bool split = ds2 < (l * it_lo + sc) * (l * it_lo + sc) || nc_p < 3;
+        if (split && !cell_is_leaf(cd)) {
+          const int fc = cell_first(cd), n = cell_n(cd);
+          for (int k = 0; k < n; k++) nxt.push_back(fc + k);
+        }
+      }
+      cur.swap(nxt);
+    }
+  }
+  return np_out;
 }
 
 }  // extern "C"

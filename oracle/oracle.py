@@ -48,6 +48,17 @@ def lib():
         L.or_direct_ld.argtypes = [ctypes.c_int, P, ctypes.c_int, P, ctypes.c_double, P]
         L.or_energies.argtypes = [ctypes.c_int, P, P, P, ctypes.c_double, P, P]
         L.or_plummer_ref.argtypes = [ctypes.c_long, ctypes.c_uint, ctypes.c_double, P, P, P]
+        L.or_f16_round.argtypes = [ctypes.c_int, P, P]
+        L.or_pair_features.argtypes = [ctypes.c_int, P, P]
+        L.or_mlp_forward.argtypes = [ctypes.c_int, P, P, ctypes.c_int, P]
+        L.or_mlp_grad.argtypes = [ctypes.c_int, P, P, P, ctypes.c_int, P, P]
+        L.or_walk_learned.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, P, P, P, P,
+                                      ctypes.c_float, P, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, P, P, P, P, P]
+        L.or_learn_pairs.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, P, P, P, P, P, P,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_float, P,
+                                     ctypes.c_int, P, P, P, P, P]
+        L.or_learn_pairs.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -272,3 +283,107 @@ def reference_pipeline(pos, nleaf, ncrit, theta, eps, level_split=5, acc_type=0,
     return dict(box=box, tree=tree, pos_tree=pos_tree, size=size, mono=mono, q0=q0, q1=q1,
                 ls=ls, keys=keys, perm=perm, sorted_keys=skeys, pos_ph=pos_ph, groups=groups,
                 acc=acc, stats=stats, pot=pot)
+
+
+# ---------------------------------------------------------------------------
+# Learned opening angle (no reference counterpart): features, the 8-32-32-1
+# network with FP16 parameters, its gradient, the walk with per-pair angles
+# and the training-pair generator.  Parameter vector: 1377 float32 in the
+# order W1[32][8], b1[32], W2[32][32], b2[32], w3[32], b3.
+# ---------------------------------------------------------------------------
+MLP_PARAMS = 1377
+
+
+def f16_round(x):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    lib().or_f16_round(x.size, _ptr(x), _ptr(out))
+    return out
+
+
+def pair_features(pin):
+    """pin: (n, 20) float32 rows gc.xyz, gs.xyz, np, com.xyz, M, l, Qxx, Qyy,
+    Qzz, Qxy, Qxz, Qyz, R, M_sys -> (n, 8) features."""
+    pin = np.ascontiguousarray(pin, np.float32)
+    out = np.empty((len(pin), 8), np.float32)
+    lib().or_pair_features(len(pin), _ptr(pin), _ptr(out))
+    return out
+
+
+def mlp_forward(params, x, round16=True):
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty(len(x), np.float32)
+    lib().or_mlp_forward(len(x), _ptr(params), _ptr(x), int(round16), _ptr(y))
+    return y
+
+
+def mlp_grad(params, x, t, round16=True):
+    """(loss, grad[1377]) of the mean squared error, FP64."""
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    t = np.ascontiguousarray(t, np.float32)
+    g = np.zeros(MLP_PARAMS, np.float64)
+    loss = np.zeros(1, np.float64)
+    lib().or_mlp_grad(len(x), _ptr(params), _ptr(x), _ptr(t), int(round16), _ptr(g), _ptr(loss))
+    return float(loss[0]), g
+
+
+def group_scale(groups, R, msys):
+    """Per-group (R, M_sys) of a single system."""
+    gs = np.empty((len(groups), 2), np.float32)
+    gs[:, 0] = R
+    gs[:, 1] = msys
+    return gs
+
+
+def walk_learned(groups, level_range, pos_ph, pos_tree, cells, size, mono, q0, q1, ls, eps,
+                 params, shift, theta_lo, theta_hi, gscale, group_cells=None):
+    """The walk with the learned per-pair angle -> (acc, stats, pot, counts)."""
+    groups = np.ascontiguousarray(groups, np.int32)
+    ng = len(groups)
+    acc = np.zeros((len(pos_ph), 4), np.float32)
+    stats = np.zeros(4, np.int64)
+    pot = np.zeros(1, np.float64)
+    cnt = np.zeros((ng, 2), np.int32)
+    if group_cells is not None:
+        group_cells = np.ascontiguousarray(group_cells, np.int32)
+    eps2 = np.float32(np.float32(eps) * np.float32(eps))
+    lib().or_walk_learned(
+        ng, _ptr(groups), _ptr(group_cells),
+        _ptr(np.ascontiguousarray(level_range, np.int32).reshape(-1)), _ptr(_f4(pos_ph)),
+        _ptr(_f4(pos_tree)), _ptr(np.ascontiguousarray(cells, np.uint32)),
+        _ptr(np.ascontiguousarray(size, np.float32)), _ptr(np.ascontiguousarray(mono, np.float32)),
+        _ptr(np.ascontiguousarray(q0, np.float32)), _ptr(np.ascontiguousarray(q1, np.float32)),
+        _ptr(np.ascontiguousarray(ls, np.float32)), ctypes.c_float(eps2),
+        _ptr(np.ascontiguousarray(params, np.float32)), ctypes.c_float(shift),
+        ctypes.c_float(theta_lo), ctypes.c_float(theta_hi),
+        _ptr(np.ascontiguousarray(gscale, np.float32)), _ptr(acc), _ptr(stats), _ptr(pot),
+        _ptr(cnt))
+    return acc, stats, float(pot[0]), cnt
+
+
+def learn_pairs(gsel, groups, level_range, pos_ph, pos_tree, cells, size, mono, q0, q1, ls,
+                gscale, theta_lo, theta_hi, eps, amag, max_pairs=1 << 20, group_cells=None):
+    """Training pairs -> dict(gc[n,2], feat[n,8], theta_need, target, err)."""
+    gsel = np.ascontiguousarray(gsel, np.int32)
+    gc = np.zeros((max_pairs, 2), np.int32)
+    feat = np.zeros((max_pairs, 8), np.float32)
+    tn = np.zeros(max_pairs, np.float32)
+    tg = np.zeros(max_pairs, np.float32)
+    er = np.zeros(max_pairs, np.float32)
+    if group_cells is not None:
+        group_cells = np.ascontiguousarray(group_cells, np.int32)
+    eps2 = np.float32(np.float32(eps) * np.float32(eps))
+    n = lib().or_learn_pairs(
+        len(gsel), _ptr(gsel), _ptr(np.ascontiguousarray(groups, np.int32)), _ptr(group_cells),
+        _ptr(np.ascontiguousarray(level_range, np.int32).reshape(-1)), _ptr(_f4(pos_ph)),
+        _ptr(_f4(pos_tree)), _ptr(np.ascontiguousarray(cells, np.uint32)),
+        _ptr(np.ascontiguousarray(size, np.float32)), _ptr(np.ascontiguousarray(mono, np.float32)),
+        _ptr(np.ascontiguousarray(q0, np.float32)), _ptr(np.ascontiguousarray(q1, np.float32)),
+        _ptr(np.ascontiguousarray(ls, np.float32)), _ptr(np.ascontiguousarray(gscale, np.float32)),
+        ctypes.c_float(theta_lo), ctypes.c_float(theta_hi), ctypes.c_float(eps2),
+        _ptr(np.ascontiguousarray(amag, np.float64)), int(max_pairs), _ptr(gc), _ptr(feat),
+        _ptr(tn), _ptr(tg), _ptr(er))
+    m = min(n, max_pairs)
+    return dict(n=n, gc=gc[:m], feat=feat[:m], theta_need=tn[:m], target=tg[:m], err=er[:m])
