@@ -3,16 +3,21 @@
 
 Default workload: N = 2^24 Plummer halo (mass 0.8) + exponential disk
 (mass 0.2, scale length 0.3, sech^2 height 0.03), float32, eps = 0.002,
-fixed theta = 0.4, NLEAF 16, NCRIT 64, levelSplit 5, dt = 1/128.  Synthetic
-seeded input, resident on the device before timing starts.
+NLEAF 16, NCRIT 64, levelSplit 5, dt = 1/128, the learned per-pair opening
+angle (bonsai2_amd/theta_mlp.json) with every step re-checked against 256
+FP64 direct sums and falling back to fixed theta = 0.4 if their p99 exceeds
+1e-3.  Synthetic seeded input, resident on the device before timing starts.
 
 Prints ONE JSON line.  value = force evaluations per second of the whole step
-(N / step time).  The walk kernel's roofline uses the reference's flop count
-(20 per P2P, 64 per P2C; code/treecode.cu:150-158) over the walk's own
-duration, taken with HIP events on the stream the kernels run on.
-cpu_baseline = the OpenMP C++ oracle (a transcription of the reference path)
-on the host cores: build, multipoles and groups in full, the walk on a seeded
-sample of the groups, scaled to all groups.
+(N / step time).  Beside it, on the same final particle state: the fixed
+theta = 0.4 walk and step (the comparison of the metric), and both walks'
+p50/p99 relative force errors against FP64 direct sums.  The walk kernel's
+roofline uses the reference's flop count (20 per P2P, 64 per P2C;
+code/treecode.cu:150-158) over the walk's own duration, taken with HIP events
+on the stream the kernels run on.  cpu_baseline = the OpenMP C++ oracle (a
+transcription of the reference path, fixed theta) on the host cores: build,
+multipoles and groups in full, the walk on a seeded sample of the groups,
+scaled to all groups.
 """
 import argparse
 import json
@@ -94,6 +99,12 @@ def main():
     ap.add_argument("--no-kahan", action="store_true")
     ap.add_argument("--walk-variant", type=int, default=0, choices=(0, 1),
                     help="0: j-data staged in LDS, 1: j-data by scalar loads")
+    ap.add_argument("--opening", default="learned", choices=("learned", "fixed"))
+    ap.add_argument("--mlp", default=None, help="trained network (default bonsai2_amd/theta_mlp.json)")
+    ap.add_argument("--check-samples", type=int, default=256)
+    ap.add_argument("--p99-max", type=float, default=1e-3)
+    ap.add_argument("--compare-steps", type=int, default=3,
+                    help="timed fixed-theta steps for the comparison (0 = walk only)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -115,6 +126,13 @@ def main():
     t.set_walk_variant(args.walk_variant)
     t.alloc(n)
     t.ptcl_h2d(pos, vel)
+    learned = args.opening == "learned"
+    if learned:
+        from bonsai2_amd import learn
+        params, budget, th_lo, th_hi = learn.load(args.mlp or learn.DEFAULT_PATH)
+        t.set_mlp_params(params)
+        t.set_opening(1, budget, th_lo, th_hi)
+        t.set_opening_check(args.check_samples, args.p99_max, seed)
     # first force evaluation (not timed), then warmup steps
     t.buildTree(args.nleaf)
     t.computeMultipoles()
@@ -141,15 +159,52 @@ def main():
     flops = np.array(flops)
     inter = np.array(inter)
     achieved = float((flops / (walk_ms * 1e-3)).mean() / 1e12)
+    chk = t.opening_check() if learned else None
     # accuracy against the FP64 direct sum on seeded samples (outside the timed region)
     rng = np.random.default_rng(seed + 1)
     idx = np.sort(rng.choice(n, size=min(n, args.samples), replace=False)).astype(np.int32)
     d = t.direct_fp64(idx)
-    a = t.acc(bt.ORDER_PH)[idx, :3].astype(np.float64)
-    err = np.linalg.norm(a - d[:, :3], axis=1) / np.linalg.norm(d[:, :3], axis=1)
-    p50, p99 = (float(x) for x in np.percentile(err, [50, 99]))
+
+    def errors():
+        a = t.acc(bt.ORDER_PH)[idx, :3].astype(np.float64)
+        e = np.linalg.norm(a - d[:, :3], axis=1) / np.linalg.norm(d[:, :3], axis=1)
+        return tuple(float(x) for x in np.percentile(e, [50, 99]))
+
+    p50, p99 = errors()
     ek, ep = t.computeEnergies()
     c = t.counts()
+    st_main, _ = t.walk_stats()
+    compare = None
+    if learned:
+        # the metric's comparison on the same particles and tree: the fixed
+        # theta = 0.4 walk (and, with --compare-steps, its full step)
+        t.set_opening_check(0, args.p99_max, seed)
+        lw = []
+        for _ in range(3):
+            t.computeForces()
+            lw.append(t.timings()["walk"])
+        t.set_opening(0)
+        t.set_params(theta=w["theta"])
+        t.computeMultipoles()
+        fw = []
+        for _ in range(3):
+            st_f = t.computeForces()
+            fw.append(t.timings()["walk"])
+        f50, f99 = errors()
+        compare = {"theta": w["theta"], "walk_ms": float(np.median(fw)),
+                   "learned_walk_ms": float(np.median(lw)),
+                   "walk_speedup": float(np.median(fw) / np.median(lw)),
+                   "p50_rel_err": f50, "p99_rel_err": f99,
+                   "interactions_per_particle": {"p2p": st_f[0], "p2c": st_f[2]},
+                   "flops_per_launch": (20.0 * st_f[0] + 64.0 * st_f[2]) * n}
+        if args.compare_steps > 0:
+            tc0 = time.perf_counter()
+            for _ in range(args.compare_steps):
+                t.step(args.dt, args.nleaf, args.level_split, args.ncrit)
+            ms_f = (time.perf_counter() - tc0) * 1e3 / args.compare_steps
+            compare["ms_per_step"] = ms_f
+            compare["value"] = n / (ms_f * 1e-3)
+            compare["step_speedup"] = ms_f / ms_step
 
     phase_ms = {k: float(np.mean([p[k] for p in phase])) for k in phase[0]}
     out = {
@@ -167,7 +222,12 @@ def main():
         "data": "synthetic (seeded %s, resident in HBM)" % w["ic"],
         "config": {"workload": args.workload, "n": n, "theta": w["theta"], "eps": w["eps"],
                    "nleaf": args.nleaf, "ncrit": args.ncrit, "level_split": args.level_split,
-                   "dt": args.dt, "opening": "fixed theta", "kahan": not args.no_kahan,
+                   "dt": args.dt,
+                   "opening": ("learned per-pair theta (budget %.3g, band [%g, %g]), FP64 check "
+                               "of %d samples per step, fallback theta %g at p99 > %g" % (
+                                   budget, th_lo, th_hi, args.check_samples, w["theta"],
+                                   args.p99_max)) if learned else "fixed theta",
+                   "kahan": not args.no_kahan,
                    "walk_jdata": "scalar" if args.walk_variant == 1 else "lds"},
         "interactions_per_s": float((inter / (walk_ms * 1e-3)).mean()),
         "walk_ms": float(walk_ms.mean()),
@@ -180,13 +240,16 @@ def main():
         "phase_ms": phase_ms,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_walk",
+                     "kernel": "k_walk<learned>" if learned else "k_walk",
                      "flops_per_launch": float(flops.mean()),
                      "avg_launch_ms": float(walk_ms.mean())},
     }
-    st, tot = t.walk_stats()
+    st = st_main
     out["interactions_per_particle"] = {"p2p": st[0], "p2c": st[2], "p2p_max": st[1],
                                         "p2c_max": st[3]}
+    if learned:
+        out["opening_check"] = chk
+        out["fixed_theta"] = compare
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pos, args.nleaf, args.ncrit, w["theta"], w["eps"],
                                            args.level_split, args.cpu_groups)

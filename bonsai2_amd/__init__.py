@@ -243,6 +243,17 @@ class Treecode:
         self._ck(self._L.bt_set_opening_check(self._h, int(nsamples), float(p99_max), int(seed)),
                  "set_opening_check")
 
+    def use_learned_theta(self, path=None, check_samples=256, p99_max=1e-3, budget=None):
+        """Loads a trained network (bonsai2_amd/theta_mlp.json by default) and
+        switches the walk to the learned per-pair angle, re-checked against
+        check_samples FP64 direct sums after every walk (0 = no check)."""
+        from . import learn
+        params, b, lo, hi = learn.load(path or learn.DEFAULT_PATH)
+        self.set_mlp_params(params)
+        self.set_opening(1, b if budget is None else budget, lo, hi)
+        self.set_opening_check(check_samples, p99_max)
+        return b if budget is None else budget
+
     def opening_check(self):
         """{p50, p99, max, fell_back, fallbacks, checks} of the last re-check."""
         o = np.zeros(6, np.float64)

@@ -144,16 +144,18 @@ __device__ __forceinline__ float mlp_block(half8_t a1, half8_t a2_0, half8_t a2_
 }
 
 // Network output for this lane's own feature vector (all 64 lanes active);
-// the wave evaluates two 32-cell blocks, the second only if need_hi.
+// the wave evaluates the block of lanes 0-31 if need_lo and the block of
+// lanes 32-63 if need_hi (wave-uniform flags); skipped lanes get 0.
 __device__ __forceinline__ float mlp_wave(const half8_t a1, const half8_t a2_0,
                                           const half8_t a2_1, const float *s_mlp, float b3,
-                                          const half8_t xown, bool need_hi) {
+                                          const half8_t xown, bool need_lo, bool need_hi) {
   const int lane = lane_id();
   const int hh = lane >> 5;
   const float *c2p = s_mlp + hh * 16;
   const float *w3p = s_mlp + 32 + hh * 16;
   const half8_t bias = bias_column();
-  const float y0 = mlp_block(a1, a2_0, a2_1, c2p, w3p, b3, hh == 0 ? xown : bias);
+  float y0 = 0.f;
+  if (need_lo) y0 = mlp_block(a1, a2_0, a2_1, c2p, w3p, b3, hh == 0 ? xown : bias);
   float y1 = 0.f;
   if (need_hi) {
     // lanes of half 0 take the features of lane + 32

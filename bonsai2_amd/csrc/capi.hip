@@ -184,7 +184,7 @@ int learned_check(Ctx &c) {
   c.check_calls++;
   BT_CK(hipMemcpyAsync(c.d_check_idx, idx.data(), sizeof(int) * ns, hipMemcpyHostToDevice,
                        c.stream));
-  double *res = c.d_check + (size_t)ns * 65 * 4;
+  double *res = c.d_check + (size_t)ns * (direct_slices(ns) + 1) * 4;
   launch_direct(c, ns, c.d_check_idx, c.d_check);
   launch_learn_check(c, ns, c.d_check_idx, c.d_check, res);
   BT_LAUNCH_CK();
@@ -766,12 +766,12 @@ int bt_direct_fp64(bt_ctx *h, int ns, const int *ph_idx, double *out4) {
   if (!c.have_groups) return fail(c, BT_ERR_STATE, "makeGroups first (targets are PH indices)");
   for (int k = 0; k < ns; k++)
     if (ph_idx[k] < 0 || ph_idx[k] >= c.n) return fail(c, BT_ERR_ARG, "target index out of range");
-  const size_t need = (size_t)ns * 65 * 4 + (size_t)ns;  // doubles (+ room for the indices)
+  const size_t need = (size_t)ns * (direct_slices(ns) + 1) * 4 + (size_t)ns;  // + the indices
   if ((int)need > c.direct_cap || !c.d_direct) {
     if (dev_alloc(c, &c.d_direct, need)) return BT_ERR_HIP;
     c.direct_cap = (int)need;
   }
-  int *d_idx = reinterpret_cast<int *>(c.d_direct + (size_t)ns * 65 * 4);
+  int *d_idx = reinterpret_cast<int *>(c.d_direct + (size_t)ns * (direct_slices(ns) + 1) * 4);
   BT_CK(hipMemcpyAsync(d_idx, ph_idx, sizeof(int) * ns, hipMemcpyHostToDevice, c.stream));
   launch_direct(c, ns, d_idx, c.d_direct);
   BT_LAUNCH_CK();
@@ -978,7 +978,7 @@ int bt_learn_collect(bt_ctx *h, int nsel, const int *gsel, float theta_lo, float
   rc |= dev_alloc(c, &L.err, max_pairs);
   rc |= dev_alloc(c, &L.tneed_f, max_pairs);
   rc |= dev_alloc(c, &L.sink, (size_t)c.walk_blocks * 4 * 64);
-  rc |= dev_alloc(c, &dd, (size_t)ns * 65 * 4);
+  rc |= dev_alloc(c, &dd, (size_t)ns * (direct_slices(ns) + 1) * 4);
   rc |= dev_alloc(c, &didx, ns);
   int found = 0;
   if (!rc) {
@@ -1050,7 +1050,7 @@ int bt_set_opening_check(bt_ctx *h, int nsamples, float p99_max, uint64_t seed) 
   if (nsamples < 0 || nsamples > 1024 || !(p99_max > 0.f))
     return fail(c, BT_ERR_ARG, "samples must be in [0, 1024], p99_max > 0");
   if (nsamples > 0) {
-    if (dev_alloc(c, &c.d_check, (size_t)nsamples * 65 * 4 + 4) ||
+    if (dev_alloc(c, &c.d_check, (size_t)nsamples * (direct_slices(nsamples) + 1) * 4 + 4) ||
         dev_alloc(c, &c.d_check_idx, nsamples))
       return BT_ERR_HIP;
   }

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_forward(const MlpDev *__restrict
     const int k = base + lane;
     float f[8];
     for (int t = 0; t < 8; t++) f[t] = k < n ? x[8 * (size_t)k + t] : 0.0f;
-    const float yk = mlp_wave(a1, a2_0, a2_1, s_mlp, b3, pack_features(f), true);
+    const float yk = mlp_wave(a1, a2_0, a2_1, s_mlp, b3, pack_features(f), true, true);
     if (k < n) y[k] = yk;
   }
 }

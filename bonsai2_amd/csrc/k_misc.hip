@@ -13,6 +13,8 @@
 //     LDS, 1/sqrt by a float v_rsq seed refined with two Newton steps in FP64.
 #include <cfloat>
 
+#include <algorithm>
+
 #include "bt_device.h"
 #include "bt_internal.h"
 
@@ -179,14 +181,22 @@ void launch_energies(Ctx &c) {
                      c.nsys, kEnergyBlocks, part, c.d_energy);
 }
 
-// d_out must hold ns * (kDirectSlices + 1) double4: the partials then the result.
+// Slices of the source range: enough blocks to fill the chip for any number
+// of targets (256 targets per block).
+int direct_slices(int ns) {
+  const int nb = (ns + kBlock - 1) / kBlock;
+  return std::min(1024, std::max(kDirectSlices, (4096 + nb - 1) / nb));
+}
+
+// d_out must hold ns * (direct_slices(ns) + 1) double4: the result, then the partials.
 void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out) {
   double4 *part = reinterpret_cast<double4 *>(d_out) + ns;
   const double eps2 = (double)c.eps * (double)c.eps;
-  hipLaunchKernelGGL(k_direct, dim3((ns + kBlock - 1) / kBlock, kDirectSlices), dim3(kBlock), 0,
+  const int slices = direct_slices(ns);
+  hipLaunchKernelGGL(k_direct, dim3((ns + kBlock - 1) / kBlock, slices), dim3(kBlock), 0,
                      c.stream, ns, d_idx, c.n, c.nsys, c.d_off, c.pos_ph, eps2, part);
   hipLaunchKernelGGL(k_direct_reduce, dim3((ns + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream,
-                     ns, kDirectSlices, part, reinterpret_cast<double4 *>(d_out));
+                     ns, slices, part, reinterpret_cast<double4 *>(d_out));
 }
 
 }  // namespace bt

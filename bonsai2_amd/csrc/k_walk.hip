@@ -46,6 +46,9 @@ constexpr int kMaxWalkSteps = 1 << 20;
 #ifndef BT_WALK_WAVES_PER_EU
 #define BT_WALK_WAVES_PER_EU 1
 #endif
+#ifndef BT_LEARN_WAVES_PER_EU
+#define BT_LEARN_WAVES_PER_EU 4
+#endif
 
 struct WalkArgs {
   int n, nsys;
@@ -96,6 +99,19 @@ __device__ __forceinline__ bool open_cell(const float4 cs, float gcx, float gcy,
   dz += fabsf(dz); dz *= 0.5f;
   const float ds2 = dx * dx + dy * dy + dz * dz;
   return ds2 < fabsf(cs.w);
+}
+
+// ds^2 of the opening test above.
+__device__ __forceinline__ float box_ds2(const float4 cs, float gcx, float gcy, float gcz,
+                                         float gsx, float gsy, float gsz) {
+#pragma clang fp contract(off)
+  float dx = fabsf(gcx - cs.x) - gsx;
+  float dy = fabsf(gcy - cs.y) - gsy;
+  float dz = fabsf(gcz - cs.z) - gsz;
+  dx += fabsf(dx); dx *= 0.5f;
+  dy += fabsf(dy); dy *= 0.5f;
+  dz += fabsf(dz); dz *= 0.5f;
+  return dx * dx + dy * dy + dz * dz;
 }
 
 struct Acc4 {
@@ -302,7 +318,8 @@ __device__ __forceinline__ void batch_p2p(const WalkArgs &a, WaveShared &S, int 
 // LEARN: each popped cell gets its own opening angle from the network
 // (bt_learn.h), evaluated by the wave on the matrix cores for up to 64 cells.
 template <bool KAHAN, bool SJ, bool LEARN>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BT_WALK_WAVES_PER_EU)))
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(LEARN ? BT_LEARN_WAVES_PER_EU : BT_WALK_WAVES_PER_EU)))
 void k_walk(WalkArgs a) {
   __shared__ WaveShared sh[kWavesPerBlock];
   __shared__ float s_mlp[LEARN ? 64 : 1];
@@ -397,7 +414,21 @@ void k_walk(WalkArgs a) {
         }
         float x[8];
         pair_features(geom, m, lsv.x, q0, q1, x);
-        const float y = mlp_wave(ma1, ma2_0, ma2_1, s_mlp, mb3, pack_features(x), take > 32);
+        // cells opened even at theta_hi or accepted even at theta_lo do not
+        // need the network: a 32-lane block without undecided cells is skipped
+        const float ds2 = box_ds2(cs, gcx, gcy, gcz, gsx, gsy, gsz);
+        float op_lo, op_hi;
+        {
+#pragma clang fp contract(off)
+          op_lo = lsv.x * a.it_max + lsv.y;
+          op_hi = lsv.x * a.it_min + lsv.y;
+          op_lo = op_lo * op_lo;
+          op_hi = op_hi * op_hi;
+        }
+        const bool undecided = use && ds2 < op_lo && ds2 >= op_hi;
+        const uint64_t um = ballot64(undecided);
+        const float y = mlp_wave(ma1, ma2_0, ma2_1, s_mlp, mb3, pack_features(x),
+                                 (um & 0xFFFFFFFFull) != 0, (um >> 32) != 0);
         const float it = learned_inv_theta(y, a.shift, a.it_min, a.it_max);
         float op;
         {
@@ -585,7 +616,9 @@ void launch_walk(Ctx &c) {
   a.dc = c.dc;
   hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, c.stream, c.dc);
   void (*kern)(WalkArgs);
-  if (c.opening == 1 && c.have_mlp)
+  if (c.opening == 1 && c.have_mlp && c.walk_variant == 1)
+    kern = c.kahan ? k_walk<true, true, true> : k_walk<false, true, true>;
+  else if (c.opening == 1 && c.have_mlp)
     kern = c.kahan ? k_walk<true, false, true> : k_walk<false, false, true>;
   else if (c.walk_variant == 1)
     kern = c.kahan ? k_walk<true, true, false> : k_walk<false, true, false>;

@@ -1,0 +1,190 @@
+"""GPU tests of the learned opening angle against the oracle: the network's
+matrix-core inference and its gradient, the training-pair generator, the walk
+with per-pair angles, training on the GPU, the per-step FP64 check with its
+fixed-angle fallback, and what the trained network gains over fixed angles.
+
+Bars: network outputs within 1e-3 absolute of the oracle (FP16 activations
+may round differently when the GPU's SiLU differs from the oracle's in the
+last bit), median within 1e-5; gradients within 1e-4 relative; pair sets
+equal up to pairs on the band edge, features within 1e-6, targets within
+1e-3 (both sides use their own moments, equal to FP32 rounding); the learned walk's
+interaction counts within 1e-3 and forces within 1e-5 relative RMS of the
+oracle's learned walk."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 1 << 15
+THETA, EPS = 0.5, 0.01
+
+
+def rel_rms(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def rand_params(seed, scale=0.3, b3=-1.2):
+    rng = np.random.default_rng(seed)
+    p = (rng.standard_normal(1377) * scale).astype(np.float32)
+    p[1376] = b3
+    return p
+
+
+@pytest.fixture(scope="module")
+def setup(bt, oracle):
+    pos, vel = bt.plummer(N, seed=21)
+    t = bt.Treecode(EPS, THETA, 64)
+    t.alloc(N)
+    t.ptcl_h2d(pos, vel)
+    t.buildTree(16)
+    t.computeMultipoles()
+    t.makeGroups(5, 64)
+    ref = oracle.reference_pipeline(pos, 16, 64, THETA, EPS)
+    mtot = float(pos[:, 3].astype(np.float64).sum())
+    gscale = oracle.group_scale(ref["groups"], ref["box"][3], np.float32(mtot))
+    yield dict(t=t, ref=ref, pos=pos, vel=vel, gscale=gscale)
+    t.close()
+
+
+def test_mlp_forward_matches_oracle(setup, oracle):
+    t = setup["t"]
+    rng = np.random.default_rng(1)
+    for seed in (2, 3):
+        p = rand_params(seed)
+        t.set_mlp_params(p)
+        x = rng.random((5000, 8)).astype(np.float32)
+        dy = np.abs(t.mlp_forward(x) - oracle.mlp_forward(p, x))
+        assert dy.max() <= 1e-3 and np.median(dy) <= 1e-5, (dy.max(), np.median(dy))
+        assert np.array_equal(t.mlp_params(), p)
+
+
+def test_mlp_gradient_matches_oracle(setup, oracle):
+    t = setup["t"]
+    rng = np.random.default_rng(4)
+    p = rand_params(5)
+    t.set_mlp_params(p)
+    x = rng.random((3000, 8)).astype(np.float32)
+    y = rng.standard_normal(3000).astype(np.float32)
+    lg, gg = t.mlp_grad(x, y)
+    lo, go = oracle.mlp_grad(p, x, y)
+    assert abs(lg - lo) <= 1e-5 * abs(lo)
+    assert np.linalg.norm(gg - go) <= 1e-4 * np.linalg.norm(go)
+
+
+def test_learn_pairs_match_oracle(setup, oracle):
+    t, ref = setup["t"], setup["ref"]
+    ng = len(ref["groups"])
+    gsel = np.array([0, 1, 17, ng // 3, ng // 2, ng - 1], np.int32)
+    col = t.learn_collect(gsel, 0.25, 1.0, max_pairs=1 << 16)
+    pidx = np.concatenate([np.arange(ref["groups"][g][0], ref["groups"][g][0] + ref["groups"][g][1])
+                           for g in gsel]).astype(np.int32)
+    d = oracle.direct_fp64(ref["pos_ph"], pidx, EPS)
+    amag = np.ones(N)
+    amag[pidx] = np.linalg.norm(d[:, :3], axis=1)
+    oc = oracle.learn_pairs(gsel, ref["groups"], ref["tree"].level_range, ref["pos_ph"],
+                            ref["pos_tree"], ref["tree"].cells, ref["size"], ref["mono"],
+                            ref["q0"], ref["q1"], ref["ls"], setup["gscale"], 0.25, 1.0, EPS, amag)
+    # the features and theta_need are built from each side's own moments,
+    # which agree to FP32 rounding (not bit for bit): a pair on the exact band
+    # edge may fall on different sides
+    assert col["n"] > 100 and abs(col["n"] - oc["n"]) <= max(2, oc["n"] // 200)
+    key = lambda gc: gc[:, 0].astype(np.int64) * (1 << 32) + gc[:, 1]
+    kg, ko = key(col["gc"]), key(oc["gc"])
+    common, ig, io = np.intersect1d(kg, ko, return_indices=True)
+    assert len(common) >= len(ko) - max(2, len(ko) // 200)
+    assert np.abs(col["feat"][ig] - oc["feat"][io]).max() <= 1e-6
+    assert np.abs(col["target"][ig] - oc["target"][io]).max() <= 1e-3
+    assert np.allclose(col["theta_need"][ig], oc["theta_need"][io], rtol=1e-5, atol=0)
+
+
+def test_learned_walk_matches_oracle(setup, oracle):
+    t, ref = setup["t"], setup["ref"]
+    p = rand_params(7, 0.3, -1.0)
+    t.set_mlp_params(p)
+    t.set_opening(1, budget=1e-4, theta_lo=0.25, theta_hi=1.0)
+    try:
+        st = t.computeForces()
+        acc = t.acc(2)
+        cnt = t.group_counts()
+    finally:
+        t.set_opening(0)
+    shift = np.float32(np.log2(np.float32(1e-4)) / np.float32(3))
+    acc_o, st_o, _, cnt_o = oracle.walk_learned(
+        ref["groups"], ref["tree"].level_range, ref["pos_ph"], ref["pos_tree"], ref["tree"].cells,
+        ref["size"], ref["mono"], ref["q0"], ref["q1"], ref["ls"], EPS, p, shift, 0.25, 1.0,
+        setup["gscale"])
+    exp = st_o / [N, 1, N, 1]
+    for a, b in zip(st, exp):
+        assert abs(a - b) <= 1e-3 * max(1.0, abs(b)), (st, exp)
+    assert (cnt != cnt_o).any(axis=1).mean() <= 0.01
+    assert rel_rms(acc[:, :3], acc_o[:, :3]) <= 1e-5
+
+
+def test_opening_check_falls_back_to_fixed_theta(setup):
+    t = setup["t"]
+    t.computeForces()
+    a_fixed = t.acc(2)
+    t.set_mlp_params(rand_params(8, 0.3, 0.0))
+    t.set_opening(1, budget=1e-2, theta_lo=0.25, theta_hi=1.0)
+    try:
+        t.set_opening_check(256, 1e-12, 3)   # unreachable: must fall back
+        t.computeForces()
+        chk = t.opening_check()
+        assert chk["fell_back"] and chk["fallbacks"] == 1 and chk["checks"] == 1
+        assert chk["p99"] > 1e-12
+        assert np.array_equal(t.acc(2), a_fixed)
+        t.set_opening_check(256, 1.0, 3)     # generous: keeps the learned forces
+        t.computeForces()
+        chk = t.opening_check()
+        assert not chk["fell_back"] and chk["checks"] == 1
+        assert not np.array_equal(t.acc(2), a_fixed)
+    finally:
+        t.set_opening_check(0, 1e-3, 1)
+        t.set_opening(0)
+
+
+def test_training_on_the_gpu_fits_the_pairs(setup, bt):
+    t, ref = setup["t"], setup["ref"]
+    gsel = np.arange(0, len(ref["groups"]), 2, dtype=np.int32)
+    col = t.learn_collect(gsel, 0.25, 1.0, max_pairs=1 << 20)
+    x, y = col["feat"], col["target"]
+    t.mlp_init(3, float(y.mean()))
+    curve = t.mlp_train(x, y, steps=600, batch=4096, lr=3e-3, seed=4)
+    resid = t.mlp_forward(x) - y
+    assert np.mean(curve[-20:]) < 0.5 * curve[0]
+    assert np.mean(resid ** 2) < 0.5 * np.var(y)
+
+
+def test_trained_network_gains_over_fixed_theta(bt):
+    """The committed network (trained on the Plummer configurations) on a
+    halo + disk system it never saw: at its calibrated budget the p99 error
+    stays <= 1e-3, and it needs fewer interaction flops than every fixed
+    angle that is at least as accurate."""
+    from bonsai2_amd import learn
+    assert os.path.exists(learn.DEFAULT_PATH), "bonsai2_amd/theta_mlp.json missing"
+    params, budget, lo, hi = learn.load()
+    n = 1 << 18
+    pos, vel = bt.halo_disk(n, seed=19810614)
+    t = learn.built_tree(pos, vel, 0.4, 0.002)
+    try:
+        p50_l, p99_l, flops_l = learn.evaluate(t, params, budget, lo, hi, n, samples=4096)
+        fixed = []
+        for th in np.arange(0.30, 0.801, 0.025):
+            t.set_params(theta=float(th))
+            t.computeMultipoles()
+            st = t.computeForces()
+            f50, f99 = learn.walk_error(t, n, 4096, 5)
+            fixed.append((float(th), f99, (20.0 * st[0] + 64.0 * st[2]) * n))
+    finally:
+        t.close()
+    print("learned: p50 %.2e p99 %.2e flops %.3e" % (p50_l, p99_l, flops_l))
+    for th, f99, fl in fixed:
+        print("fixed theta %.3f: p99 %.2e flops %.3e" % (th, f99, fl))
+    assert p99_l <= 1e-3
+    as_accurate = [fl for th, f99, fl in fixed if f99 <= p99_l]
+    assert as_accurate, "no fixed angle reaches the learned p99"
+    assert flops_l <= 0.9 * min(as_accurate)
