@@ -33,9 +33,18 @@ sys.path.insert(0, ROOT)
 METRIC = ("force-evals/s at N=2^24, p99 rel force err ≤1e-3 (learned vs fixed θ); "
           "interactions/s")
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per walk launch from two rocprofv3 --pmc passes (FETCH_SIZE, then
+# WRITE_SIZE) scaled by a 1 GiB device copy (FETCH_SIZE reads 0.5x, WRITE_SIZE
+# 1.0x): profiles/hbm_traffic_halo_disk_2p24_pmc.md.  Counters cannot be read
+# from inside the bench run, so only the measured default configurations carry a
+# figure; every other configuration reports null.
+WALK_TRAFFIC_BYTES = {("halo_disk_2p24", "learned"): 2.382e10,
+                      ("halo_disk_2p24", "fixed"): 3.876e10}
 
 WORKLOADS = {
     "halo_disk_2p24": dict(n=1 << 24, ic="halo_disk", theta=0.4, eps=0.002),
+    "clustered_2p24": dict(n=1 << 24, ic="clustered", theta=0.5, eps=0.002),
+    "ensemble_256": dict(n=0, ic="ensemble", theta=0.5, eps=0.01),
     "plummer_2p24": dict(n=1 << 24, ic="plummer", theta=0.5, eps=0.002),
     "plummer_2p22": dict(n=1 << 22, ic="plummer", theta=0.5, eps=0.005),
     "plummer_2p20": dict(n=1 << 20, ic="plummer", theta=0.5, eps=0.005),
@@ -82,6 +91,68 @@ def cpu_baseline(pos, nleaf, ncrit, theta, eps, level_split, sample_groups, seed
                 walk_s_scaled=t_walk, build_s=t1 - t0, multipoles_s=t2 - t1, groups_s=t3 - t2)
 
 
+def bench_ensemble(args, bt):
+    """256 independent Plummer systems, N_k = round(2^U[10, 16]), in one context:
+    autotuned NLEAF / NCRIT / walk grid / stack, the step captured in a HIP
+    graph and replayed; aggregate interactions/s and per-system energy drift."""
+    seed = 19810614
+    pos, vel, off = bt.ensemble(256, seed=seed)
+    n = int(off[-1])
+    w = WORKLOADS["ensemble_256"]
+    t = bt.Treecode(w["eps"], w["theta"], 64)
+    t.alloc(n, off)
+    t.ptcl_h2d(pos, vel)
+    tt0 = time.perf_counter()
+    best, table = t.autotune(args.level_split)
+    tune_s = time.perf_counter() - tt0
+    nl, nc = best["nleaf"], best["ncrit"]
+    t.buildTree(nl)
+    t.computeMultipoles()
+    t.makeGroups(args.level_split, nc)
+    t.computeForces()
+    ek0, ep0 = t.computeEnergies()
+    steps = args.steps
+    # eager steps for reference (and warm-up), then the captured step
+    te0 = time.perf_counter()
+    for _ in range(args.warmup):
+        t.step(args.dt, nl, args.level_split, nc)
+    ms_eager = (time.perf_counter() - te0) * 1e3 / max(args.warmup, 1)
+    st, tot = t.walk_stats()
+    t.step_graph(args.dt, nl, args.level_split, nc, nsteps=1)   # capture + first replay
+    t0 = time.perf_counter()
+    t.step_graph(args.dt, nl, args.level_split, nc, nsteps=steps)
+    ms_step = (time.perf_counter() - t0) * 1e3 / steps
+    st2, tot2 = t.walk_stats()
+    ek1, ep1 = t.computeEnergies()
+    e0 = ek0 + ep0
+    e1 = ek1 + ep1
+    drift = np.abs(e1 - e0) / np.abs(e0)
+    inter = tot2[0] + tot2[1]
+    out = {
+        "metric": "interactions/s (ensemble of 256 systems, one persistent walk per step)",
+        "value": inter / (ms_step * 1e-3),
+        "unit": "interactions/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (256 seeded Plummer systems, resident in HBM)",
+        "config": {"workload": "ensemble_256", "n": n, "nsys": 256, "theta": w["theta"],
+                   "eps": w["eps"], "dt": args.dt, "level_split": args.level_split,
+                   "autotuned": best, "hip_graph": True},
+        "force_evals_per_s": n / (ms_step * 1e-3),
+        "eager_ms_per_step": ms_eager,
+        "autotune_s": tune_s,
+        "autotune_table": table,
+        "interactions_per_particle": {"p2p": st2[0], "p2c": st2[2]},
+        "energy_drift": {"median": float(np.median(drift)), "p90": float(np.percentile(drift, 90)),
+                         "max": float(drift.max()), "steps": args.warmup + steps + 1},
+        "system_sizes": {"min": int(np.diff(off).min()), "max": int(np.diff(off).max()),
+                         "total": n},
+        "cpu_baseline": None,
+    }
+    t.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +174,11 @@ def main():
     ap.add_argument("--mlp", default=None, help="trained network (default bonsai2_amd/theta_mlp.json)")
     ap.add_argument("--check-samples", type=int, default=256)
     ap.add_argument("--p99-max", type=float, default=1e-3)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device with torch (a plain copy kernel "
+                         "of known size, for scaling HBM byte counters)")
+    ap.add_argument("--kahan-compare", action="store_true",
+                    help="also report the fixed-theta walk with and without compensated sums")
     ap.add_argument("--compare-steps", type=int, default=3,
                     help="timed fixed-theta steps for the comparison (0 = walk only)")
     args = ap.parse_args()
@@ -112,7 +188,19 @@ def main():
               "(multi-GPU sharding is design-only); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.calibrate_copy > 0:
+        import torch
+        src = torch.ones(args.calibrate_copy * (1 << 18), dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     import bonsai2_amd as bt
+    if args.workload == "ensemble_256":
+        bench_ensemble(args, bt)
+        return
     w = dict(WORKLOADS[args.workload])
     if args.n:
         w["n"] = args.n
@@ -138,6 +226,8 @@ def main():
     t.computeMultipoles()
     t.makeGroups(args.level_split, args.ncrit)
     t.computeForces()
+    ek0, ep0 = t.computeEnergies()
+    e_start = float(ek0.sum() + ep0.sum())
     for _ in range(args.warmup):
         t.step(args.dt, args.nleaf, args.level_split, args.ncrit)
 
@@ -172,8 +262,31 @@ def main():
 
     p50, p99 = errors()
     ek, ep = t.computeEnergies()
+    e_end = float(ek.sum() + ep.sum())
     c = t.counts()
     st_main, _ = t.walk_stats()
+    kahan_cmp = None
+    if args.kahan_compare:
+        # the fixed-theta walk on the final state, compensated and plain FP32 sums
+        if learned:
+            t.set_opening_check(0, args.p99_max, seed)
+            t.set_opening(0)
+        res = {}
+        accs = {}
+        for kahan in (1, 0):
+            t.set_walk_options(kahan=kahan)
+            t.computeForces()
+            accs[kahan] = t.acc(bt.ORDER_PH)[:, :3].astype(np.float64)
+            k50, k99 = errors()
+            res["kahan" if kahan else "plain"] = {"p50_rel_err": k50, "p99_rel_err": k99,
+                                                  "walk_ms": t.timings()["walk"]}
+        res["rel_rms_difference"] = float(np.sqrt(((accs[1] - accs[0]) ** 2).sum() /
+                                                  (accs[1] ** 2).sum()))
+        t.set_walk_options(kahan=0 if args.no_kahan else 1)
+        if learned:
+            t.set_opening(1, budget, th_lo, th_hi)
+            t.set_opening_check(args.check_samples, args.p99_max, seed)
+        kahan_cmp = res
     compare = None
     if learned:
         # the metric's comparison on the same particles and tree: the fixed
@@ -235,11 +348,17 @@ def main():
         "p50_rel_err": p50,
         "p99_rel_err": p99,
         "error_samples": int(len(idx)),
-        "energy": {"kinetic": float(ek[0]), "potential": float(ep[0])},
+        "energy": {"kinetic": float(ek.sum()), "potential": float(ep.sum()),
+                   "E_start": e_start, "E_end": e_end,
+                   "rel_drift": abs(e_end - e_start) / abs(e_start),
+                   "steps_integrated": args.warmup + args.steps},
         "tree": {"ncells": c["ncells"], "nlevels": c["nlevels"], "ngroups": c["ngroups"]},
         "phase_ms": phase_ms,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": (WALK_TRAFFIC_BYTES.get((args.workload, args.opening))
+                                 if not args.n and args.nleaf == 16 and args.ncrit == 64
+                                 else None),
                      "kernel": "k_walk<learned>" if learned else "k_walk",
                      "flops_per_launch": float(flops.mean()),
                      "avg_launch_ms": float(walk_ms.mean())},
@@ -250,6 +369,8 @@ def main():
     if learned:
         out["opening_check"] = chk
         out["fixed_theta"] = compare
+    if kahan_cmp is not None:
+        out["kahan_compare"] = kahan_cmp
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pos, args.nleaf, args.ncrit, w["theta"], w["eps"],
                                            args.level_split, args.cpu_groups)

@@ -14,7 +14,7 @@ import os
 import numpy as np
 
 __all__ = ["Treecode", "lib", "BonsaiError", "ORDER_INPUT", "ORDER_TREE", "ORDER_PH",
-           "plummer", "clustered", "halo_disk", "read_tipsy", "write_tipsy",
+           "plummer", "clustered", "halo_disk", "ensemble", "read_tipsy", "write_tipsy",
            "EXPORTED_SYMBOLS"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -59,6 +59,8 @@ _SIGS = {
     "bt_get_timings": [P, P],
     "bt_set_walk_options": [P, I, I, I],
     "bt_set_walk_variant": [P, I],
+    "bt_step_graph": [P, F, I, I, I, I],
+    "bt_autotune": [P, I, I, P, P, P],
     "bt_mlp_init": [P, ctypes.c_uint64, F],
     "bt_mlp_set_params": [P, P],
     "bt_mlp_get_params": [P, P],
@@ -176,6 +178,26 @@ class Treecode:
     def set_walk_variant(self, variant):
         """0 = LDS-staged j-data (default), 1 = scalar-load j-data."""
         self._ck(self._L.bt_set_walk_variant(self._h, int(variant)), "set_walk_variant")
+
+    # -- captured step and autotuner (no reference counterpart) ---------------
+    def step_graph(self, dt, nLeaf=16, levelSplit=5, nCrit=64, nsteps=1):
+        """nsteps replays of the step captured in a HIP graph."""
+        self._ck(self._L.bt_step_graph(self._h, float(dt), int(nLeaf), int(levelSplit),
+                                       int(nCrit), int(nsteps)), "step_graph")
+
+    def autotune(self, levelSplit=5):
+        """-> (best dict, table rows) ; the chosen set-up stays active."""
+        table = np.zeros((16, 6), np.float64)
+        nt = np.zeros(1, np.int32)
+        best = np.zeros(4, np.int32)
+        self._ck(self._L.bt_autotune(self._h, int(levelSplit), 0, _p(table), _p(nt), _p(best)),
+                 "autotune")
+        rows = [dict(nleaf=int(r[0]), ncrit=int(r[1]), blocks_per_cu=int(r[2]), step_ms=float(r[3]),
+                     walk_ms=float(r[4]), deepest_stack=int(r[5])) for r in table[:int(nt[0])]]
+        b = dict(nleaf=int(best[0]), ncrit=int(best[1]), blocks_per_cu=int(best[2]),
+                 stack_ints=int(best[3]))
+        self.nLeaf, self.nCrit = b["nleaf"], b["ncrit"]
+        return b, rows
 
     # -- learned opening angle (no reference counterpart) --------------------
     MLP_PARAMS = 1377
@@ -431,6 +453,19 @@ def clustered(n, nclust=64, seed=19810614, box=10.0, mmin=1e-3, a0=0.5):
 
 def halo_disk(n, seed=19810614, mh=0.8, rd=0.3, z0=0.03):
     return _ic(lib().bt_ic_halo_disk, n, ctypes.c_uint64(seed), float(mh), float(rd), float(z0))
+
+
+def ensemble(nsys=256, seed=19810614, log2_min=10.0, log2_max=16.0, rmax=22.8):
+    """Independent Plummer systems (Henon units each, mass 1 per system) with
+    N_k = round(2^U[log2_min, log2_max]) from a seeded draw -> (pos, vel,
+    offsets) for Treecode.alloc(n, offsets)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(2.0 ** rng.uniform(log2_min, log2_max, nsys)).astype(np.int64)
+    parts = [plummer(int(nk), seed=seed + 1 + k, rmax=rmax) for k, nk in enumerate(sizes)]
+    pos = np.concatenate([p for p, _ in parts])
+    vel = np.concatenate([v for _, v in parts])
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    return pos, vel, offsets
 
 
 def read_tipsy(path):

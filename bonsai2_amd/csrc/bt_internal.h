@@ -42,7 +42,8 @@ struct DevCounters {
   int flags;   // BT_FLAG_*
   int err;     // 1 = cell capacity, 2 = group capacity, 4 = walk stack
   int queue;   // walk group queue head
-  int pad[8];
+  int stack_max;  // deepest walk stack of the last walk (entries)
+  int pad[7];
 };
 
 struct Ctx {
@@ -146,6 +147,15 @@ struct Ctx {
   // state
   bool have_particles = false, have_tree = false, have_mp = false, have_groups = false,
        have_acc = false;
+
+  // captured step (bt_step_graph).  The tree-order gather writes the particles
+  // into the other of two buffers, so there is one graph per buffer holding
+  // the particles at the start of the step: [0] for graph_pos0, [1] for the other.
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  float4 *graph_pos0 = nullptr;
+  float graph_dt = 0.f;
+  int graph_key[4] = {0, 0, 0, 0};   // nleaf, level_split, ncrit, opening
   int tree_version = 0;
 
   // timing

@@ -99,11 +99,30 @@ int alloc_cells(Ctx &c, int cap) {
   return rc ? BT_ERR_HIP : 0;
 }
 
+void release_graph(Ctx &c) {
+  for (int k = 0; k < 2; k++) {
+    if (c.graph_exec[k]) (void)hipGraphExecDestroy(c.graph_exec[k]);
+    if (c.graph[k]) (void)hipGraphDestroy(c.graph[k]);
+    c.graph_exec[k] = nullptr;
+    c.graph[k] = nullptr;
+  }
+  c.graph_pos0 = nullptr;
+}
+
+// The host-side effect of the tree-order gather: the particle arrays and
+// their reorder targets trade places.
+void swap_particle_buffers(Ctx &c) {
+  std::swap(c.pos, c.pos2);
+  std::swap(c.vel, c.vel2);
+  std::swap(c.id, c.id2);
+}
+
 int ensure_stack(Ctx &c) {
   const int bpc = c.blocks_per_cu > 0 ? c.blocks_per_cu : 8;
   c.walk_blocks = std::max(1, c.num_cus * bpc);
   const size_t need = (size_t)c.walk_blocks * 4 * (size_t)c.stack_ints;
   if (need > c.stack_alloc) {
+    release_graph(c);
     if (dev_alloc(c, &c.stack, need)) return BT_ERR_HIP;
     c.stack_alloc = need;
   }
@@ -215,6 +234,7 @@ int build_checked(Ctx &c) {
   if (c.hc.err & 1) {
     const long long cap = 3LL * c.n + 64LL * c.nsys + 1024;
     if (cap > 0x1FFFFFFF) return fail(c, BT_ERR_CAPACITY, "cell capacity exceeded");
+    release_graph(c);
     free_cell_arrays(c);
     if (alloc_cells(c, (int)cap)) return BT_ERR_HIP;
     launch_levels(c);
@@ -286,6 +306,7 @@ int bt_destroy(bt_ctx *h) {
   if (!h) return 0;
   Ctx &c = h->c;
   (void)hipStreamSynchronize(c.stream);
+  release_graph(c);
   free_particle_arrays(c);
   free_cell_arrays(c);
   dev_free(&c.stack);
@@ -361,6 +382,7 @@ int bt_alloc(bt_ctx *h, int n, int nsys, const int *offsets) {
     off[1] = n;
   }
   (void)hipStreamSynchronize(c.stream);
+  release_graph(c);
   const bool regrow = n > c.cap_n || nsys != c.nsys || !c.pos;
   c.n = n;
   c.nsys = nsys;
@@ -1071,6 +1093,150 @@ int bt_get_opening_check(bt_ctx *h, double out[6]) {
   out[3] = c.last_fallback;
   out[4] = (double)c.fallbacks;
   out[5] = (double)c.checks;
+  return 0;
+}
+
+
+// ---- captured step and autotuner -------------------------------------------
+
+int bt_step_graph(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit, int nsteps) {
+  if (!h || nsteps <= 0) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  c.err.clear();
+  if (!valid_nleaf(nleaf)) return fail(c, BT_ERR_ARG, "nleaf must be 16, 24, 32, 48 or 64");
+  if (ncrit < 1 || ncrit > 64) return fail(c, BT_ERR_ARG, "ncrit must be in [1, 64]");
+  if (!c.have_acc) return fail(c, BT_ERR_STATE, "compute forces (or one bt_step) first");
+  const int key[4] = {nleaf, std::max(1, level_split), ncrit, c.opening};
+  if (c.graph_exec[0] && (c.graph_dt != dt || std::memcmp(key, c.graph_key, sizeof(key)) != 0))
+    release_graph(c);
+  if (!c.graph_exec[0]) {
+    if (ensure_stack(c)) return BT_ERR_HIP;
+    c.nleaf = nleaf;
+    c.ncrit = ncrit;
+    c.level_split = key[1];
+    BT_CK(hipStreamSynchronize(c.stream));
+    const bool timing = c.timing;
+    c.timing = false;
+    float4 *const pos0 = c.pos;
+    int rc = 0;
+    for (int g = 0; g < 2 && !rc; g++) {
+      // graph g starts from the particles in buffer g (0: pos0, 1: the other)
+      if (g == 1) swap_particle_buffers(c);
+      if (hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        if (g == 1) swap_particle_buffers(c);
+        rc = BT_ERR_HIP;
+        break;
+      }
+      launch_kick(c, 0.5f * dt);
+      launch_drift(c, dt);
+      rc = phase_build(c);   // swaps the buffers on the host side
+      if (!rc) rc = phase_multipoles(c);
+      if (!rc) rc = phase_groups(c);
+      if (!rc) rc = phase_forces(c);
+      launch_kick(c, 0.5f * dt);
+      if (hipStreamEndCapture(c.stream, &c.graph[g]) != hipSuccess) rc = BT_ERR_HIP;
+      if (!rc && hipGraphInstantiate(&c.graph_exec[g], c.graph[g], nullptr, nullptr, 0) !=
+                     hipSuccess)
+        rc = BT_ERR_HIP;
+      swap_particle_buffers(c);  // undo the capture's host-side swap
+      if (g == 1) swap_particle_buffers(c);
+    }
+    c.timing = timing;
+    if (rc || c.pos != pos0) {
+      release_graph(c);
+      return fail(c, BT_ERR_HIP, "step capture failed");
+    }
+    c.graph_pos0 = pos0;
+    c.graph_dt = dt;
+    std::memcpy(c.graph_key, key, sizeof(key));
+  }
+  for (int k = 0; k < nsteps; k++) {
+    BT_CK(hipGraphLaunch(c.graph_exec[c.pos == c.graph_pos0 ? 0 : 1], c.stream));
+    swap_particle_buffers(c);
+  }
+  if (read_counters(c)) return BT_ERR_HIP;
+  c.have_tree = c.have_mp = c.have_groups = c.have_acc = false;
+  if (c.hc.err & 3) {
+    release_graph(c);
+    return fail(c, BT_ERR_CAPACITY, "cell or group capacity exceeded: run bt_step to regrow");
+  }
+  if (c.hc.err & 4) return fail(c, BT_ERR_STACK, "walk stack overflow");
+  c.have_tree = c.have_mp = c.have_groups = c.have_acc = true;
+  return 0;
+}
+
+// Times one step with dt = 0 (build, multipoles, groups, walk; particles do
+// not move) on the GPU's own clock: returns the step's and the walk's ms.
+static int timed_evaluation(bt_ctx *h, int nleaf, int level_split, int ncrit, float *total_ms,
+                            float *walk_ms) {
+  Ctx &c = h->c;
+  const int rc = bt_step(h, 0.0f, nleaf, level_split, ncrit);
+  if (rc) return rc;
+  float t = 0.f;
+  for (int k = 0; k < 9; k++) t += c.ms[k];
+  *total_ms = t;
+  *walk_ms = c.ms[7];
+  return 0;
+}
+
+int bt_autotune(bt_ctx *h, int level_split, int flags, double *table, int *ntable, int best[4]) {
+  if (!h || !best) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  c.err.clear();
+  if (!c.have_particles) return fail(c, BT_ERR_STATE, "no particles");
+  (void)flags;
+  const bool timing = c.timing;
+  c.timing = true;
+  static const int kLeaf[] = {16, 32, 64};
+  static const int kCrit[] = {32, 48, 64};
+  static const int kBpc[] = {4, 8, 12, 16};
+  int nt = 0;
+  float best_t = 1e30f;
+  int bl = 16, bc = 64, bb = c.blocks_per_cu > 0 ? c.blocks_per_cu : 8;
+  auto row = [&](int nleaf, int ncrit, int bpc, float tot, float walk) {
+    if (table) {
+      double *r = table + 6 * nt;
+      r[0] = nleaf; r[1] = ncrit; r[2] = bpc; r[3] = tot; r[4] = walk; r[5] = c.hc.stack_max;
+    }
+    nt++;
+  };
+  // NLEAF x NCRIT on the whole force evaluation (second run of each is timed)
+  for (int nleaf : kLeaf)
+    for (int ncrit : kCrit) {
+      float tot = 0.f, walk = 0.f;
+      int rc = timed_evaluation(h, nleaf, level_split, ncrit, &tot, &walk);
+      if (!rc) rc = timed_evaluation(h, nleaf, level_split, ncrit, &tot, &walk);
+      if (rc) { c.timing = timing; return rc; }
+      row(nleaf, ncrit, bb, tot, walk);
+      if (tot < best_t) { best_t = tot; bl = nleaf; bc = ncrit; }
+    }
+  // persistent-grid occupancy for the chosen NLEAF, NCRIT
+  float best_w = 1e30f;
+  int deepest = 0, rc = 0;
+  for (int bpc : kBpc) {
+    c.blocks_per_cu = bpc;
+    float tot = 0.f, w = 0.f;
+    rc = timed_evaluation(h, bl, level_split, bc, &tot, &w);
+    if (!rc) rc = timed_evaluation(h, bl, level_split, bc, &tot, &w);
+    if (rc) { c.timing = timing; return rc; }
+    deepest = std::max(deepest, c.hc.stack_max);
+    row(bl, bc, bpc, tot, w);
+    if (w < best_w) { best_w = w; bb = bpc; }
+  }
+  c.blocks_per_cu = bb;
+  // traversal stack: 4x the deepest stack seen, at least 1024 entries
+  const int stack = std::max(1024, ((4 * deepest + 255) / 256) * 256);
+  c.stack_ints = std::min(stack, 1 << 20);
+  if (ensure_stack(c)) { c.timing = timing; return BT_ERR_HIP; }
+  float tot = 0.f, w = 0.f;
+  rc = timed_evaluation(h, bl, level_split, bc, &tot, &w);  // forces of the chosen set-up
+  c.timing = timing;
+  if (rc) return rc;
+  best[0] = bl;
+  best[1] = bc;
+  best[2] = bb;
+  best[3] = c.stack_ints;
+  if (ntable) *ntable = nt;
   return 0;
 }
 

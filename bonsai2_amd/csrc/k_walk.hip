@@ -379,6 +379,7 @@ void k_walk(WalkArgs a) {
     int top = rc.y;
     int na = 0, nd = 0;          // pending list lengths
     int n_app = 0, n_dir = 0;    // interactions evaluated (stats)
+    int deepest = top;           // stack high-water mark of this group
     bool overflow = false;
     int steps = 0;
     __builtin_amdgcn_s_waitcnt(0);
@@ -463,6 +464,7 @@ void k_walk(WalkArgs a) {
         for (int k = 0; k < nch; k++) stk[base + k] = f + k;
       }
       top += ctot;
+      deepest = max(deepest, top);
 
       // approximate list (P2C)
       const uint64_t am = ballot64(approx);
@@ -533,6 +535,7 @@ void k_walk(WalkArgs a) {
     const double gp = wave_sum(valid ? (double)pi.w * (double)s.w : 0.0);
     a.gcounts[g] = overflow ? make_int2(-1, -1) : make_int2(n_app, n_dir);
     a.gpot[g] = gp;
+    atomicMax(&a.dc->stack_max, lane == 0 ? deepest : 0);
   }
 }
 
@@ -580,6 +583,7 @@ __global__ __launch_bounds__(1024) void k_walk_stats(const int2 *__restrict__ gr
 __global__ void k_walk_reset(DevCounters *dc) {
   if (threadIdx.x == 0) {
     dc->queue = 0;
+    dc->stack_max = 0;
     dc->err &= ~4;
   }
 }

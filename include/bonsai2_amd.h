@@ -159,6 +159,25 @@ int bt_set_walk_options(bt_ctx *ctx, int kahan, int stack_ints, int blocks_per_c
  * same interaction lists; forces agree to FP32 rounding. */
 int bt_set_walk_variant(bt_ctx *ctx, int variant);
 
+/* ---- captured step and autotuner (no reference counterpart) ---------------
+ * bt_step_graph: the step of bt_step (kick, drift, buildTree,
+ * computeMultipoles, makeGroups, computeForces, kick) captured once into a
+ * HIP graph and replayed nsteps times with no host work between kernels.  The
+ * graph is re-captured when dt, nleaf, level_split, ncrit or the opening mode
+ * change, and dropped when arrays are reallocated.  Needs forces (or one
+ * bt_step) first.  The learned angle's per-step FP64 check needs the host and
+ * is not part of the graph.  A capacity overflow returns BT_ERR_CAPACITY:
+ * bt_step then regrows the arrays. */
+int bt_step_graph(bt_ctx *ctx, float dt, int nleaf, int level_split, int ncrit, int nsteps);
+/* Picks NLEAF, NCRIT (3 x 3 candidates, timed zero-dt steps), the persistent
+ * walk grid (4, 8, 12, 16 blocks per CU) and the traversal stack depth
+ * (4x the deepest stack seen) for the current particles, and leaves them set.
+ * table (may be NULL) gets 6 doubles per timed candidate: nleaf, ncrit,
+ * blocks_per_cu, step ms, walk ms, deepest stack; *ntable their count (<= 13).
+ * best = {nleaf, ncrit, blocks_per_cu, stack entries}.  flags: reserved, 0. */
+int bt_autotune(bt_ctx *ctx, int level_split, int flags, double *table, int *ntable,
+                int best[4]);
+
 /* ---- learned opening angle (no reference counterpart) --------------------
  * The reference opens every cell with one angle theta (code/computeForces.cu
  * :30-54).  With opening mode 1 a network picks the angle of each
