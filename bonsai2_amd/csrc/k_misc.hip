@@ -101,10 +101,12 @@ __global__ void k_energy_final(int nsys, int nb, const double *__restrict__ part
   }
 }
 
+// 1/sqrt in FP64: the FP32 hardware estimate (relative error ~1e-7) and two
+// Newton steps (each squares the relative error: ~1e-14, then below FP64
+// rounding).
 __device__ __forceinline__ double rsqrt_fp64(double r2) {
   const float f = (float)fmin(fmax(r2, (double)FLT_MIN), (double)FLT_MAX);
   double y = (double)__builtin_amdgcn_rsqf(f);
-  y = y * (1.5 - 0.5 * r2 * y * y);
   y = y * (1.5 - 0.5 * r2 * y * y);
   y = y * (1.5 - 0.5 * r2 * y * y);
   return y;

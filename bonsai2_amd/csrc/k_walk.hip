@@ -23,7 +23,8 @@
 // The interaction set equals the reference's (the order of traversal does
 // not change which cells are opened); FP32 sums run in a different order.
 // P2P is code/computeForces.cu:58-78 (20 flops), P2C :83-128 (64 flops),
-// rearranged algebraically with the cell's half trace precomputed.
+// rearranged algebraically on a per-cell record with the quadrupole
+// pre-scaled by 3 and the trace terms precomputed (35 VALU instructions).
 #include "bt_device.h"
 #include "bt_internal.h"
 #include "bt_learn.h"
@@ -79,8 +80,8 @@ struct WalkArgs {
 
 struct WaveShared {
   float4 jA[kWave];   // P2C: (com - gc, M)         P2P: (x - gc, m)
-  float4 jB[kWave];   // P2C: (Qxx, Qyy, Qzz, Qxy)
-  float4 jC[kWave];   // P2C: (Qxz, Qyz, trQ/2, 0)
+  float4 jB[kWave];   // P2C: (3Qxx, 3Qyy, 3Qzz, 3Qxy)
+  float4 jC[kWave];   // P2C: (3Qxz, 3Qyz, trQ/2, 1.5 trQ)
   int alist[2 * kWave];
   int dlist[2 * kWave];
   int lpre[kWave];
@@ -134,27 +135,30 @@ __device__ __forceinline__ void fold(Acc4 &s, Acc4 &c, const Acc4 &b) {
 // Monopole + quadrupole of one cell (code/computeForces.cu:83-128):
 //   phi -= M/r - (trQ/2) M/r^3 + 1.5 (dr.Q.dr) M/r^5
 //   a   += (-M/r^3 + 1.5 trQ M/r^5 - 7.5 (dr.Q.dr) M/r^7) dr + 3 M/r^5 Q.dr
-__device__ __forceinline__ void p2c(Acc4 &b, float xi, float yi, float zi, const float4 A,
-                                    const float4 B, const float4 C, float eps2) {
+// evaluated from the cell's walk record, whose quadrupole is pre-scaled by 3
+// (B = 3Qxx, 3Qyy, 3Qzz, 3Qxy; C = 3Qxz, 3Qyz, trQ/2, 1.5 trQ): with
+// dr = x_i - com and u = dr.3Q.dr / r^2 the terms become
+//   phi -= M/r - (trQ/2) M/r^3 + (u/2) M/r^3
+//   a   += (M/r^5 (1.5 trQ - 2.5 u) - M/r^3) dr + M/r^5 (3Q).dr
+__device__ __forceinline__ void p2c_rec(Acc4 &b, float xi, float yi, float zi, const float4 A,
+                                        const float4 B, const float4 C, float eps2) {
   const float dx = xi - A.x, dy = yi - A.y, dz = zi - A.z;
   const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
   const float rinv = __builtin_amdgcn_rsqf(r2);
   const float rinv2 = rinv * rinv;
-  const float mr1 = A.w * rinv;
-  const float mr3 = rinv2 * mr1;
-  const float mr5 = rinv2 * mr3;
-  const float mr7 = rinv2 * mr5;
+  const float m1 = A.w * rinv;
+  const float m3 = rinv2 * m1;
+  const float m5 = rinv2 * m3;
   const float qx = fmaf(B.x, dx, fmaf(B.w, dy, C.x * dz));
   const float qy = fmaf(B.w, dx, fmaf(B.y, dy, C.y * dz));
   const float qz = fmaf(C.x, dx, fmaf(C.y, dy, B.z * dz));
-  const float qrr = fmaf(qx, dx, fmaf(qy, dy, qz * dz));
-  const float hq = C.z;
-  b.w -= fmaf(1.5f * qrr, mr5, fmaf(-hq, mr3, mr1));
-  const float cc = fmaf(-7.5f * qrr, mr7, fmaf(3.0f * hq, mr5, -mr3));
-  const float d2 = 3.0f * mr5;
-  b.x = fmaf(cc, dx, fmaf(d2, qx, b.x));
-  b.y = fmaf(cc, dy, fmaf(d2, qy, b.y));
-  b.z = fmaf(cc, dz, fmaf(d2, qz, b.z));
+  const float u = fmaf(qx, dx, fmaf(qy, dy, qz * dz)) * rinv2;
+  b.w -= m1;
+  b.w = fmaf(-m3, fmaf(0.5f, u, -C.z), b.w);
+  const float cc = fmaf(m5, fmaf(-2.5f, u, C.w), -m3);
+  b.x = fmaf(cc, dx, fmaf(m5, qx, b.x));
+  b.y = fmaf(cc, dy, fmaf(m5, qy, b.y));
+  b.z = fmaf(cc, dz, fmaf(m5, qz, b.z));
 }
 
 // Particle-particle, code/computeForces.cu:58-78.
@@ -178,21 +182,19 @@ __device__ __forceinline__ void eval_p2c(const WalkArgs &a, WaveShared &S, int c
   const int lane = lane_id();
   if (lane < cnt) {
     const int ci = S.alist[lane];
-    const float4 m = a.cmono[ci];
-    const float4 q0 = a.cq0[ci];
-    const float2 q1 = a.cq1[ci];
-    S.jA[lane] = make_float4(m.x - gcx, m.y - gcy, m.z - gcz, m.w);
-    S.jB[lane] = q0;
-    S.jC[lane] = make_float4(q1.x, q1.y, 0.5f * (q0.x + q0.y + q0.z), 0.0f);
+    const float4 r0 = a.cwalk[3 * ci];
+    S.jA[lane] = make_float4(r0.x - gcx, r0.y - gcy, r0.z - gcz, r0.w);
+    S.jB[lane] = a.cwalk[3 * ci + 1];
+    S.jC[lane] = a.cwalk[3 * ci + 2];
   }
   wave_lds_sync();
   Acc4 b = {0.f, 0.f, 0.f, 0.f};
   if (cnt == kWave) {
 #pragma unroll BT_UNROLL_LDS
-    for (int j = 0; j < kWave; j++) p2c(b, xi, yi, zi, S.jA[j], S.jB[j], S.jC[j], a.eps2);
+    for (int j = 0; j < kWave; j++) p2c_rec(b, xi, yi, zi, S.jA[j], S.jB[j], S.jC[j], a.eps2);
   } else {
 #pragma unroll 2
-    for (int j = 0; j < cnt; j++) p2c(b, xi, yi, zi, S.jA[j], S.jB[j], S.jC[j], a.eps2);
+    for (int j = 0; j < cnt; j++) p2c_rec(b, xi, yi, zi, S.jA[j], S.jB[j], S.jC[j], a.eps2);
   }
   fold<KAHAN>(s, c, b);
   wave_lds_sync();
@@ -229,32 +231,6 @@ typedef __attribute__((address_space(4))) const f4v cf4v;
 __device__ __forceinline__ float4 ld_uniform(const cf4v *p, int i) {
   const f4v v = p[i];
   return make_float4(v[0], v[1], v[2], v[3]);
-}
-
-// P2C from a cell record with the quadrupole pre-scaled by 3 (R1 = 3Q diag +
-// 3Qxy, R2 = 3Qxz, 3Qyz, trQ/2, 1.5 trQ); dr = x_i - com and u = dr.3Q.dr/r^2:
-//   phi -= M/r - (trQ/2) M/r^3 + (u/2) M/r^3
-//   a   += (M/r^5 (1.5 trQ - 2.5 u) - M/r^3) dr + M/r^5 (3Q).dr
-// the same terms as the LDS path's p2c.
-__device__ __forceinline__ void p2c_rec(Acc4 &b, float xi, float yi, float zi, const float4 A,
-                                        const float4 B, const float4 C, float eps2) {
-  const float dx = xi - A.x, dy = yi - A.y, dz = zi - A.z;
-  const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-  const float rinv = __builtin_amdgcn_rsqf(r2);
-  const float rinv2 = rinv * rinv;
-  const float m1 = A.w * rinv;
-  const float m3 = rinv2 * m1;
-  const float m5 = rinv2 * m3;
-  const float qx = fmaf(B.x, dx, fmaf(B.w, dy, C.x * dz));
-  const float qy = fmaf(B.w, dx, fmaf(B.y, dy, C.y * dz));
-  const float qz = fmaf(C.x, dx, fmaf(C.y, dy, B.z * dz));
-  const float u = fmaf(qx, dx, fmaf(qy, dy, qz * dz)) * rinv2;
-  b.w -= m1;
-  b.w = fmaf(-m3, fmaf(0.5f, u, -C.z), b.w);
-  const float cc = fmaf(m5, fmaf(-2.5f, u, C.w), -m3);
-  b.x = fmaf(cc, dx, fmaf(m5, qx, b.x));
-  b.y = fmaf(cc, dy, fmaf(m5, qy, b.y));
-  b.z = fmaf(cc, dz, fmaf(m5, qz, b.z));
 }
 
 template <bool KAHAN>
