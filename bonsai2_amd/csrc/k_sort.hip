@@ -8,7 +8,10 @@
 // One pass = 8-bit digit: (1) per-tile histograms in LDS, written digit-major;
 // (2) exclusive scan of the digit-major histogram = global scatter offsets;
 // (3) per-tile stable ranks from 8 wave ballots (lanes with equal digits form
-// a peer mask) plus per-wave digit counters in LDS, then the scatter.
+// a peer mask) plus per-wave digit counters in LDS; each pair is first placed
+// at its sorted position within the tile in LDS, then the tile is written out
+// in that order, so consecutive threads store consecutive addresses of one
+// digit's run (whole cache lines instead of one scattered store per pair).
 // A tile is 256 threads x 16 rounds = 4096 pairs; HBM traffic per pass is
 // ~8 B (histogram read) + 12 B read + 12 B written per pair.
 #include <climits>
@@ -39,6 +42,37 @@ __device__ __forceinline__ int digit_of(uint64_t key, int val, const DigitSrc &s
   return (int)((key >> s.shift) & 0xFFull);
 }
 
+// ---- scans ---------------------------------------------------------------
+
+struct OpSum {
+  static __device__ __forceinline__ int id() { return 0; }
+  static __device__ __forceinline__ int op(int a, int b) { return a + b; }
+};
+struct OpMax {
+  static __device__ __forceinline__ int id() { return INT_MIN; }
+  static __device__ __forceinline__ int op(int a, int b) { return a > b ? a : b; }
+};
+
+// Inclusive scan across a 256-thread block; returns inclusive value, *total = block total.
+template <class Op>
+__device__ __forceinline__ int block_incl_scan(int v, int *sh_wave, int *total) {
+  const int l = lane_id(), w = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int t = __shfl_up(v, d, kWave);
+    if (l >= d) v = Op::op(t, v);
+  }
+  if (l == kWave - 1) sh_wave[w] = v;
+  __syncthreads();
+  int pre = Op::id();
+  for (int k = 0; k < w; k++) pre = Op::op(pre, sh_wave[k]);
+  int tot = Op::id();
+  for (int k = 0; k < (int)(blockDim.x >> 6); k++) tot = Op::op(tot, sh_wave[k]);
+  *total = tot;
+  __syncthreads();
+  return Op::op(pre, v);
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__restrict__ keys,
                                                              const int *__restrict__ vals,
                                                              int n, DigitSrc src, int nblocks,
@@ -60,11 +94,28 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint64_t *__restrict__ keys, const int *__restrict__ vals, int n, DigitSrc src,
     int nblocks, const int *__restrict__ goff, uint64_t *__restrict__ keys_out,
     int *__restrict__ vals_out) {
-  __shared__ int run[256];        // running base per digit for this tile
+  __shared__ uint64_t skeys[kSortTile];   // the tile in sorted order
+  __shared__ int svals[kSortTile];
+  __shared__ int run[256];        // running tile-local position per digit
+  __shared__ int tstart[256];     // first tile-local position of each digit
+  __shared__ int gbase[256];      // global position of this tile's first pair of each digit
   __shared__ int wcnt[4][256];    // per-wave digit counts -> offsets
+  __shared__ int sh_wave[4];
   const int tid = threadIdx.x;
   const int w = tid >> 6;
-  run[tid] = goff[tid * nblocks + blockIdx.x];
+  const int hist_n = 256 * nblocks;
+  {
+    // this tile's count of digit tid from the scanned histogram (digit-major:
+    // the next entry is the same digit's next tile, or the next digit's first)
+    const int idx = tid * nblocks + blockIdx.x;
+    const int g0 = goff[idx];
+    const int g1 = idx + 1 < hist_n ? goff[idx + 1] : n;
+    int tot;
+    const int incl = block_incl_scan<OpSum>(g1 - g0, sh_wave, &tot);
+    tstart[tid] = incl - (g1 - g0);
+    run[tid] = incl - (g1 - g0);
+    gbase[tid] = g0;
+  }
   const int base = blockIdx.x * kSortTile;
   for (int r = 0; r < kSortRounds; r++) {
 #pragma unroll
@@ -101,44 +152,23 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     }
     __syncthreads();
     if (valid) {
-      const int dst = wcnt[w][d] + rank;
-      keys_out[dst] = key;
-      vals_out[dst] = val;
+      const int pos = wcnt[w][d] + rank;
+      skeys[pos] = key;
+      svals[pos] = val;
     }
-    __syncthreads();
+    __syncthreads();   // wcnt is cleared by the next round
+  }
+  const int tile_n = min(kSortTile, n - base);
+  for (int j = tid; j < tile_n; j += kSortThreads) {
+    const uint64_t key = skeys[j];
+    const int val = svals[j];
+    const int d = digit_of(key, val, src);
+    const int dst = gbase[d] + (j - tstart[d]);
+    keys_out[dst] = key;
+    vals_out[dst] = val;
   }
 }
 
-// ---- scans ---------------------------------------------------------------
-
-struct OpSum {
-  static __device__ __forceinline__ int id() { return 0; }
-  static __device__ __forceinline__ int op(int a, int b) { return a + b; }
-};
-struct OpMax {
-  static __device__ __forceinline__ int id() { return INT_MIN; }
-  static __device__ __forceinline__ int op(int a, int b) { return a > b ? a : b; }
-};
-
-// Inclusive scan across a 256-thread block; returns inclusive value, *total = block total.
-template <class Op>
-__device__ __forceinline__ int block_incl_scan(int v, int *sh_wave, int *total) {
-  const int l = lane_id(), w = threadIdx.x >> 6;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const int t = __shfl_up(v, d, kWave);
-    if (l >= d) v = Op::op(t, v);
-  }
-  if (l == kWave - 1) sh_wave[w] = v;
-  __syncthreads();
-  int pre = Op::id();
-  for (int k = 0; k < w; k++) pre = Op::op(pre, sh_wave[k]);
-  int tot = Op::id();
-  for (int k = 0; k < (int)(blockDim.x >> 6); k++) tot = Op::op(tot, sh_wave[k]);
-  *total = tot;
-  __syncthreads();
-  return Op::op(pre, v);
-}
 
 template <class Op>
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int *__restrict__ in, int n,
