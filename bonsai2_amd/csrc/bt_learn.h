@@ -7,7 +7,7 @@
 // [theta_lo, theta_hi], and the cell is opened iff ds^2 < (l/theta + s)^2
 // (the same improved Barnes-Hut test with a per-pair angle).
 //
-// Features (8, in [0, 1], computed with IEEE +,-,*,/,sqrt only):
+// Features (8, in [0, 1]):
 //   o = com - group centre, d = max(|o|, 1e-30)
 //   x0 = rg / (rg + d)        rg = |group half-size vector|
 //   x1 = np / 64
@@ -62,7 +62,7 @@ struct GroupGeom {
   float rg;        // |group half-size vector|
   float np64;      // np / 64
   int axis;        // longest half size: 0 x, 1 y, 2 z (ties -> lower axis)
-  float R, Msys;   // system scales
+  float R, inv_msys;   // system scales
 };
 
 __device__ inline GroupGeom make_group_geom(float gcx, float gcy, float gcz, float gsx,
@@ -75,26 +75,30 @@ __device__ inline GroupGeom make_group_geom(float gcx, float gcy, float gcz, flo
   g.np64 = (float)np * 0.015625f;
   g.axis = (gsx >= gsy && gsx >= gsz) ? 0 : (gsy >= gsz ? 1 : 2);
   g.R = scale.x;
-  g.Msys = scale.y;
+  g.inv_msys = 1.0f / scale.y;
   return g;
 }
 
+// The hardware reciprocal and square root (about 1 ulp) replace the IEEE
+// divide and square-root sequences: the features feed an FP16 network, and
+// this keeps their cost at a few instructions per cell.
 __device__ inline void pair_features(const GroupGeom &g, float4 m, float l, float4 q0, float2 q1,
                                      float x[8]) {
 #pragma clang fp contract(off)
   const float ox = m.x - g.gcx, oy = m.y - g.gcy, oz = m.z - g.gcz;
-  const float d = fmaxf(sqrtf(ox * ox + oy * oy + oz * oz), 1e-30f);
-  x[0] = g.rg / (g.rg + d);
+  const float d = fmaxf(__builtin_amdgcn_sqrtf(ox * ox + oy * oy + oz * oz), 1e-30f);
+  const float id = __builtin_amdgcn_rcpf(d);
+  x[0] = g.rg * __builtin_amdgcn_rcpf(g.rg + d);
   x[1] = g.np64;
-  x[2] = l / (l + d);
-  x[3] = d / (d + g.R);
-  x[4] = fminf(m.w / g.Msys, 1.0f);
-  const float qn = sqrtf(q0.x * q0.x + q0.y * q0.y + q0.z * q0.z +
-                         2.0f * (q0.w * q0.w + q1.x * q1.x + q1.y * q1.y));
-  x[5] = qn / (qn + m.w * l * l + 1e-30f);
-  x[6] = fabsf(oz) / d;
+  x[2] = l * __builtin_amdgcn_rcpf(l + d);
+  x[3] = d * __builtin_amdgcn_rcpf(d + g.R);
+  x[4] = fminf(m.w * g.inv_msys, 1.0f);
+  const float qn = __builtin_amdgcn_sqrtf(q0.x * q0.x + q0.y * q0.y + q0.z * q0.z +
+                                          2.0f * (q0.w * q0.w + q1.x * q1.x + q1.y * q1.y));
+  x[5] = qn * __builtin_amdgcn_rcpf(qn + m.w * l * l + 1e-30f);
+  x[6] = fabsf(oz) * id;
   const float ok = g.axis == 0 ? ox : (g.axis == 1 ? oy : oz);
-  x[7] = fabsf(ok) / d;
+  x[7] = fabsf(ok) * id;
 }
 
 __device__ __forceinline__ float silu(float z) {

@@ -175,11 +175,14 @@ void or_geo_keys(int n, const F4 *pos, const float *box, uint64_t *keys) {
 // stats[6] = ncells, nnodes, nleaves, nlevels, depth_cap_hit, nbodies_leaf.
 // Returns 0, or -1 if ncell_max is too small.
 // ---------------------------------------------------------------------------
+// ocb: octant scratch of n bytes; a call uses only [b, e), disjoint from
+// every concurrent call's range and released before its children run.
 static void stable_partition_rec(std::vector<int> &idx, std::vector<int> &tmp,
-                                 const F4 *pos, int b, int e, BoxF box, int level) {
+                                 std::vector<unsigned char> &ocb, const F4 *pos, int b, int e,
+                                 BoxF box, int level) {
   if (e - b <= 1 || level >= kMaxLevel) return;
   int cnt[8] = {0};
-  std::vector<unsigned char> oc(e - b);
+  unsigned char *oc = ocb.data() + b;
   for (int i = b; i < e; i++) {
     const F4 &p = pos[idx[i]];
     const int o = octant_of(box.cx, box.cy, box.cz, p.x, p.y, p.z);
@@ -199,10 +202,10 @@ static void stable_partition_rec(std::vector<int> &idx, std::vector<int> &tmp,
     const BoxF cb = child_box(box, o);
     const int cb0 = off[o], ce0 = off[o + 1];
     if (par) {
-#pragma omp task shared(idx, tmp)
-      stable_partition_rec(idx, tmp, pos, cb0, ce0, cb, level + 1);
+#pragma omp task shared(idx, tmp, ocb)
+      stable_partition_rec(idx, tmp, ocb, pos, cb0, ce0, cb, level + 1);
     } else {
-      stable_partition_rec(idx, tmp, pos, cb0, ce0, cb, level + 1);
+      stable_partition_rec(idx, tmp, ocb, pos, cb0, ce0, cb, level + 1);
     }
   }
   if (par) {
@@ -215,12 +218,13 @@ int or_build_tree(int n, const F4 *pos, int nroot, const int *offsets,
                   int *order, U4 *cells, int *level_range, int *root_cells,
                   int *stats) {
   std::vector<int> idx(n), tmp(n);
+  std::vector<unsigned char> ocb(n);
   for (int i = 0; i < n; i++) idx[i] = i;
 #pragma omp parallel
 #pragma omp single
   for (int r = 0; r < nroot; r++) {
     const BoxF b{boxes[4 * r], boxes[4 * r + 1], boxes[4 * r + 2], boxes[4 * r + 3]};
-    stable_partition_rec(idx, tmp, pos, offsets[r], offsets[r + 1], b, 0);
+    stable_partition_rec(idx, tmp, ocb, pos, offsets[r], offsets[r + 1], b, 0);
   }
   for (int i = 0; i < n; i++) order[i] = idx[i];
 
@@ -508,8 +512,9 @@ int or_make_groups(int n, const uint64_t *sorted_keys, const int *seg_id,
 // code/computeForces.cu:30-54 with a per-pair angle (or if it holds < 3
 // particles, :284-285).
 //
-// Features (8, all in [0, 1], IEEE +,-,*,/,sqrt only so they are exact on
-// any IEEE machine): with o = com - group centre, d = |o| (>= 1e-30):
+// Features (8, all in [0, 1]; IEEE +,-,*,/,sqrt here, the hardware reciprocal
+// and square root (about 1 ulp) on the GPU): with o = com - group centre,
+// d = |o| (>= 1e-30):
 //   x0 = rg / (rg + d)         rg = |group half-size vector|   group radius
 //   x1 = np / 64                                              group count
 //   x2 = l / (l + d)           l = cell bbox size              cell size
