@@ -75,7 +75,7 @@ def collect(n, theta, eps, ngroups_sample, seed, theta_lo, theta_hi, ic="plummer
     return col["feat"][order], col["target"][order], info
 
 
-def train(x, y, steps=4000, batch=8192, lr=3e-3, seed=1):
+def train(x, y, steps=64000, batch=8192, lr=3e-3, seed=1):
     """Trains on the GPU; returns (params, loss curve)."""
     t = Treecode(0.01, 0.5, 64)
     t.mlp_init(seed, float(np.mean(y)))
@@ -108,12 +108,12 @@ def evaluate(t, params, budget, theta_lo, theta_hi, n, samples=4096, seed=5):
 
 def calibrate(params, theta_lo, theta_hi, p99_target, n=1 << 22, theta=0.5, eps=0.005,
               seed=SEED + 1, samples=4096):
-    """Largest budget (log-spaced grid) whose learned walk keeps p99 <= p99_target
-    on a Plummer sphere of n particles."""
+    """Largest budget (grid of 8 steps per decade) whose learned walk keeps
+    p99 <= p99_target on a Plummer sphere of n particles."""
     pos, vel = plummer(n, seed=seed)
     t = built_tree(pos, vel, theta, eps)
     best, table = None, []
-    for b in 10.0 ** np.arange(-7.0, -1.9, 0.25):
+    for b in 10.0 ** np.arange(-6.5, -2.9, 0.125):
         p50, p99, fl = evaluate(t, params, b, theta_lo, theta_hi, n, samples)
         table.append(dict(budget=float(b), p50=p50, p99=p99, flops=fl))
         if p99 <= p99_target:
@@ -139,7 +139,7 @@ def load(path=DEFAULT_PATH):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", default=DEFAULT_PATH)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=64000)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--theta-lo", type=float, default=0.25)
