@@ -201,7 +201,7 @@ void launch_kick(Ctx &c, float dt_half);
 void launch_drift(Ctx &c, float dt);
 void launch_energies(Ctx &c);
 int direct_slices(int ns);   // d_out of launch_direct: ns * (slices + 1) double4
-void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out);
+void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out, bool check = false);
 
 // host-side helpers
 int grid_for(int n, int block, int cap);

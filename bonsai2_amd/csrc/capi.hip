@@ -204,7 +204,7 @@ int learned_check(Ctx &c) {
   BT_CK(hipMemcpyAsync(c.d_check_idx, idx.data(), sizeof(int) * ns, hipMemcpyHostToDevice,
                        c.stream));
   double *res = c.d_check + (size_t)ns * (direct_slices(ns) + 1) * 4;
-  launch_direct(c, ns, c.d_check_idx, c.d_check);
+  launch_direct(c, ns, c.d_check_idx, c.d_check, true);
   launch_learn_check(c, ns, c.d_check_idx, c.d_check, res);
   BT_LAUNCH_CK();
   BT_CK(hipMemcpyAsync(c.last_check, res, sizeof(double) * 3, hipMemcpyDeviceToHost, c.stream));

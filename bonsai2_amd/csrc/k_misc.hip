@@ -101,47 +101,53 @@ __global__ void k_energy_final(int nsys, int nb, const double *__restrict__ part
   }
 }
 
-// 1/sqrt in FP64: the FP32 hardware estimate (relative error ~1e-7) and two
-// Newton steps (each squares the relative error: ~1e-14, then below FP64
-// rounding).
+// 1/sqrt in FP64: the FP32 hardware estimate (relative error ~1e-7) refined
+// by NEWTON Newton steps (each squares the relative error: ~5e-15 after one,
+// below FP64 rounding after two).
+template <int NEWTON>
 __device__ __forceinline__ double rsqrt_fp64(double r2) {
   const float f = (float)fmin(fmax(r2, (double)FLT_MIN), (double)FLT_MAX);
   double y = (double)__builtin_amdgcn_rsqf(f);
-  y = y * (1.5 - 0.5 * r2 * y * y);
-  y = y * (1.5 - 0.5 * r2 * y * y);
+#pragma unroll
+  for (int k = 0; k < NEWTON; k++) y = y * (1.5 - 0.5 * r2 * y * y);
   return y;
 }
 
-// grid (ceil(ns/256), kDirectSlices): each block sums a slice of the source
-// index range for 256 targets.  Sources outside a target's system are skipped.
+// grid (ceil(ns/256), slices): each block sums a slice of the source index
+// range for 256 targets, the slice staged in LDS as FP64.  FOREST: sources
+// outside a target's system are skipped.
+template <int NEWTON, bool FOREST>
 __global__ __launch_bounds__(kBlock) void k_direct(int ns, const int *__restrict__ idx, int n,
                                                    int nsys, const int *__restrict__ off,
                                                    const float4 *__restrict__ pos, double eps2,
                                                    double4 *__restrict__ part) {
-  __shared__ float4 tile[kBlock];
+  __shared__ double4 tile[kBlock];
   const int t = blockIdx.x * kBlock + threadIdx.x;
   const bool active = t < ns;
   const int k = active ? idx[t] : 0;
-  const int r = nsys > 1 ? system_of(k, nsys, off) : 0;
-  const int lo = off[r], hi = off[r + 1];
+  const int r = FOREST ? system_of(k, nsys, off) : 0;
+  const int lo = FOREST ? off[r] : 0, hi = FOREST ? off[r + 1] : n;
   const float4 pt = pos[k];
+  const double px = pt.x, py = pt.y, pz = pt.z;
   const int chunk = (n + gridDim.y - 1) / gridDim.y;
   const int s0 = blockIdx.y * chunk, s1 = min(n, s0 + chunk);
   double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
   for (int j0 = s0; j0 < s1; j0 += kBlock) {
     const int j = j0 + threadIdx.x;
-    tile[threadIdx.x] = j < s1 ? pos[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 pj = j < s1 ? pos[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    tile[threadIdx.x] = make_double4(pj.x, pj.y, pj.z, pj.w);
     __syncthreads();
     const int cnt = min(kBlock, s1 - j0);
-    for (int q = 0; q < cnt; q++) {
-      const int jj = j0 + q;
-      if (jj < lo || jj >= hi) continue;
-      const float4 p = tile[q];
-      const double dx = (double)p.x - pt.x, dy = (double)p.y - pt.y, dz = (double)p.z - pt.z;
+    const int q0 = FOREST ? max(0, lo - j0) : 0;
+    const int q1 = FOREST ? min(cnt, hi - j0) : cnt;
+#pragma unroll 4
+    for (int q = q0; q < q1; q++) {
+      const double4 p = tile[q];
+      const double dx = p.x - px, dy = p.y - py, dz = p.z - pz;
       const double r2 = dx * dx + dy * dy + dz * dz + eps2;
-      const double ri = rsqrt_fp64(r2);
-      const double mr = (double)p.w * ri;
-      const double mr3 = mr * ri * ri;
+      const double ri = rsqrt_fp64<NEWTON>(r2);
+      const double mr = p.w * ri;
+      const double mr3 = mr * (ri * ri);
       ax += mr3 * dx; ay += mr3 * dy; az += mr3 * dz; ph -= mr;
     }
     __syncthreads();
@@ -191,11 +197,16 @@ int direct_slices(int ns) {
 }
 
 // d_out must hold ns * (direct_slices(ns) + 1) double4: the result, then the partials.
-void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out) {
+void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out, bool check) {
   double4 *part = reinterpret_cast<double4 *>(d_out) + ns;
   const double eps2 = (double)c.eps * (double)c.eps;
   const int slices = direct_slices(ns);
-  hipLaunchKernelGGL(k_direct, dim3((ns + kBlock - 1) / kBlock, slices), dim3(kBlock), 0,
+  // the per-step check of a learned walk needs relative errors of ~1e-6:
+  // one Newton step (5e-15) is ample; the measurement path keeps two
+  void (*kern)(int, const int *, int, int, const int *, const float4 *, double, double4 *);
+  if (c.nsys > 1) kern = check ? k_direct<1, true> : k_direct<2, true>;
+  else kern = check ? k_direct<1, false> : k_direct<2, false>;
+  hipLaunchKernelGGL(kern, dim3((ns + kBlock - 1) / kBlock, slices), dim3(kBlock), 0,
                      c.stream, ns, d_idx, c.n, c.nsys, c.d_off, c.pos_ph, eps2, part);
   hipLaunchKernelGGL(k_direct_reduce, dim3((ns + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream,
                      ns, slices, part, reinterpret_cast<double4 *>(d_out));

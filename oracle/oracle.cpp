@@ -639,6 +639,7 @@ struct LearnCfg {
   float shift;         // log2(theta) offset (error budget)
   float theta_lo, theta_hi;
   const F2 *group_scale;  // per group (R, M_sys)
+  long long *visits;   // optional: {cells tested, cells whose decision needed the network}
 };
 
 // 1 / theta for one pair, clamped to [1/theta_hi, 1/theta_lo]
@@ -751,6 +752,18 @@ static void walk_impl(int ngroups, const I2 *groups, const int *group_cells,
             split = open_test(csize[c], gcx, gcy, gcz, gsx, gsy, gsz, cell_ls + c,
                               learned_inv_theta(*lc, xf)) ||
                     ((int)cd.w - (int)cd.z < 3);
+            if (lc->visits) {
+              const bool open_hi = open_test(csize[c], gcx, gcy, gcz, gsx, gsy, gsz, cell_ls + c,
+                                             1.0f / lc->theta_hi);
+              const bool open_lo = open_test(csize[c], gcx, gcy, gcz, gsx, gsy, gsz, cell_ls + c,
+                                             1.0f / lc->theta_lo);
+#pragma omp atomic
+              lc->visits[0]++;
+              if (open_lo && !open_hi) {
+#pragma omp atomic
+                lc->visits[1]++;
+              }
+            }
           } else {
             split = open_test(csize[c], gcx, gcy, gcz, gsx, gsy, gsz,
                               lsp ? lsp + c : nullptr, it) ||
@@ -980,10 +993,12 @@ void or_walk_learned(int ngroups, const I2 *groups, const int *group_cells,
                      const U4 *cells, const F4 *csize, const F4 *mono, const F4 *quad0,
                      const F2 *quad1, const F2 *cell_ls, float eps2, const float *params,
                      float shift, float theta_lo, float theta_hi, const F2 *group_scale,
-                     F4 *acc_out, long long *stats_out, double *pot_out, int *counts_out) {
+                     F4 *acc_out, long long *stats_out, double *pot_out, int *counts_out,
+                     long long *visits) {
   std::vector<float> pr(params, params + kMlpParams);
   for (auto &v : pr) v = f16r(v);
-  const LearnCfg lc{pr.data(), shift, theta_lo, theta_hi, group_scale};
+  if (visits) visits[0] = visits[1] = 0;
+  const LearnCfg lc{pr.data(), shift, theta_lo, theta_hi, group_scale, visits};
   walk_impl<float>(ngroups, groups, group_cells, level_range, pos_ph, pos_tree, cells, csize,
                    mono, quad0, quad1, eps2, cell_ls, nullptr, acc_out, stats_out, pot_out,
                    counts_out, &lc);

@@ -54,7 +54,7 @@ def lib():
         L.or_mlp_grad.argtypes = [ctypes.c_int, P, P, P, ctypes.c_int, P, P]
         L.or_walk_learned.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, P, P, P, P,
                                       ctypes.c_float, P, ctypes.c_float, ctypes.c_float,
-                                      ctypes.c_float, P, P, P, P, P]
+                                      ctypes.c_float, P, P, P, P, P, P]
         L.or_learn_pairs.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, P, P, P, P, P, P,
                                      ctypes.c_float, ctypes.c_float, ctypes.c_float, P,
                                      ctypes.c_int, P, P, P, P, P]
@@ -338,8 +338,10 @@ def group_scale(groups, R, msys):
 
 
 def walk_learned(groups, level_range, pos_ph, pos_tree, cells, size, mono, q0, q1, ls, eps,
-                 params, shift, theta_lo, theta_hi, gscale, group_cells=None):
-    """The walk with the learned per-pair angle -> (acc, stats, pot, counts)."""
+                 params, shift, theta_lo, theta_hi, gscale, group_cells=None, visits=None):
+    """The walk with the learned per-pair angle -> (acc, stats, pot, counts).
+    visits (int64[2], optional) receives {cells tested, cells whose decision
+    depended on the network}."""
     groups = np.ascontiguousarray(groups, np.int32)
     ng = len(groups)
     acc = np.zeros((len(pos_ph), 4), np.float32)
@@ -359,7 +361,7 @@ def walk_learned(groups, level_range, pos_ph, pos_tree, cells, size, mono, q0, q
         _ptr(np.ascontiguousarray(params, np.float32)), ctypes.c_float(shift),
         ctypes.c_float(theta_lo), ctypes.c_float(theta_hi),
         _ptr(np.ascontiguousarray(gscale, np.float32)), _ptr(acc), _ptr(stats), _ptr(pot),
-        _ptr(cnt))
+        _ptr(cnt), _ptr(visits))
     return acc, stats, float(pot[0]), cnt
 
 
