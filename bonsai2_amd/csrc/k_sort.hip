@@ -7,11 +7,13 @@
 //
 // One pass = 8-bit digit: (1) per-tile histograms in LDS, written digit-major;
 // (2) exclusive scan of the digit-major histogram = global scatter offsets;
-// (3) per-tile stable ranks from 8 wave ballots (lanes with equal digits form
-// a peer mask) plus per-wave digit counters in LDS; each pair is first placed
-// at its sorted position within the tile in LDS, then the tile is written out
-// in that order, so consecutive threads store consecutive addresses of one
-// digit's run (whole cache lines instead of one scattered store per pair).
+// (3) stable ranks: each wave ranks its contiguous quarter of the tile with 8
+// ballots per 64 pairs (lanes with equal digits form a peer mask) against its
+// own digit counters in LDS, keeping the pairs in registers; the waves' counts
+// are then chained per digit, each pair is placed at its sorted position
+// within the tile in LDS, and the tile is written out in that order, so
+// consecutive threads store consecutive addresses of one digit's run (whole
+// cache lines instead of one scattered store per pair).
 // A tile is 256 threads x 16 rounds = 4096 pairs; HBM traffic per pass is
 // ~8 B (histogram read) + 12 B read + 12 B written per pair.
 #include <climits>
@@ -96,13 +98,13 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     int *__restrict__ vals_out) {
   __shared__ uint64_t skeys[kSortTile];   // the tile in sorted order
   __shared__ int svals[kSortTile];
-  __shared__ int run[256];        // running tile-local position per digit
   __shared__ int tstart[256];     // first tile-local position of each digit
   __shared__ int gbase[256];      // global position of this tile's first pair of each digit
-  __shared__ int wcnt[4][256];    // per-wave digit counts -> offsets
+  __shared__ int wcnt[4][256];    // per-wave digit counts -> per-wave digit bases
   __shared__ int sh_wave[4];
   const int tid = threadIdx.x;
   const int w = tid >> 6;
+  const int lane = lane_id();
   const int hist_n = 256 * nblocks;
   {
     // this tile's count of digit tid from the scanned histogram (digit-major:
@@ -113,15 +115,19 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     int tot;
     const int incl = block_incl_scan<OpSum>(g1 - g0, sh_wave, &tot);
     tstart[tid] = incl - (g1 - g0);
-    run[tid] = incl - (g1 - g0);
     gbase[tid] = g0;
   }
-  const int base = blockIdx.x * kSortTile;
-  for (int r = 0; r < kSortRounds; r++) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) wcnt[k][tid] = 0;
-    __syncthreads();
-    const int i = base + r * kSortThreads + tid;
+  for (int k = 0; k < 4; k++) wcnt[k][tid] = 0;
+  __syncthreads();
+  // Each wave ranks its own contiguous quarter of the tile (16 rounds of 64
+  // pairs) against its private digit counters; pairs stay in registers.
+  const int wbase = blockIdx.x * kSortTile + w * (kSortRounds * kWave);
+  uint64_t kk[kSortRounds];
+  int vv[kSortRounds], dd[kSortRounds], rr[kSortRounds];
+#pragma unroll
+  for (int r = 0; r < kSortRounds; r++) {
+    const int i = wbase + r * kWave + lane;
     const bool valid = i < n;
     uint64_t key = 0;
     int val = 0, d = 0;
@@ -138,26 +144,37 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
       peers &= bit ? m : ~m;
     }
     const int rank = __popcll(peers & lanemask_lt());
-    if (valid && rank == 0) wcnt[w][d] = __popcll(peers);
-    __syncthreads();
-    {
-      int acc = run[tid];
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const int c = wcnt[k][tid];
-        wcnt[k][tid] = acc;
-        acc += c;
-      }
-      run[tid] = acc;
-    }
-    __syncthreads();
-    if (valid) {
-      const int pos = wcnt[w][d] + rank;
-      skeys[pos] = key;
-      svals[pos] = val;
-    }
-    __syncthreads();   // wcnt is cleared by the next round
+    const int before = wcnt[w][d];
+    wave_lds_sync();
+    if (valid && rank == 0) wcnt[w][d] = before + __popcll(peers);
+    wave_lds_sync();
+    kk[r] = key;
+    vv[r] = val;
+    dd[r] = valid ? d : -1;
+    rr[r] = before + rank;
   }
+  __syncthreads();
+  {
+    // digit tid: the waves' pairs follow each other in tile order
+    int acc = tstart[tid];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int c = wcnt[k][tid];
+      wcnt[k][tid] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortRounds; r++) {
+    if (dd[r] >= 0) {
+      const int pos = wcnt[w][dd[r]] + rr[r];
+      skeys[pos] = kk[r];
+      svals[pos] = vv[r];
+    }
+  }
+  __syncthreads();
+  const int base = blockIdx.x * kSortTile;
   const int tile_n = min(kSortTile, n - base);
   for (int j = tid; j < tile_n; j += kSortThreads) {
     const uint64_t key = skeys[j];
