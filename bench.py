@@ -177,6 +177,8 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device with torch (a plain copy kernel "
                          "of known size, for scaling HBM byte counters)")
+    ap.add_argument("--monopole", action="store_true",
+                    help="monopole-only P2C (the reference without QUADRUPOLE; 20 flops each)")
     ap.add_argument("--kahan-compare", action="store_true",
                     help="also report the fixed-theta walk with and without compensated sums")
     ap.add_argument("--compare-steps", type=int, default=3,
@@ -214,6 +216,10 @@ def main():
     t.set_walk_variant(args.walk_variant)
     t.alloc(n)
     t.ptcl_h2d(pos, vel)
+    if args.monopole:
+        args.opening = "fixed"
+        t.set_quadrupole(False)
+    flops_p2c = 20.0 if args.monopole else 64.0   # code/treecode.cu:149-153
     learned = args.opening == "learned"
     if learned:
         from bonsai2_amd import learn
@@ -240,7 +246,7 @@ def main():
         phase.append(tm)
         walk_ms.append(tm["walk"])
         _, tot = t.walk_stats()
-        flops.append(20.0 * tot[0] + 64.0 * tot[1])
+        flops.append(20.0 * tot[0] + flops_p2c * tot[1])
         inter.append(tot[0] + tot[1])
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / args.steps
@@ -340,7 +346,7 @@ def main():
                                "of %d samples per step, fallback theta %g at p99 > %g" % (
                                    budget, th_lo, th_hi, args.check_samples, w["theta"],
                                    args.p99_max)) if learned else "fixed theta",
-                   "kahan": not args.no_kahan,
+                   "kahan": not args.no_kahan, "quadrupole": not args.monopole,
                    "walk_jdata": "scalar" if args.walk_variant == 1 else "lds"},
         "interactions_per_s": float((inter / (walk_ms * 1e-3)).mean()),
         "walk_ms": float(walk_ms.mean()),

@@ -59,6 +59,7 @@ _SIGS = {
     "bt_get_timings": [P, P],
     "bt_set_walk_options": [P, I, I, I],
     "bt_set_walk_variant": [P, I],
+    "bt_set_quadrupole": [P, I],
     "bt_step_graph": [P, F, I, I, I, I],
     "bt_autotune": [P, I, I, P, P, P],
     "bt_mlp_init": [P, ctypes.c_uint64, F],
@@ -178,6 +179,10 @@ class Treecode:
     def set_walk_variant(self, variant):
         """0 = LDS-staged j-data (default), 1 = scalar-load j-data."""
         self._ck(self._L.bt_set_walk_variant(self._h, int(variant)), "set_walk_variant")
+
+    def set_quadrupole(self, on=True):
+        """code/Treecode.h:14-16: quadrupole P2C (default) or monopole only."""
+        self._ck(self._L.bt_set_quadrupole(self._h, 1 if on else 0), "set_quadrupole")
 
     # -- captured step and autotuner (no reference counterpart) ---------------
     def step_graph(self, dt, nLeaf=16, levelSplit=5, nCrit=64, nsteps=1):

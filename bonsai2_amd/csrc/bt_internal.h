@@ -114,6 +114,7 @@ struct Ctx {
   int blocks_per_cu = 0;
   int kahan = 1;
   int walk_variant = 0;        // 0: j-data staged in LDS, 1: j-data through scalar loads
+  int quadrupole = 1;          // 0: monopole-only P2C (the reference without QUADRUPOLE)
   int num_cus = 256;
   unsigned long long *d_wstats = nullptr;  // direct_sum, direct_max, approx_sum, approx_max
   double *d_wpot = nullptr;

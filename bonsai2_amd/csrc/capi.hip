@@ -365,6 +365,15 @@ int bt_set_walk_variant(bt_ctx *h, int variant) {
   return 0;
 }
 
+int bt_set_quadrupole(bt_ctx *h, int on) {
+  if (!h) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  if (on != 0 && on != 1) return fail(c, BT_ERR_ARG, "quadrupole must be 0 or 1");
+  if (!on && c.opening == 1) return fail(c, BT_ERR_STATE, "the learned angle needs quadrupoles");
+  c.quadrupole = on;
+  return 0;
+}
+
 int bt_alloc(bt_ctx *h, int n, int nsys, const int *offsets) {
   if (!h) return BT_ERR_ARG;
   Ctx &c = h->c;
@@ -1056,6 +1065,7 @@ int bt_set_opening(bt_ctx *h, int mode, float budget, float theta_lo, float thet
   if (mode != 0 && mode != 1) return fail(c, BT_ERR_ARG, "opening mode must be 0 or 1");
   if (mode == 1) {
     if (!c.have_mlp) return fail(c, BT_ERR_STATE, "no network parameters");
+    if (!c.quadrupole) return fail(c, BT_ERR_STATE, "the learned angle needs quadrupoles");
     if (!(budget > 0.f) || !(theta_lo > 0.f) || !(theta_hi > theta_lo))
       return fail(c, BT_ERR_ARG, "bad budget or theta band");
     c.learn_shift = std::log2(budget) / 3.0f;

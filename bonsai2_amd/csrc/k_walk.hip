@@ -222,6 +222,31 @@ __device__ __forceinline__ void eval_p2p(const WalkArgs &a, WaveShared &S, int c
   wave_lds_sync();
 }
 
+// Monopole-only P2C (the reference built without QUADRUPOLE,
+// code/computeForces.cu:198-225): the cell's mass at its centre of mass,
+// evaluated like a particle (20 flops).
+template <bool KAHAN>
+__device__ __forceinline__ void eval_p2c_mono(const WalkArgs &a, WaveShared &S, int cnt,
+                                              float gcx, float gcy, float gcz, float xi,
+                                              float yi, float zi, Acc4 &s, Acc4 &c) {
+  const int lane = lane_id();
+  if (lane < cnt) {
+    const float4 r0 = a.cwalk[3 * S.alist[lane]];
+    S.jA[lane] = make_float4(r0.x - gcx, r0.y - gcy, r0.z - gcz, r0.w);
+  }
+  wave_lds_sync();
+  Acc4 b = {0.f, 0.f, 0.f, 0.f};
+  if (cnt == kWave) {
+#pragma unroll 8
+    for (int j = 0; j < kWave; j++) p2p(b, xi, yi, zi, S.jA[j], a.eps2);
+  } else {
+#pragma unroll 2
+    for (int j = 0; j < cnt; j++) p2p(b, xi, yi, zi, S.jA[j], a.eps2);
+  }
+  fold<KAHAN>(s, c, b);
+  wave_lds_sync();
+}
+
 // Cell records and particles do not change during the walk: reading them
 // through the constant address space lets the compiler fetch a wave-uniform
 // record with s_load (scalar cache) instead of a vector load.
@@ -275,11 +300,12 @@ __device__ __forceinline__ void eval_p2p_s(const WalkArgs &a, WaveShared &S, int
 
 // One batch of cnt P2C (list in S.alist) or P2P (S.dlist) interactions.  The
 // LDS path takes i relative to the group centre, the scalar path absolute i.
-template <bool KAHAN, bool SJ>
+template <bool KAHAN, bool SJ, bool MONO>
 __device__ __forceinline__ void batch_p2c(const WalkArgs &a, WaveShared &S, int cnt, float gcx,
                                           float gcy, float gcz, float xi, float yi, float zi,
                                           Acc4 &s, Acc4 &c) {
-  if (SJ) eval_p2c_s<KAHAN>(a, S, cnt, xi, yi, zi, s, c);
+  if (MONO) eval_p2c_mono<KAHAN>(a, S, cnt, gcx, gcy, gcz, xi, yi, zi, s, c);
+  else if (SJ) eval_p2c_s<KAHAN>(a, S, cnt, xi, yi, zi, s, c);
   else eval_p2c<KAHAN>(a, S, cnt, gcx, gcy, gcz, xi, yi, zi, s, c);
 }
 
@@ -293,7 +319,8 @@ __device__ __forceinline__ void batch_p2p(const WalkArgs &a, WaveShared &S, int 
 
 // LEARN: each popped cell gets its own opening angle from the network
 // (bt_learn.h), evaluated by the wave on the matrix cores for up to 64 cells.
-template <bool KAHAN, bool SJ, bool LEARN>
+// MONO: monopole-only P2C (the reference's build without QUADRUPOLE).
+template <bool KAHAN, bool SJ, bool LEARN, bool MONO = false>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(LEARN ? BT_LEARN_WAVES_PER_EU : BT_WALK_WAVES_PER_EU)))
 void k_walk(WalkArgs a) {
@@ -483,7 +510,7 @@ void k_walk(WalkArgs a) {
       }
       wave_lds_sync();
       if (na >= kWave) {
-        batch_p2c<KAHAN, SJ>(a, S, kWave, gcx, gcy, gcz, xi, yi, zi, s, cmp);
+        batch_p2c<KAHAN, SJ, MONO>(a, S, kWave, gcx, gcy, gcz, xi, yi, zi, s, cmp);
         n_app += kWave;
         na -= kWave;
         if (lane < na) S.alist[lane] = S.alist[kWave + lane];
@@ -499,7 +526,7 @@ void k_walk(WalkArgs a) {
       nd = 0;
     }
     if (na > 0) {
-      batch_p2c<KAHAN, SJ>(a, S, na, gcx, gcy, gcz, xi, yi, zi, s, cmp);
+      batch_p2c<KAHAN, SJ, MONO>(a, S, na, gcx, gcy, gcz, xi, yi, zi, s, cmp);
       n_app += na;
     }
     if (nd > 0) {
@@ -596,7 +623,9 @@ void launch_walk(Ctx &c) {
   a.dc = c.dc;
   hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, c.stream, c.dc);
   void (*kern)(WalkArgs);
-  if (c.opening == 1 && c.have_mlp && c.walk_variant == 1)
+  if (!c.quadrupole)
+    kern = c.kahan ? k_walk<true, false, false, true> : k_walk<false, false, false, true>;
+  else if (c.opening == 1 && c.have_mlp && c.walk_variant == 1)
     kern = c.kahan ? k_walk<true, true, true> : k_walk<false, true, true>;
   else if (c.opening == 1 && c.have_mlp)
     kern = c.kahan ? k_walk<true, false, true> : k_walk<false, false, true>;

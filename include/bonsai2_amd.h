@@ -158,6 +158,11 @@ int bt_set_walk_options(bt_ctx *ctx, int kahan, int stack_ints, int blocks_per_c
  * the immutable cell / particle arrays and used from SGPRs.  Both evaluate the
  * same interaction lists; forces agree to FP32 rounding. */
 int bt_set_walk_variant(bt_ctx *ctx, int variant);
+/* code/Treecode.h:14-16 (#define QUADRUPOLE): 1 (default) = monopole +
+ * quadrupole P2C (code/computeForces.cu:83-128, 64 flops); 0 = monopole-only
+ * P2C, the cell evaluated as a point mass at its centre of mass
+ * (code/computeForces.cu:198-225, 20 flops; code/treecode.cu:149-153). */
+int bt_set_quadrupole(bt_ctx *ctx, int on);
 
 /* ---- captured step and autotuner (no reference counterpart) ---------------
  * bt_step_graph: the step of bt_step (kick, drift, buildTree,

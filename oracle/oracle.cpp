@@ -709,7 +709,7 @@ static void walk_impl(int ngroups, const I2 *groups, const int *group_cells,
                       const F2 *quad1, float eps2, const F2 *cell_ls,
                       const float *group_inv_theta, F4 *acc_out,
                       long long *stats_out, double *pot_out, int *counts_out,
-                      const LearnCfg *lc = nullptr) {
+                      const LearnCfg *lc = nullptr, bool monopole_only = false) {
   long long dsum = 0, asum = 0, dmax = 0, amax = 0;
   double pot = 0;
 #pragma omp parallel reduction(+ : dsum, asum, pot) reduction(max : dmax, amax)
@@ -786,7 +786,10 @@ static void walk_impl(int ngroups, const I2 *groups, const int *group_cells,
       for (int i = pbeg; i < pbeg + np; i++) {
         A acc[4] = {0, 0, 0, 0};
         const float px = pos_ph[i].x, py = pos_ph[i].y, pz = pos_ph[i].z;
-        for (int c : alist) p2c(acc, px, py, pz, mono[c], quad0[c], quad1[c], eps2);
+        if (monopole_only)  // code/computeForces.cu:198-225: the cell as a point mass
+          for (int c : alist) p2p(acc, px, py, pz, mono[c], eps2);
+        else
+          for (int c : alist) p2c(acc, px, py, pz, mono[c], quad0[c], quad1[c], eps2);
         for (int p : dlist) p2p(acc, px, py, pz, pos_tree[p], eps2);
         acc_out[i] = F4{(float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]};
         pot += 0.5 * (double)pos_ph[i].w * (double)(float)acc[3];
@@ -805,15 +808,15 @@ void or_walk(int ngroups, const I2 *groups, const int *group_cells, const int *l
              const F4 *pos_ph, const F4 *pos_tree, const U4 *cells, const F4 *csize,
              const F4 *mono, const F4 *quad0, const F2 *quad1, float eps2, int acc_type,
              const F2 *cell_ls, const float *group_inv_theta, F4 *acc_out,
-             long long *stats_out, double *pot_out, int *counts_out) {
+             long long *stats_out, double *pot_out, int *counts_out, int monopole_only) {
   if (acc_type == 0)
     walk_impl<float>(ngroups, groups, group_cells, level_range, pos_ph, pos_tree, cells, csize,
                      mono, quad0, quad1, eps2, cell_ls, group_inv_theta, acc_out, stats_out,
-                     pot_out, counts_out);
+                     pot_out, counts_out, nullptr, monopole_only != 0);
   else
     walk_impl<double>(ngroups, groups, group_cells, level_range, pos_ph, pos_tree, cells,
                       csize, mono, quad0, quad1, eps2, cell_ls, group_inv_theta, acc_out,
-                      stats_out, pot_out, counts_out);
+                      stats_out, pot_out, counts_out, nullptr, monopole_only != 0);
 }
 
 // ---------------------------------------------------------------------------

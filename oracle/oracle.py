@@ -43,7 +43,7 @@ def lib():
         L.or_make_groups.argtypes = [ctypes.c_int, P, P, ctypes.c_int, ctypes.c_int, P]
         L.or_make_groups.restype = ctypes.c_int
         L.or_walk.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, P, P, P, ctypes.c_float,
-                              ctypes.c_int, P, P, P, P, P, P]
+                              ctypes.c_int, P, P, P, P, P, P, ctypes.c_int]
         L.or_direct_fp64.argtypes = [ctypes.c_int, P, ctypes.c_int, P, ctypes.c_double, P]
         L.or_direct_ld.argtypes = [ctypes.c_int, P, ctypes.c_int, P, ctypes.c_double, P]
         L.or_energies.argtypes = [ctypes.c_int, P, P, P, ctypes.c_double, P, P]
@@ -193,8 +193,11 @@ def make_groups(sorted_keys, level_split, ncrit, seg_id=None):
 
 
 def walk(groups, level_range, pos_ph, pos_tree, cells, size, mono, q0, q1, eps,
-         acc_type=0, group_cells=None, cell_ls=None, group_inv_theta=None, counts=False):
-    """code/computeForces.cu:30-727 -> (acc[N,4] PH order, stats[4], pot[, counts])."""
+         acc_type=0, group_cells=None, cell_ls=None, group_inv_theta=None, counts=False,
+         monopole=False):
+    """code/computeForces.cu:30-727 -> (acc[N,4] PH order, stats[4], pot[, counts]).
+    monopole: P2C with the cell's monopole only (the reference without
+    QUADRUPOLE, code/computeForces.cu:198-225)."""
     groups = np.ascontiguousarray(groups, np.int32)
     ng = len(groups)
     level_range = np.ascontiguousarray(level_range, np.int32).reshape(-1)
@@ -218,7 +221,7 @@ def walk(groups, level_range, pos_ph, pos_tree, cells, size, mono, q0, q1, eps,
                   _ptr(np.ascontiguousarray(q0, np.float32)),
                   _ptr(np.ascontiguousarray(q1, np.float32)), ctypes.c_float(eps2),
                   int(acc_type), _ptr(cell_ls), _ptr(group_inv_theta), _ptr(acc), _ptr(stats),
-                  _ptr(pot), _ptr(cnt))
+                  _ptr(pot), _ptr(cnt), int(bool(monopole)))
     if counts:
         return acc, stats, float(pot[0]), cnt
     return acc, stats, float(pot[0])

@@ -148,6 +148,30 @@ def test_scalar_load_walk_matches(run):
     assert np.array_equal(t.acc(2), a0)  # the default walk is deterministic
 
 
+def test_monopole_only_walk_matches(run, oracle):
+    """bt_set_quadrupole(0): the reference built without QUADRUPOLE
+    (code/Treecode.h:14-16, code/computeForces.cu:198-225) against the oracle's
+    monopole walk on the same tree: counts equal, forces within 1e-5."""
+    t, ref = run["t"], run["ref"]
+    t.set_quadrupole(False)
+    try:
+        st = t.computeForces()
+        acc = t.acc(2)
+    finally:
+        t.set_quadrupole(True)
+        t.computeForces()
+    acc_o, st_o, _ = oracle.walk(ref["groups"], ref["tree"].level_range, ref["pos_ph"],
+                                 ref["pos_tree"], ref["tree"].cells, ref["size"], ref["mono"],
+                                 ref["q0"], ref["q1"], run["eps"], monopole=True)
+    n = run["n"]
+    exp = (st_o[0] / n, st_o[1], st_o[2] / n, st_o[3])
+    for a, b in zip(st, exp):
+        assert abs(a - b) <= 1e-4 * max(1.0, abs(b)), (st, exp)
+    assert rel_rms(acc[:, :3], acc_o[:, :3]) <= 1e-5
+    # the quadrupole term is a real correction at this theta
+    assert rel_rms(acc[:, :3], ref["acc"][:, :3]) > 1e-5
+
+
 def test_potential_matches(run):
     assert abs(run["t"].grav_potential - run["ref"]["pot"]) <= 1e-5 * abs(run["ref"]["pot"])
 

@@ -274,6 +274,33 @@ def test_quadrupole_error_scales_as_third_power(oracle):
     assert errs[0] / errs[1] > 6 and errs[1] / errs[2] > 6  # ~8 per doubling
 
 
+def test_monopole_mode_is_a_point_mass_with_second_order_error(oracle):
+    """Monopole-only P2C (the reference without QUADRUPOLE,
+    code/computeForces.cu:198-225) is the cell's mass at its centre of mass:
+    equal to that closed form, and its truncation error falls as (a/d)^2."""
+    a = 0.05
+    errs = []
+    for d in (1.0, 2.0, 4.0):
+        pts = np.array([[a, a / 3, 0, 0.3], [-a, -a / 3, 0, 0.3], [a / 2, -a, a / 4, 0.2],
+                        [-a / 2, a, -a / 4, 0.2]], np.float64)
+        cell = _cell(pts.astype(np.float32))
+        size, mono, q0, q1, ls = oracle.multipoles(cell, pts.astype(np.float32), 0.5)
+        target = np.array([[d, 0.3 * d, 0.1 * d, 1e-9]], np.float32)
+        lr = np.zeros(64, np.int32)
+        lr[2:4] = [0, 0]
+        acc, st, _ = oracle.walk(np.array([[0, 1]], np.int32), lr, target,
+                                 pts.astype(np.float32), cell, size * 0, mono, q0, q1, 0.0,
+                                 acc_type=1, monopole=True)
+        assert st[2] == 1  # one P2C
+        x = target[0, :3].astype(np.float64)
+        com, M = mono[0, :3].astype(np.float64), float(mono[0, 3])
+        point = M * (com - x) / np.linalg.norm(com - x) ** 3
+        assert np.allclose(acc[0, :3], point, rtol=1e-6)
+        ex = sum(p[3] * (p[:3] - x) / np.linalg.norm(p[:3] - x) ** 3 for p in pts)
+        errs.append(np.linalg.norm(acc[0, :3] - ex) / np.linalg.norm(ex))
+    assert 3 < errs[0] / errs[1] < 6 and 3 < errs[1] / errs[2] < 6  # ~4 per doubling
+
+
 def test_direct_fp64_against_long_double(oracle, bt):
     pos, _ = bt.plummer(300, seed=8)
     idx = np.arange(300, dtype=np.int32)
