@@ -43,6 +43,7 @@ struct DevCounters {
   int err;     // 1 = cell capacity, 2 = group capacity, 4 = walk stack
   int queue;   // walk group queue head
   int stack_max;  // deepest walk stack of the last walk (entries)
+  int xq[8];      // walk group queues, one per XCD range
   int pad[7];
 };
 

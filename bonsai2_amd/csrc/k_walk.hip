@@ -50,6 +50,28 @@ constexpr int kMaxWalkSteps = 1 << 20;
 #ifndef BT_LEARN_WAVES_PER_EU
 #define BT_LEARN_WAVES_PER_EU 4
 #endif
+#ifndef BT_XCD_QUEUE
+#define BT_XCD_QUEUE 1
+#endif
+constexpr int kXcd = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+
+// Next group for this wave.  With per-XCD queues the groups (PH order) are cut
+// into kXcd contiguous ranges; a wave drains the range of its own XCD first,
+// so neighbouring groups share that XCD's L2, then helps the other ranges.
+// Every lane joins the atomics (lane 0 adds 1) and all branches are uniform.
+__device__ __forceinline__ int next_group(DevCounters *dc, int ngroups, int home, int &xr) {
+  const int lane = lane_id();
+  if (!BT_XCD_QUEUE) return wave_uniform(atomicAdd(&dc->queue, lane == 0 ? 1 : 0));
+  while (xr < kXcd) {
+    const int x = (home + xr) & (kXcd - 1);
+    const int lo = (int)((long long)ngroups * x / kXcd);
+    const int hi = (int)((long long)ngroups * (x + 1) / kXcd);
+    const int k = wave_uniform(atomicAdd(&dc->xq[x], lane == 0 ? 1 : 0));
+    if (lo + k < hi) return lo + k;
+    xr++;
+  }
+  return ngroups;
+}
 
 struct WalkArgs {
   int n, nsys;
@@ -352,8 +374,10 @@ void k_walk(WalkArgs a) {
   // the other lanes to loop on a stale group index.  Every lane therefore
   // joins the queue atomic (lane 0 adds 1, the others 0) and the per-group
   // stores below are branch-free.
+  const int home = (int)(blockIdx.x % kXcd);
+  int xr = 0;
   for (;;) {
-    const int g = wave_uniform(atomicAdd(&a.dc->queue, lane == 0 ? 1 : 0));
+    const int g = next_group(a.dc, ngroups, home, xr);
     if (g >= ngroups) break;
     const int2 grp = a.groups[g];
     const int pbeg = grp.x, np = max(grp.y, 1);
@@ -587,6 +611,7 @@ __global__ void k_walk_reset(DevCounters *dc) {
   if (threadIdx.x == 0) {
     dc->queue = 0;
     dc->stack_max = 0;
+    for (int x = 0; x < kXcd; x++) dc->xq[x] = 0;
     dc->err &= ~4;
   }
 }
