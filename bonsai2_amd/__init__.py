@@ -59,6 +59,8 @@ _SIGS = {
     "bt_get_timings": [P, P],
     "bt_set_walk_options": [P, I, I, I],
     "bt_set_walk_variant": [P, I],
+    "bt_set_walk_lds_stack": [P, I],
+    "bt_get_walk_stack_max": [P, P],
     "bt_set_quadrupole": [P, I],
     "bt_step_graph": [P, F, I, I, I, I],
     "bt_autotune": [P, I, I, P, P, P],
@@ -179,6 +181,16 @@ class Treecode:
     def set_walk_variant(self, variant):
         """0 = LDS-staged j-data (default), 1 = scalar-load j-data."""
         self._ck(self._L.bt_set_walk_variant(self._h, int(variant)), "set_walk_variant")
+
+    def set_walk_lds_stack(self, entries):
+        """Learned-angle walk: stack entries per wave kept in LDS (512..1024)."""
+        self._ck(self._L.bt_set_walk_lds_stack(self._h, int(entries)), "set_walk_lds_stack")
+
+    def walk_stack_max(self):
+        """Deepest traversal stack (entries) of any group in the last walk."""
+        v = np.zeros(1, np.int32)
+        self._ck(self._L.bt_get_walk_stack_max(self._h, _p(v)), "walk_stack_max")
+        return int(v[0])
 
     def set_quadrupole(self, on=True):
         """code/Treecode.h:14-16: quadrupole P2C (default) or monopole only."""

@@ -34,6 +34,12 @@ struct LearnBuffers {
 };
 
 // Device-resident counters of one context (one allocation).
+// Learned walk: stack entries per wave kept in LDS (bt_set_walk_lds_stack).
+constexpr int kLdsStack = 1024;
+#ifndef BT_LDS_STACK_MIN
+#define BT_LDS_STACK_MIN 512   // the children one walk step can push (8 * 64)
+#endif
+
 struct DevCounters {
   int lvl_start[kLevels + 2];
   int lvl_count[kLevels + 2];
@@ -111,6 +117,7 @@ struct Ctx {
   int *stack = nullptr;
   size_t stack_alloc = 0;
   int stack_ints = 16384;
+  int lds_stack = kLdsStack;   // learned walk: stack entries per wave kept in LDS
   int walk_blocks = 0;         // persistent grid (blocks of 256)
   int blocks_per_cu = 0;
   int kahan = 1;

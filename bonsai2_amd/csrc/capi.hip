@@ -358,6 +358,22 @@ int bt_set_walk_options(bt_ctx *h, int kahan, int stack_ints, int blocks_per_cu)
   return 0;
 }
 
+int bt_set_walk_lds_stack(bt_ctx *h, int entries) {
+  if (!h) return BT_ERR_ARG;
+  if (entries < BT_LDS_STACK_MIN || entries > kLdsStack)
+    return fail(h->c, BT_ERR_ARG, "LDS stack entries out of range");
+  h->c.lds_stack = entries;
+  return 0;
+}
+
+int bt_get_walk_stack_max(bt_ctx *h, int *entries) {
+  if (!h || !entries) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  if (c.dc && read_counters(c)) return BT_ERR_HIP;
+  *entries = c.hc.stack_max;
+  return 0;
+}
+
 int bt_set_walk_variant(bt_ctx *h, int variant) {
   if (!h) return BT_ERR_ARG;
   if (variant < 0 || variant > 1) return fail(h->c, BT_ERR_ARG, "walk variant must be 0 or 1");

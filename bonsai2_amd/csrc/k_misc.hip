@@ -10,7 +10,7 @@
 //     -m/eps of the P2P kernel with dr = 0 (code/computeForces.cu:64-72);
 //     partial sums are reduced in a fixed order (deterministic);
 //   * FP64 direct sum: the error oracle for the walk, sources staged through
-//     LDS, 1/sqrt by a float v_rsq seed refined with two Newton steps in FP64.
+//     LDS, 1/sqrt by the v_rsq_f64 estimate refined with Newton steps in FP64.
 #include <cfloat>
 
 #include <algorithm>
@@ -101,15 +101,16 @@ __global__ void k_energy_final(int nsys, int nb, const double *__restrict__ part
   }
 }
 
-// 1/sqrt in FP64: the FP32 hardware estimate (relative error ~1e-7) refined
-// by NEWTON Newton steps (each squares the relative error: ~5e-15 after one,
-// below FP64 rounding after two).
+// 1/sqrt in FP64: the hardware FP64 estimate v_rsq_f64 (relative error
+// about 2^-23) refined by NEWTON Newton steps (each squares the relative
+// error: ~3e-14 after one, below FP64 rounding after two).  r2 is clamped to
+// FLT_MIN so that a zero distance (no softening, the self pair) stays finite.
 template <int NEWTON>
 __device__ __forceinline__ double rsqrt_fp64(double r2) {
-  const float f = (float)fmin(fmax(r2, (double)FLT_MIN), (double)FLT_MAX);
-  double y = (double)__builtin_amdgcn_rsqf(f);
+  const double x = fmax(r2, (double)FLT_MIN);
+  double y = __builtin_amdgcn_rsq(x);
 #pragma unroll
-  for (int k = 0; k < NEWTON; k++) y = y * (1.5 - 0.5 * r2 * y * y);
+  for (int k = 0; k < NEWTON; k++) y = y * (1.5 - 0.5 * x * y * y);
   return y;
 }
 
@@ -202,7 +203,7 @@ void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out, bool check) 
   const double eps2 = (double)c.eps * (double)c.eps;
   const int slices = direct_slices(ns);
   // the per-step check of a learned walk needs relative errors of ~1e-6:
-  // one Newton step (5e-15) is ample; the measurement path keeps two
+  // one Newton step (~3e-14) is ample; the measurement path keeps two
   void (*kern)(int, const int *, int, int, const int *, const float4 *, double, double4 *);
   if (c.nsys > 1) kern = check ? k_direct<1, true> : k_direct<2, true>;
   else kern = check ? k_direct<1, false> : k_direct<2, false>;

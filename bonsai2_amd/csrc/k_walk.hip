@@ -5,7 +5,8 @@
 // retired_groupCount loop, :514-522).  Per group:
 //   * the group box is the min/max of its particles (:539-555);
 //   * the walk starts at the group's level-1 cells (:259-263) and keeps a
-//     per-wave LIFO of cells in global memory; each step pops up to 64 cells,
+//     per-wave LIFO of cells in global memory (the learned-angle walk keeps
+//     its top 1024 entries in LDS); each step pops up to 64 cells,
 //     one per lane, and applies the reference's rules: open iff
 //     ds^2 < cellOp^2 (:30-54) or the cell holds < 3 particles (:284-285);
 //     opened nodes push their children, unopened cells join the P2C list,
@@ -54,6 +55,13 @@ constexpr int kMaxWalkSteps = 1 << 20;
 #define BT_XCD_QUEUE 1
 #endif
 constexpr int kXcd = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+// Entries of a learned walk's stack held in LDS per wave (kLdsStack = 1024,
+// 4 KiB).  The learned walk runs at 4 waves per SIMD, so 4 blocks share a CU's
+// 160 KiB of LDS; the fixed-angle walk runs at up to 8 and keeps its whole
+// stack in global memory.  A deeper stack spills its oldest entries to the
+// wave's global stack.  The part used (WalkArgs::lstk_cap) is at least the
+// 8 * 64 children one step can push, so a step's pushes always fit after a
+// spill.
 
 // Next group for this wave.  With per-XCD queues the groups (PH order) are cut
 // into kXcd contiguous ranges; a wave drains the range of its own XCD first,
@@ -97,6 +105,7 @@ struct WalkArgs {
   double *gpot;
   int *stack;
   int stack_ints;
+  int lstk_cap;          // LDS part of a learned walk's stack, entries per wave
   DevCounters *dc;
 };
 
@@ -348,10 +357,16 @@ __attribute__((amdgpu_waves_per_eu(LEARN ? BT_LEARN_WAVES_PER_EU : BT_WALK_WAVES
 void k_walk(WalkArgs a) {
   __shared__ WaveShared sh[kWavesPerBlock];
   __shared__ float s_mlp[LEARN ? 64 : 1];
+  __shared__ int s_stack[LEARN ? kWavesPerBlock * kLdsStack : 1];
   const int lane = lane_id();
   const int wv = threadIdx.x >> 6;
   WaveShared &S = sh[wv];
+  // LSTK: the top of the stack (up to kLdsStack entries) lives in LDS, the
+  // older entries below it in global memory (stk[0, gtop)); otherwise the
+  // whole stack is in global memory (stk[0, top))
+  constexpr bool LSTK = LEARN;
   int *stk = a.stack + (size_t)(blockIdx.x * kWavesPerBlock + wv) * (size_t)a.stack_ints;
+  int *lstk = s_stack + (LSTK ? wv * kLdsStack : 0);
   half8_t ma1, ma2_0, ma2_1;
   float mb3 = 0.f;
   if (LEARN) {
@@ -402,8 +417,12 @@ void k_walk(WalkArgs a) {
 
     Acc4 s = {0.f, 0.f, 0.f, 0.f}, cmp = {0.f, 0.f, 0.f, 0.f};
     const int2 rc = a.root_cells[sys];
-    if (lane < rc.y) stk[lane] = rc.x + lane;
+    if (lane < rc.y) {
+      if (LSTK) lstk[lane] = rc.x + lane;
+      else stk[lane] = rc.x + lane;
+    }
     int top = rc.y;
+    int gtop = 0;                // LSTK: entries spilled to global memory
     int na = 0, nd = 0;          // pending list lengths
     int n_app = 0, n_dir = 0;    // interactions evaluated (stats)
     int deepest = top;           // stack high-water mark of this group
@@ -411,12 +430,25 @@ void k_walk(WalkArgs a) {
     int steps = 0;
     __builtin_amdgcn_s_waitcnt(0);
 
-    while (top > 0) {
+    while (top > 0 || gtop > 0) {
       // every cell is visited at most once per group; a longer walk means a
       // corrupt tree, so stop rather than loop
       if (++steps > kMaxWalkSteps) {
         overflow = true;
         break;
+      }
+      if (LSTK && top < kWave && gtop > 0) {
+        // top the LDS part up with the newest spilled entries (slid in below
+        // the ones left), so that the step pops the same cells as it would
+        // from one stack
+        const int r = min(gtop, a.lstk_cap / 2);
+        const int v = lane < top ? lstk[lane] : 0;
+        wave_lds_sync();
+        if (lane < top) lstk[r + lane] = v;
+        for (int k = lane; k < r; k += kWave) lstk[k] = stk[gtop - r + k];
+        gtop -= r;
+        top += r;
+        wave_lds_sync();
       }
       const int take = min(kWave, top);
       top -= take;
@@ -425,25 +457,19 @@ void k_walk(WalkArgs a) {
       uint4 cd = make_uint4(0, 0, 0, 0);
       float4 cs = make_float4(0.f, 0.f, 0.f, 0.f);
       if (use) {
-        ci = stk[top + lane];
+        ci = LSTK ? lstk[top + lane] : stk[top + lane];
         cd = a.cells[ci];
         cs = a.csize[ci];
       }
       bool split = false, leaf = false;
       if (LEARN) {
-        // every lane evaluates the network (idle lanes on a zero cell)
-        float4 m = make_float4(0.f, 0.f, 0.f, 0.f), q0 = m;
-        float2 q1 = make_float2(0.f, 0.f), lsv = q1;
-        if (use) {
-          m = a.cmono[ci];
-          q0 = a.cq0[ci];
-          q1 = a.cq1[ci];
-          lsv = a.cls[ci];
-        }
-        float x[8];
-        pair_features(geom, m, lsv.x, q0, q1, x);
-        // cells opened even at theta_hi or accepted even at theta_lo do not
-        // need the network: a 32-lane block without undecided cells is skipped
+        float2 lsv = make_float2(0.f, 0.f);
+        if (use) lsv = a.cls[ci];
+        // A cell opened even at theta_hi or accepted even at theta_lo is
+        // decided without the network: (l/theta + s)^2 is monotone in 1/theta,
+        // so every angle of the band gives it the same answer.  Only a step
+        // with an undecided cell loads the moments and computes features, and
+        // only a 32-lane block holding one evaluates the network.
         const float ds2 = box_ds2(cs, gcx, gcy, gcz, gsx, gsy, gsz);
         float op_lo, op_hi;
         {
@@ -455,9 +481,22 @@ void k_walk(WalkArgs a) {
         }
         const bool undecided = use && ds2 < op_lo && ds2 >= op_hi;
         const uint64_t um = ballot64(undecided);
-        const float y = mlp_wave(ma1, ma2_0, ma2_1, s_mlp, mb3, pack_features(x),
-                                 (um & 0xFFFFFFFFull) != 0, (um >> 32) != 0);
-        const float it = learned_inv_theta(y, a.shift, a.it_min, a.it_max);
+        float it = a.it_min;
+        if (um != 0) {
+          // every lane evaluates the network (decided and idle lanes on a zero cell)
+          float4 m = make_float4(0.f, 0.f, 0.f, 0.f), q0 = m;
+          float2 q1 = make_float2(0.f, 0.f);
+          if (undecided) {
+            m = a.cmono[ci];
+            q0 = a.cq0[ci];
+            q1 = a.cq1[ci];
+          }
+          float x[8];
+          pair_features(geom, m, lsv.x, q0, q1, x);
+          const float y = mlp_wave(ma1, ma2_0, ma2_1, s_mlp, mb3, pack_features(x),
+                                   (um & 0xFFFFFFFFull) != 0, (um >> 32) != 0);
+          it = learned_inv_theta(y, a.shift, a.it_min, a.it_max);
+        }
         float op;
         {
 #pragma clang fp contract(off)
@@ -465,8 +504,8 @@ void k_walk(WalkArgs a) {
           op = op * op;
         }
         if (use) {
-          split = open_cell(make_float4(cs.x, cs.y, cs.z, op), gcx, gcy, gcz, gsx, gsy, gsz) ||
-                  ((int)cd.w - (int)cd.z < 3);
+          // the opening test of open_cell with this pair's radius (same ds^2)
+          split = ds2 < op || ((int)cd.w - (int)cd.z < 3);
           leaf = cell_leaf(cd);
         }
       } else if (use) {
@@ -481,17 +520,40 @@ void k_walk(WalkArgs a) {
       const int nch = push ? cell_nchild(cd) : 0;
       const int chi = wave_incl_scan(nch);
       const int ctot = __shfl(chi, kWave - 1, kWave);
-      if (top + ctot > a.stack_ints) {
+      if (LSTK && top + ctot > a.lstk_cap) {
+        // spill the oldest LDS entries to the global stack, leaving a quarter
+        // of the LDS part free after this step's pushes where possible
+        const int sp = min(top, top + ctot - a.lstk_cap + a.lstk_cap / 4);
+        if (top - sp + ctot > a.lstk_cap || gtop + sp > a.stack_ints) {
+          overflow = true;
+          break;
+        }
+        for (int k = lane; k < sp; k += kWave) stk[gtop + k] = lstk[k];
+        wave_lds_sync();
+        // move the rest down; each chunk is read in full before it is written
+        for (int k0 = 0; k0 < top - sp; k0 += kWave) {
+          const int k = k0 + lane;
+          const int v = k < top - sp ? lstk[sp + k] : 0;
+          wave_lds_sync();
+          if (k < top - sp) lstk[k] = v;
+          wave_lds_sync();
+        }
+        gtop += sp;
+        top -= sp;
+      } else if (!LSTK && top + ctot > a.stack_ints) {
         overflow = true;
         break;
       }
       if (push) {
         const int f = cell_first(cd);
         const int base = top + chi - nch;
-        for (int k = 0; k < nch; k++) stk[base + k] = f + k;
+        for (int k = 0; k < nch; k++) {
+          if (LSTK) lstk[base + k] = f + k;
+          else stk[base + k] = f + k;
+        }
       }
       top += ctot;
-      deepest = max(deepest, top);
+      deepest = max(deepest, top + gtop);
 
       // approximate list (P2C)
       const uint64_t am = ballot64(approx);
@@ -645,6 +707,7 @@ void launch_walk(Ctx &c) {
   a.gpot = c.gpot;
   a.stack = c.stack;
   a.stack_ints = c.stack_ints;
+  a.lstk_cap = c.lds_stack;
   a.dc = c.dc;
   hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, c.stream, c.dc);
   void (*kern)(WalkArgs);

@@ -12,7 +12,13 @@ budget then sets the angle: theta = exp2(y + log2(budget) / 3).
     python -m bonsai2_amd.learn --out bonsai2_amd/theta_mlp.json
 
 writes the parameters, the band and the calibrated budget; Treecode.use_learned_theta
-reads them back.
+reads them back.  The band used at inference may be wider than the band the
+pairs were collected in (the network extrapolates smoothly past theta_hi);
+
+    python -m bonsai2_amd.learn --recalibrate --theta-hi 1.2
+
+keeps the trained parameters of --out and recalibrates the budget for the new
+inference band.
 """
 import argparse
 import json
@@ -136,6 +142,26 @@ def load(path=DEFAULT_PATH):
             float(d["theta_hi"]))
 
 
+def recalibrate(path, theta_lo, theta_hi, p99_target):
+    """Calibrates the budget of a trained network for another inference band."""
+    with open(path) as f:
+        d = json.load(f)
+    params = np.asarray(d["params"], np.float32)
+    meta = d["meta"]
+    meta.setdefault("train_band", [d["theta_lo"], d["theta_hi"]])
+    t0 = time.time()
+    budget, table = calibrate(params, theta_lo, theta_hi, p99_target)
+    for r in table:
+        print("budget %.2e  p50 %.2e  p99 %.2e  flops %.3e" %
+              (r["budget"], r["p50"], r["p99"], r["flops"]), flush=True)
+    if budget is None:
+        raise SystemExit("no budget reaches p99 <= %g" % p99_target)
+    meta.update(calibration=table, p99_target=p99_target)
+    save(path, params, budget, theta_lo, theta_hi, meta)
+    print("band [%g, %g]: budget %.3e -> %s (%.1f s)" %
+          (theta_lo, theta_hi, budget, path, time.time() - t0), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", default=DEFAULT_PATH)
@@ -145,8 +171,13 @@ def main():
     ap.add_argument("--theta-lo", type=float, default=0.25)
     ap.add_argument("--theta-hi", type=float, default=1.0)
     ap.add_argument("--p99-target", type=float, default=7e-4)
+    ap.add_argument("--recalibrate", action="store_true",
+                    help="keep the parameters of --out; calibrate the budget for the band given")
     args = ap.parse_args()
     t0 = time.time()
+    if args.recalibrate:
+        recalibrate(args.out, args.theta_lo, args.theta_hi, args.p99_target)
+        return
     xs, ys, infos = [], [], []
     for k, (name, n, theta, eps, ngs) in enumerate(TRAIN_CONFIGS):
         x, y, info = collect(n, theta, eps, ngs, SEED + k, args.theta_lo, args.theta_hi)
@@ -167,7 +198,7 @@ def main():
               (r["budget"], r["p50"], r["p99"], r["flops"]), flush=True)
     if budget is None:
         raise SystemExit("no budget reaches p99 <= %g" % args.p99_target)
-    meta = dict(configs=infos, steps=args.steps, batch=args.batch, lr=args.lr,
+    meta = dict(configs=infos, train_band=[args.theta_lo, args.theta_hi], steps=args.steps, batch=args.batch, lr=args.lr,
                 loss_first=float(curve[0]), loss_last=float(np.mean(curve[-50:])),
                 target_var=float(np.var(y)), calibration=table, p99_target=args.p99_target,
                 seconds=time.time() - t0)

@@ -158,6 +158,14 @@ int bt_set_walk_options(bt_ctx *ctx, int kahan, int stack_ints, int blocks_per_c
  * the immutable cell / particle arrays and used from SGPRs.  Both evaluate the
  * same interaction lists; forces agree to FP32 rounding. */
 int bt_set_walk_variant(bt_ctx *ctx, int variant);
+/* Learned-angle walk only (no reference counterpart): entries of each wave's
+ * traversal stack kept in LDS, 512..1024 (default 1024); deeper entries spill
+ * to the global stack of bt_set_walk_options and come back in LIFO order, so
+ * the traversal and the forces do not depend on it. */
+int bt_set_walk_lds_stack(bt_ctx *ctx, int entries);
+/* Deepest traversal stack of any group in the last walk (entries, LDS and
+ * global parts together): the stack_ints a walk of this tree needs. */
+int bt_get_walk_stack_max(bt_ctx *ctx, int *entries);
 /* code/Treecode.h:14-16 (#define QUADRUPOLE): 1 (default) = monopole +
  * quadrupole P2C (code/computeForces.cu:83-128, 64 flops); 0 = monopole-only
  * P2C, the cell evaluated as a point mass at its centre of mass

@@ -188,3 +188,31 @@ def test_trained_network_gains_over_fixed_theta(bt):
     as_accurate = [fl for th, f99, fl in fixed if f99 <= p99_l]
     assert as_accurate, "no fixed angle reaches the learned p99"
     assert flops_l <= 0.9 * min(as_accurate)
+
+
+def test_lds_stack_spill_leaves_the_walk_unchanged(bt):
+    """The learned walk keeps the top of each wave's stack in LDS and spills
+    deeper entries to global memory.  On the N = 2^24 halo + disk system many
+    groups go deeper than 512 entries: with 512 LDS entries the interaction
+    counts and the forces are bitwise those with 1024."""
+    from bonsai2_amd import learn
+    n = 1 << 24
+    pos, vel = bt.halo_disk(n, seed=19810614)
+    t = learn.built_tree(pos, vel, 0.4, 0.002)
+    try:
+        t.use_learned_theta(check_samples=0)
+        with pytest.raises(RuntimeError):
+            t.set_walk_lds_stack(511)
+        with pytest.raises(RuntimeError):
+            t.set_walk_lds_stack(1025)
+        out = {}
+        for entries in (1024, 512):
+            t.set_walk_lds_stack(entries)
+            t.computeForces()
+            out[entries] = (t.acc(bt.ORDER_PH), t.group_counts(), t.walk_stack_max())
+    finally:
+        t.close()
+    print("deepest stack: %d entries" % out[1024][2])
+    assert out[1024][2] > 512 and out[512][2] == out[1024][2]
+    assert np.array_equal(out[512][1], out[1024][1])
+    assert np.array_equal(out[512][0], out[1024][0])
