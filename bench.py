@@ -38,8 +38,8 @@ FP32_PEAK_TFLOPS = 157.3
 # 1.0x): profiles/hbm_traffic_halo_disk_2p24_pmc.md.  Counters cannot be read
 # from inside the bench run, so only the measured default configurations carry a
 # figure; every other configuration reports null.
-WALK_TRAFFIC_BYTES = {("halo_disk_2p24", "learned"): 2.124e10,
-                      ("halo_disk_2p24", "fixed"): 3.858e10}
+WALK_TRAFFIC_BYTES = {("halo_disk_2p24", "learned"): 4.082e9,
+                      ("halo_disk_2p24", "fixed"): 1.124e10}
 
 WORKLOADS = {
     "halo_disk_2p24": dict(n=1 << 24, ic="halo_disk", theta=0.4, eps=0.002),
@@ -165,7 +165,7 @@ def main():
     ap.add_argument("--level-split", type=int, default=5)
     ap.add_argument("--dt", type=float, default=1.0 / 128)
     ap.add_argument("--samples", type=int, default=4096)
-    ap.add_argument("--cpu-groups", type=int, default=1024)
+    ap.add_argument("--cpu-groups", type=int, default=32768)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kahan", action="store_true")
     ap.add_argument("--walk-variant", type=int, default=0, choices=(0, 1),

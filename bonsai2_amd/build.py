@@ -69,6 +69,15 @@ def build(verbose=False):
     if rc != 0:
         raise RuntimeError("driver link failed")
     os.replace(exe + ".tmp", exe)
+    # the issue-rate probe behind the walk's roofline (a standalone program)
+    probe = os.path.join(HERE, "probe_issue")
+    rc, cmd, out = _run([HIPCC] + [f for f in HIP_FLAGS if f != "-fPIC"] +
+                        ["-I" + CSRC, os.path.join(CSRC, "probe_issue.hip"), "-o", probe + ".tmp"])
+    if verbose or rc != 0:
+        sys.stderr.write(cmd + "\n" + out)
+    if rc != 0:
+        raise RuntimeError("probe build failed")
+    os.replace(probe + ".tmp", probe)
     return OUT
 
 

@@ -50,7 +50,23 @@ struct DevCounters {
   int queue;   // walk group queue head
   int stack_max;  // deepest walk stack of the last walk (entries)
   int xq[8];      // walk group queues, one per XCD range
-  int pad[7];
+  // per-step check of the learned angle, decided on the device
+  int check_calls;  // checks drawn so far (seeds the sample indices)
+  int fallback;     // 1: the last check failed and the fixed-angle walk ran
+  int checks, fallbacks;
+  int pad[3];
+};
+
+// The values captured graphs bake into their kernel arguments: a graph is
+// replayed only while all of them are unchanged (compared bytewise).
+struct GraphKey {
+  float dt, eps, theta;
+  int nleaf, level_split, ncrit;
+  int opening, kahan, quadrupole, walk_variant, stack_ints, blocks_per_cu, lds_stack;
+  float learn_shift, learn_theta_lo, learn_theta_hi;
+  int check_ns;
+  float check_p99;
+  unsigned long long check_seed;
 };
 
 struct Ctx {
@@ -138,12 +154,8 @@ struct Ctx {
   int check_ns = 0;              // FP64 samples re-checked after each learned walk
   float check_p99 = 1e-3f;       // above this p99 the step falls back to fixed theta
   unsigned long long check_seed = 1;
-  unsigned long long check_calls = 0;
   int *d_check_idx = nullptr;
   double *d_check = nullptr;     // direct results (4 * ns) + {p50, p99, max}
-  double last_check[3] = {0, 0, 0};
-  int last_fallback = 0;
-  long long fallbacks = 0, checks = 0;
 
   // energies / direct sum scratch
   double *d_energy = nullptr;  // 2 * nsys
@@ -163,8 +175,7 @@ struct Ctx {
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   float4 *graph_pos0 = nullptr;
-  float graph_dt = 0.f;
-  int graph_key[4] = {0, 0, 0, 0};   // nleaf, level_split, ncrit, opening
+  GraphKey graph_key{};   // every by-value launch parameter the graphs hold
   int tree_version = 0;
 
   // timing
@@ -192,7 +203,9 @@ void launch_ph_keys(Ctx &c);
 void launch_gather_ph(Ctx &c);
 void launch_groups(Ctx &c);
 
-void launch_walk(Ctx &c);
+// run_if: device flag; the walk (reset, traversal, statistics) does nothing
+// unless it is nonzero (nullptr = always run)
+void launch_walk(Ctx &c, const int *run_if = nullptr);
 
 void launch_mlp_prepare(Ctx &c);
 void launch_mlp_forward(Ctx &c, int n, const float *x, float *y);
@@ -204,6 +217,11 @@ int mlp_grad_blocks();
 void launch_mlp_grad(Ctx &c, const float *params, int n, const float *feat, const float *target,
                      int batch, unsigned long long seed, int step, float *partial, double *grad);
 void launch_mlp_adam(Ctx &c, float *p, float *m, float *v, const double *grad, float lr, int step);
+// sample indices of the next check, drawn on the device from check_seed and
+// the device's check count
+void launch_check_idx(Ctx &c, int ns);
+// p50/p99/max of the relative force errors into out[0..2]; sets the device's
+// fallback flag (p99 > check_p99) and counts checks and fallbacks
 void launch_learn_check(Ctx &c, int ns, const int *d_idx, const double *d_direct, double *out);
 
 void launch_kick(Ctx &c, float dt_half);

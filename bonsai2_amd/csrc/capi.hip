@@ -183,42 +183,33 @@ int phase_forces(Ctx &c) {
   return 0;
 }
 
-uint64_t mix64(uint64_t z) {
+static uint64_t mix64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
 }
 
-// After a learned-angle walk: relative force errors of check_ns seeded
-// samples against the FP64 direct sum; if their p99 exceeds check_p99 the
-// forces are recomputed with the fixed angle (the fallback).  Synchronises.
-int learned_check(Ctx &c) {
-  c.last_fallback = 0;
-  if (c.opening != 1 || !c.have_mlp || c.check_ns <= 0) return 0;
+// {p50, p99, max} of the last check, after the direct-sum buffer of check_ns
+// samples (every ns <= 1024 uses the same number of source slices).
+double *check_result(Ctx &c) {
+  return c.d_check + (size_t)c.check_ns * (direct_slices(c.check_ns) + 1) * 4;
+}
+
+// After a learned-angle walk: the relative force errors of check_ns seeded
+// samples against the FP64 direct sum, and the decision on the device: if
+// their p99 exceeds check_p99 the fixed-angle walk (the fallback) recomputes
+// the forces, otherwise its launches return at once.  No host
+// synchronisation, so a captured step holds the check too.
+void launch_learned_check(Ctx &c) {
+  if (c.opening != 1 || !c.have_mlp || c.check_ns <= 0) return;
   const int ns = std::min(c.check_ns, c.n);
-  std::vector<int> idx(ns);
-  for (int k = 0; k < ns; k++)
-    idx[k] = (int)(mix64(c.check_seed ^ (c.check_calls << 32) ^ (uint64_t)k) % (uint64_t)c.n);
-  c.check_calls++;
-  BT_CK(hipMemcpyAsync(c.d_check_idx, idx.data(), sizeof(int) * ns, hipMemcpyHostToDevice,
-                       c.stream));
-  double *res = c.d_check + (size_t)ns * (direct_slices(ns) + 1) * 4;
+  launch_check_idx(c, ns);
   launch_direct(c, ns, c.d_check_idx, c.d_check, true);
-  launch_learn_check(c, ns, c.d_check_idx, c.d_check, res);
-  BT_LAUNCH_CK();
-  BT_CK(hipMemcpyAsync(c.last_check, res, sizeof(double) * 3, hipMemcpyDeviceToHost, c.stream));
-  BT_CK(hipStreamSynchronize(c.stream));
-  c.checks++;
-  if (c.last_check[1] > (double)c.check_p99) {
-    c.opening = 0;
-    const int rc = phase_forces(c);
-    c.opening = 1;
-    if (rc) return rc;
-    c.last_fallback = 1;
-    c.fallbacks++;
-  }
-  return 0;
+  launch_learn_check(c, ns, c.d_check_idx, c.d_check, check_result(c));
+  c.opening = 0;
+  launch_walk(c, &c.dc->fallback);
+  c.opening = 1;
 }
 
 int check_build_errors(Ctx &c) {
@@ -529,9 +520,10 @@ int bt_compute_forces(bt_ctx *h, int intcount, double stats[4]) {
     return fail(c, BT_ERR_STATE, "computeMultipoles and makeGroups first");
   record(c, 7);
   if (phase_forces(c)) return BT_ERR_HIP;
+  launch_learned_check(c);
+  BT_LAUNCH_CK();
   if (read_counters(c)) return BT_ERR_HIP;
   if (c.hc.err & 4) return fail(c, BT_ERR_STACK, "walk stack overflow");
-  if (learned_check(c)) return BT_ERR_HIP;
   c.have_acc = true;
   if (intcount && stats) {
     unsigned long long w[4];
@@ -608,10 +600,7 @@ int bt_step(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit) {
   launch_drift(c, dt);
   if (phase_build(c) || phase_multipoles(c) || phase_groups(c) || phase_forces(c))
     return BT_ERR_HIP;
-  if (c.opening == 1 && c.check_ns > 0) {
-    if (read_counters(c)) return BT_ERR_HIP;
-    if (!(c.hc.err & 7) && learned_check(c)) return BT_ERR_HIP;
-  }
+  launch_learned_check(c);
   launch_kick(c, 0.5f * dt);
   record(c, 9);
   BT_LAUNCH_CK();
@@ -1105,20 +1094,29 @@ int bt_set_opening_check(bt_ctx *h, int nsamples, float p99_max, uint64_t seed) 
   c.check_ns = nsamples;
   c.check_p99 = p99_max;
   c.check_seed = seed;
-  c.check_calls = 0;
-  c.fallbacks = c.checks = 0;
+  // restart the device's check count (the sample draws) and statistics
+  const int zero[4] = {0, 0, 0, 0};
+  BT_CK(hipMemcpyAsync(&c.dc->check_calls, zero, sizeof(zero), hipMemcpyHostToDevice, c.stream));
+  if (nsamples > 0) {
+    const double z3[3] = {0.0, 0.0, 0.0};
+    BT_CK(hipMemcpyAsync(check_result(c), z3, sizeof(z3), hipMemcpyHostToDevice, c.stream));
+  }
+  BT_CK(hipStreamSynchronize(c.stream));
   return 0;
 }
 
 int bt_get_opening_check(bt_ctx *h, double out[6]) {
   if (!h || !out) return BT_ERR_ARG;
-  const Ctx &c = h->c;
-  out[0] = c.last_check[0];
-  out[1] = c.last_check[1];
-  out[2] = c.last_check[2];
-  out[3] = c.last_fallback;
-  out[4] = (double)c.fallbacks;
-  out[5] = (double)c.checks;
+  Ctx &c = h->c;
+  for (int k = 0; k < 6; k++) out[k] = 0.0;
+  if (c.check_ns > 0 && c.d_check) {
+    BT_CK(hipMemcpyAsync(out, check_result(c), sizeof(double) * 3, hipMemcpyDeviceToHost,
+                         c.stream));
+  }
+  if (read_counters(c)) return BT_ERR_HIP;
+  out[3] = c.hc.fallback;
+  out[4] = (double)c.hc.fallbacks;
+  out[5] = (double)c.hc.checks;
   return 0;
 }
 
@@ -1132,14 +1130,35 @@ int bt_step_graph(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit, in
   if (!valid_nleaf(nleaf)) return fail(c, BT_ERR_ARG, "nleaf must be 16, 24, 32, 48 or 64");
   if (ncrit < 1 || ncrit > 64) return fail(c, BT_ERR_ARG, "ncrit must be in [1, 64]");
   if (!c.have_acc) return fail(c, BT_ERR_STATE, "compute forces (or one bt_step) first");
-  const int key[4] = {nleaf, std::max(1, level_split), ncrit, c.opening};
-  if (c.graph_exec[0] && (c.graph_dt != dt || std::memcmp(key, c.graph_key, sizeof(key)) != 0))
-    release_graph(c);
+  GraphKey key;
+  std::memset(&key, 0, sizeof(key));   // padding too: keys are compared bytewise
+  key.dt = dt;
+  key.eps = c.eps;
+  key.theta = c.theta;
+  key.nleaf = nleaf;
+  key.level_split = std::max(1, level_split);
+  key.ncrit = ncrit;
+  key.opening = c.opening;
+  key.kahan = c.kahan;
+  key.quadrupole = c.quadrupole;
+  key.walk_variant = c.walk_variant;
+  key.stack_ints = c.stack_ints;
+  key.blocks_per_cu = c.blocks_per_cu;
+  key.lds_stack = c.lds_stack;
+  if (c.opening == 1) {
+    key.learn_shift = c.learn_shift;
+    key.learn_theta_lo = c.learn_theta_lo;
+    key.learn_theta_hi = c.learn_theta_hi;
+    key.check_ns = c.check_ns;
+    key.check_p99 = c.check_p99;
+    key.check_seed = c.check_seed;
+  }
+  if (c.graph_exec[0] && std::memcmp(&key, &c.graph_key, sizeof(key)) != 0) release_graph(c);
   if (!c.graph_exec[0]) {
     if (ensure_stack(c)) return BT_ERR_HIP;
     c.nleaf = nleaf;
     c.ncrit = ncrit;
-    c.level_split = key[1];
+    c.level_split = key.level_split;
     BT_CK(hipStreamSynchronize(c.stream));
     const bool timing = c.timing;
     c.timing = false;
@@ -1159,6 +1178,7 @@ int bt_step_graph(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit, in
       if (!rc) rc = phase_multipoles(c);
       if (!rc) rc = phase_groups(c);
       if (!rc) rc = phase_forces(c);
+      launch_learned_check(c);
       launch_kick(c, 0.5f * dt);
       if (hipStreamEndCapture(c.stream, &c.graph[g]) != hipSuccess) rc = BT_ERR_HIP;
       if (!rc && hipGraphInstantiate(&c.graph_exec[g], c.graph[g], nullptr, nullptr, 0) !=
@@ -1173,8 +1193,7 @@ int bt_step_graph(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit, in
       return fail(c, BT_ERR_HIP, "step capture failed");
     }
     c.graph_pos0 = pos0;
-    c.graph_dt = dt;
-    std::memcpy(c.graph_key, key, sizeof(key));
+    std::memcpy(&c.graph_key, &key, sizeof(key));
   }
   for (int k = 0; k < nsteps; k++) {
     BT_CK(hipGraphLaunch(c.graph_exec[c.pos == c.graph_pos0 ? 0 : 1], c.stream));

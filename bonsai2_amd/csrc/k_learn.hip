@@ -443,9 +443,21 @@ __global__ void k_mlp_adam(float *__restrict__ p, float *__restrict__ m, float *
 
 // Relative force errors of the walk against the FP64 direct sum on ns
 // samples, then (one block) a sort: out = {p50, p99, max}.
+// Sample indices of a check: idx[k] = mix64(seed ^ (calls << 32) ^ k) mod n,
+// calls = the checks drawn before; one block.
+__global__ __launch_bounds__(1024) void k_check_idx(int ns, int n, unsigned long long seed,
+                                                    DevCounters *dc, int *__restrict__ idx) {
+  const unsigned long long calls = (unsigned long long)(unsigned)dc->check_calls;
+  for (int k = threadIdx.x; k < ns; k += blockDim.x)
+    idx[k] = (int)(mix64(seed ^ (calls << 32) ^ (unsigned long long)k) % (unsigned long long)n);
+  __syncthreads();
+  if (threadIdx.x == 0) dc->check_calls = (int)(calls + 1);
+}
+
 __global__ __launch_bounds__(1024) void k_learn_check(int ns, const int *__restrict__ idx,
                                                       const double4 *__restrict__ d,
                                                       const float4 *__restrict__ acc,
+                                                      double p99_max, DevCounters *dc,
                                                       double *__restrict__ out) {
   __shared__ float e[1024];
   const int t = threadIdx.x;
@@ -471,9 +483,15 @@ __global__ __launch_bounds__(1024) void k_learn_check(int ns, const int *__restr
     }
   }
   if (t == 0 && ns > 0) {
+    const double p99 = e[(int)(0.99 * (ns - 1))];
     out[0] = e[(ns - 1) / 2];
-    out[1] = e[(int)(0.99 * (ns - 1))];
+    out[1] = p99;
     out[2] = e[ns - 1];
+    // above the threshold the fixed-angle walk recomputes the forces
+    const int fb = p99 > p99_max ? 1 : 0;
+    dc->fallback = fb;
+    dc->checks += 1;
+    dc->fallbacks += fb;
   }
 }
 
@@ -567,9 +585,15 @@ void launch_mlp_adam(Ctx &c, float *p, float *m, float *v, const double *grad, f
                      grad, lr, b1, b2, 1e-8f, bc1, bc2);
 }
 
+void launch_check_idx(Ctx &c, int ns) {
+  hipLaunchKernelGGL(k_check_idx, dim3(1), dim3(1024), 0, c.stream, ns, c.n,
+                     (unsigned long long)c.check_seed, c.dc, c.d_check_idx);
+}
+
 void launch_learn_check(Ctx &c, int ns, const int *d_idx, const double *d_direct, double *out) {
   hipLaunchKernelGGL(k_learn_check, dim3(1), dim3(1024), 0, c.stream, ns, d_idx,
-                     reinterpret_cast<const double4 *>(d_direct), c.acc, out);
+                     reinterpret_cast<const double4 *>(d_direct), c.acc, (double)c.check_p99,
+                     c.dc, out);
 }
 
 }  // namespace bt

@@ -25,6 +25,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kEnergyBlocks = 64;
 constexpr int kDirectSlices = 64;
+constexpr int kDirectTargets = kBlock;         // targets per block of the direct sum
 
 __global__ __launch_bounds__(kBlock) void k_kick(int n, const int *__restrict__ perm,
                                                  const float4 *__restrict__ acc,
@@ -116,7 +117,9 @@ __device__ __forceinline__ double rsqrt_fp64(double r2) {
 
 // grid (ceil(ns/256), slices): each block sums a slice of the source index
 // range for 256 targets, the slice staged in LDS as FP64.  FOREST: sources
-// outside a target's system are skipped.
+// outside a target's system are skipped.  The kernel is bound by FP64 VALU
+// issue (about 20 FP64 instructions per pair), not by the LDS broadcasts:
+// two targets per thread, halving the LDS reads, measured no faster.
 template <int NEWTON, bool FOREST>
 __global__ __launch_bounds__(kBlock) void k_direct(int ns, const int *__restrict__ idx, int n,
                                                    int nsys, const int *__restrict__ off,
@@ -156,17 +159,23 @@ __global__ __launch_bounds__(kBlock) void k_direct(int ns, const int *__restrict
   if (active) part[(size_t)blockIdx.y * ns + t] = make_double4(ax, ay, az, ph);
 }
 
+// One wave per target: lane l sums slices l, l + 64, ... in order, then the
+// wave adds its lanes in a fixed tree order (deterministic).
 __global__ __launch_bounds__(kBlock) void k_direct_reduce(int ns, int nslices,
                                                           const double4 *__restrict__ part,
                                                           double4 *__restrict__ out) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= ns) return;
+  const int t = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+  if (t >= ns) return;   // wave-uniform
   double4 s = make_double4(0, 0, 0, 0);
-  for (int q = 0; q < nslices; q++) {
+  for (int q = lane_id(); q < nslices; q += kWave) {
     const double4 v = part[(size_t)q * ns + t];
     s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
   }
-  out[t] = s;
+  s.x = wave_sum(s.x);
+  s.y = wave_sum(s.y);
+  s.z = wave_sum(s.z);
+  s.w = wave_sum(s.w);
+  if (lane_id() == 0) out[t] = s;
 }
 
 }  // namespace
@@ -193,8 +202,8 @@ void launch_energies(Ctx &c) {
 // Slices of the source range: enough blocks to fill the chip for any number
 // of targets (256 targets per block).
 int direct_slices(int ns) {
-  const int nb = (ns + kBlock - 1) / kBlock;
-  return std::min(1024, std::max(kDirectSlices, (4096 + nb - 1) / nb));
+  const int nb = (ns + kDirectTargets - 1) / kDirectTargets;
+  return std::min(2048, std::max(kDirectSlices, (8192 + nb - 1) / nb));
 }
 
 // d_out must hold ns * (direct_slices(ns) + 1) double4: the result, then the partials.
@@ -204,13 +213,15 @@ void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out, bool check) 
   const int slices = direct_slices(ns);
   // the per-step check of a learned walk needs relative errors of ~1e-6:
   // one Newton step (~3e-14) is ample; the measurement path keeps two
+  const dim3 grid((ns + kDirectTargets - 1) / kDirectTargets, slices);
   void (*kern)(int, const int *, int, int, const int *, const float4 *, double, double4 *);
   if (c.nsys > 1) kern = check ? k_direct<1, true> : k_direct<2, true>;
   else kern = check ? k_direct<1, false> : k_direct<2, false>;
-  hipLaunchKernelGGL(kern, dim3((ns + kBlock - 1) / kBlock, slices), dim3(kBlock), 0,
-                     c.stream, ns, d_idx, c.n, c.nsys, c.d_off, c.pos_ph, eps2, part);
-  hipLaunchKernelGGL(k_direct_reduce, dim3((ns + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream,
-                     ns, slices, part, reinterpret_cast<double4 *>(d_out));
+  hipLaunchKernelGGL(kern, grid, dim3(kBlock), 0, c.stream, ns, d_idx, c.n, c.nsys, c.d_off,
+                     c.pos_ph, eps2, part);
+  const int tpb = kBlock / kWave;   // targets per reduce block
+  hipLaunchKernelGGL(k_direct_reduce, dim3((ns + tpb - 1) / tpb), dim3(kBlock), 0, c.stream, ns,
+                     slices, part, reinterpret_cast<double4 *>(d_out));
 }
 
 }  // namespace bt

@@ -28,6 +28,7 @@
 // pre-scaled by 3 and the trace terms precomputed (35 VALU instructions).
 #include "bt_device.h"
 #include "bt_internal.h"
+#include "bt_interact.h"
 #include "bt_learn.h"
 
 namespace bt {
@@ -106,6 +107,7 @@ struct WalkArgs {
   int *stack;
   int stack_ints;
   int lstk_cap;          // LDS part of a learned walk's stack, entries per wave
+  const int *run_if;     // nullptr, or a device flag: walk only if it is nonzero
   DevCounters *dc;
 };
 
@@ -146,10 +148,6 @@ __device__ __forceinline__ float box_ds2(const float4 cs, float gcx, float gcy, 
   return dx * dx + dy * dy + dz * dz;
 }
 
-struct Acc4 {
-  float x, y, z, w;
-};
-
 template <bool KAHAN>
 __device__ __forceinline__ void fold(Acc4 &s, Acc4 &c, const Acc4 &b) {
   if (KAHAN) {
@@ -161,49 +159,6 @@ __device__ __forceinline__ void fold(Acc4 &s, Acc4 &c, const Acc4 &b) {
   } else {
     s.x += b.x; s.y += b.y; s.z += b.z; s.w += b.w;
   }
-}
-
-// Monopole + quadrupole of one cell (code/computeForces.cu:83-128):
-//   phi -= M/r - (trQ/2) M/r^3 + 1.5 (dr.Q.dr) M/r^5
-//   a   += (-M/r^3 + 1.5 trQ M/r^5 - 7.5 (dr.Q.dr) M/r^7) dr + 3 M/r^5 Q.dr
-// evaluated from the cell's walk record, whose quadrupole is pre-scaled by 3
-// (B = 3Qxx, 3Qyy, 3Qzz, 3Qxy; C = 3Qxz, 3Qyz, trQ/2, 1.5 trQ): with
-// dr = x_i - com and u = dr.3Q.dr / r^2 the terms become
-//   phi -= M/r - (trQ/2) M/r^3 + (u/2) M/r^3
-//   a   += (M/r^5 (1.5 trQ - 2.5 u) - M/r^3) dr + M/r^5 (3Q).dr
-__device__ __forceinline__ void p2c_rec(Acc4 &b, float xi, float yi, float zi, const float4 A,
-                                        const float4 B, const float4 C, float eps2) {
-  const float dx = xi - A.x, dy = yi - A.y, dz = zi - A.z;
-  const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-  const float rinv = __builtin_amdgcn_rsqf(r2);
-  const float rinv2 = rinv * rinv;
-  const float m1 = A.w * rinv;
-  const float m3 = rinv2 * m1;
-  const float m5 = rinv2 * m3;
-  const float qx = fmaf(B.x, dx, fmaf(B.w, dy, C.x * dz));
-  const float qy = fmaf(B.w, dx, fmaf(B.y, dy, C.y * dz));
-  const float qz = fmaf(C.x, dx, fmaf(C.y, dy, B.z * dz));
-  const float u = fmaf(qx, dx, fmaf(qy, dy, qz * dz)) * rinv2;
-  b.w -= m1;
-  b.w = fmaf(-m3, fmaf(0.5f, u, -C.z), b.w);
-  const float cc = fmaf(m5, fmaf(-2.5f, u, C.w), -m3);
-  b.x = fmaf(cc, dx, fmaf(m5, qx, b.x));
-  b.y = fmaf(cc, dy, fmaf(m5, qy, b.y));
-  b.z = fmaf(cc, dz, fmaf(m5, qz, b.z));
-}
-
-// Particle-particle, code/computeForces.cu:58-78.
-__device__ __forceinline__ void p2p(Acc4 &b, float xi, float yi, float zi, const float4 A,
-                                    float eps2) {
-  const float dx = A.x - xi, dy = A.y - yi, dz = A.z - zi;
-  const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-  const float rinv = __builtin_amdgcn_rsqf(r2);
-  const float mr1 = A.w * rinv;
-  const float mr3 = mr1 * rinv * rinv;
-  b.w -= mr1;
-  b.x = fmaf(mr3, dx, b.x);
-  b.y = fmaf(mr3, dy, b.y);
-  b.z = fmaf(mr3, dz, b.z);
 }
 
 template <bool KAHAN>
@@ -379,6 +334,7 @@ void k_walk(WalkArgs a) {
     mb3 = a.mlp->b3;
     __syncthreads();
   }
+  if (a.run_if && *a.run_if == 0) return;   // a fallback walk that is not needed
   const int ngroups = a.dc->ngroups;
   // a failed build (cell or group capacity) leaves placeholder links: do not walk it
   if (a.dc->err & 3) return;
@@ -635,7 +591,9 @@ __global__ __launch_bounds__(1024) void k_walk_stats(const int2 *__restrict__ gr
                                                      const int2 *__restrict__ gcounts,
                                                      const double *__restrict__ gpot,
                                                      const DevCounters *dc,
-                                                     unsigned long long *out, double *pot) {
+                                                     unsigned long long *out, double *pot,
+                                                     const int *run_if) {
+  if (run_if && *run_if == 0) return;
   __shared__ unsigned long long s_ds[16], s_as[16];
   __shared__ unsigned int s_dm[16], s_am[16];
   __shared__ double s_p[16];
@@ -669,8 +627,8 @@ __global__ __launch_bounds__(1024) void k_walk_stats(const int2 *__restrict__ gr
   }
 }
 
-__global__ void k_walk_reset(DevCounters *dc) {
-  if (threadIdx.x == 0) {
+__global__ void k_walk_reset(DevCounters *dc, const int *run_if) {
+  if (threadIdx.x == 0 && !(run_if && *run_if == 0)) {
     dc->queue = 0;
     dc->stack_max = 0;
     for (int x = 0; x < kXcd; x++) dc->xq[x] = 0;
@@ -680,7 +638,7 @@ __global__ void k_walk_reset(DevCounters *dc) {
 
 }  // namespace
 
-void launch_walk(Ctx &c) {
+void launch_walk(Ctx &c, const int *run_if) {
   WalkArgs a;
   a.n = c.n;
   a.nsys = c.nsys;
@@ -708,8 +666,9 @@ void launch_walk(Ctx &c) {
   a.stack = c.stack;
   a.stack_ints = c.stack_ints;
   a.lstk_cap = c.lds_stack;
+  a.run_if = run_if;
   a.dc = c.dc;
-  hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, c.stream, c.dc);
+  hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, c.stream, c.dc, run_if);
   void (*kern)(WalkArgs);
   if (!c.quadrupole)
     kern = c.kahan ? k_walk<true, false, false, true> : k_walk<false, false, false, true>;
@@ -723,7 +682,7 @@ void launch_walk(Ctx &c) {
     kern = c.kahan ? k_walk<true, false, false> : k_walk<false, false, false>;
   hipLaunchKernelGGL(kern, dim3(c.walk_blocks), dim3(kBlock), 0, c.stream, a);
   hipLaunchKernelGGL(k_walk_stats, dim3(1), dim3(1024), 0, c.stream, c.groups, c.gcounts, c.gpot,
-                     c.dc, c.d_wstats, c.d_wpot);
+                     c.dc, c.d_wstats, c.d_wpot, run_if);
 }
 
 }  // namespace bt

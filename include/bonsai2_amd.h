@@ -176,11 +176,13 @@ int bt_set_quadrupole(bt_ctx *ctx, int on);
  * bt_step_graph: the step of bt_step (kick, drift, buildTree,
  * computeMultipoles, makeGroups, computeForces, kick) captured once into a
  * HIP graph and replayed nsteps times with no host work between kernels.  The
- * graph is re-captured when dt, nleaf, level_split, ncrit or the opening mode
- * change, and dropped when arrays are reallocated.  Needs forces (or one
- * bt_step) first.  The learned angle's per-step FP64 check needs the host and
- * is not part of the graph.  A capacity overflow returns BT_ERR_CAPACITY:
- * bt_step then regrows the arrays. */
+ * graph is re-captured when any value it holds changes (dt, nleaf,
+ * level_split, ncrit, eps, theta, walk options, the opening mode, budget and
+ * band, the check's settings), and dropped when arrays are reallocated.
+ * Needs forces (or one bt_step) first.  With the learned angle the per-step
+ * FP64 check and its fixed-angle fallback are part of the graph (the device
+ * decides).  A capacity overflow returns BT_ERR_CAPACITY: bt_step then
+ * regrows the arrays. */
 int bt_step_graph(bt_ctx *ctx, float dt, int nleaf, int level_split, int ncrit, int nsteps);
 /* Picks NLEAF, NCRIT (3 x 3 candidates, timed zero-dt steps), the persistent
  * walk grid (4, 8, 12, 16 blocks per CU) and the traversal stack depth
@@ -230,7 +232,10 @@ int bt_learn_collect(bt_ctx *ctx, int nsel, const int *gsel, float theta_lo, flo
 int bt_set_opening(bt_ctx *ctx, int mode, float budget, float theta_lo, float theta_hi);
 /* After each learned-angle walk, nsamples (<= 1024; 0 = off) seeded particles
  * are re-checked against the FP64 direct sum; if their p99 relative force
- * error exceeds p99_max the forces are recomputed with the fixed angle. */
+ * error exceeds p99_max the forces are recomputed with the fixed angle.  The
+ * decision is taken on the device (no host synchronisation); the samples of
+ * check k are mix64(seed ^ (k << 32) ^ j) mod n, j < nsamples, k counted
+ * from this call. */
 int bt_set_opening_check(bt_ctx *ctx, int nsamples, float p99_max, uint64_t seed);
 /* {p50, p99, max of the last check, last call fell back (0/1), fallbacks,
  * checks} */

@@ -216,3 +216,58 @@ def test_lds_stack_spill_leaves_the_walk_unchanged(bt):
     assert out[1024][2] > 512 and out[512][2] == out[1024][2]
     assert np.array_equal(out[512][1], out[1024][1])
     assert np.array_equal(out[512][0], out[1024][0])
+
+
+@pytest.mark.parametrize("p99_max", [1.0, 1e-12], ids=["learned", "fallback"])
+def test_graph_step_with_check_matches_eager(bt, p99_max):
+    """The learned-angle step with its per-step FP64 check and the fixed-angle
+    fallback, decided on the device, captured in the step graph: replays equal
+    eager steps bitwise, checks and fallbacks included; a changed budget
+    recaptures the graph."""
+    from bonsai2_amd import learn
+    params, budget, lo, hi = learn.load()
+    n = 1 << 16
+    pos, vel = bt.plummer(n, seed=14)
+    ts = []
+    for _ in range(2):
+        t = bt.Treecode(0.01, 0.4, 64)
+        t.alloc(n)
+        t.ptcl_h2d(pos, vel)
+        t.buildTree(16)
+        t.computeMultipoles()
+        t.makeGroups(5, 64)
+        t.set_mlp_params(params)
+        t.set_opening(1, budget, lo, hi)
+        t.set_opening_check(256, p99_max, 9)
+        t.computeForces()
+        ts.append(t)
+    eager, graph = ts
+
+    def same(tag):
+        pe, ve = eager.ptcl_d2h(bt.ORDER_INPUT)
+        pg, vg = graph.ptcl_d2h(bt.ORDER_INPUT)
+        assert np.array_equal(pe, pg) and np.array_equal(ve, vg), tag
+        assert np.array_equal(eager.acc(bt.ORDER_INPUT), graph.acc(bt.ORDER_INPUT)), tag
+        ce, cg = eager.opening_check(), graph.opening_check()
+        assert ce == cg, (tag, ce, cg)
+        return ce
+
+    try:
+        for step in range(3):
+            eager.step(1.0 / 128, 16, 5, 64)
+            graph.step_graph(1.0 / 128, 16, 5, 64, nsteps=1)
+            chk = same(step)
+        assert chk["checks"] == 4
+        assert chk["fallbacks"] == (4 if p99_max < 1 else 0)
+        for _ in range(2):
+            eager.step(1.0 / 128, 16, 5, 64)
+        graph.step_graph(1.0 / 128, 16, 5, 64, nsteps=2)
+        assert same("replays")["checks"] == 6
+        for t in ts:
+            t.set_opening(1, 2.0 * budget, lo, hi)
+        eager.step(1.0 / 128, 16, 5, 64)
+        graph.step_graph(1.0 / 128, 16, 5, 64, nsteps=1)
+        assert same("new budget")["checks"] == 7
+    finally:
+        eager.close()
+        graph.close()
