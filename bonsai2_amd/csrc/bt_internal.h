@@ -34,6 +34,10 @@ struct LearnBuffers {
 };
 
 // Device-resident counters of one context (one allocation).
+// Blocks of the walk's statistics reduction; d_wstats holds the 4 results,
+// then 4 words of partials per block.
+constexpr int kWalkStatBlocks = 256;
+
 // Learned walk: stack entries per wave kept in LDS (bt_set_walk_lds_stack).
 constexpr int kLdsStack = 1024;
 #ifndef BT_LDS_STACK_MIN

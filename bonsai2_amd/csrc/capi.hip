@@ -278,7 +278,7 @@ int bt_create(bt_ctx **out, float eps, float theta, int ncrit) {
   c.own_stream = true;
   for (int k = 0; k < 12; k++) (void)hipEventCreate(&c.ev[k]);
   if (dev_alloc(c, &c.dc, 1) || dev_alloc(c, &c.level_beg, kLevels) ||
-      dev_alloc(c, &c.d_wstats, 4) || dev_alloc(c, &c.d_wpot, 1) ||
+      dev_alloc(c, &c.d_wstats, 4 + 4 * (size_t)kWalkStatBlocks) || dev_alloc(c, &c.d_wpot, 1) ||
       dev_alloc(c, &c.lvl_blk, kLevelBlocks) || dev_alloc(c, &c.mlp_params, kMlpParams) ||
       dev_alloc(c, &c.mlp_m, kMlpParams) || dev_alloc(c, &c.mlp_v, kMlpParams) ||
       dev_alloc(c, &c.mlp_dev, 1)) {

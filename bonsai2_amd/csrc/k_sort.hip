@@ -34,13 +34,16 @@ constexpr int kScanTile = kScanThreads * kScanItems;
 
 struct DigitSrc {
   int shift;        // bit offset of the digit
-  int by_system;    // digit taken from system_of(value) instead of the key
   int nsys;
   const int *off;
 };
 
+// BYSYS: the digit is taken from system_of(value) instead of the key (a
+// compile-time choice, so the key passes carry no branch and their loads can
+// all be issued before the first is used).
+template <bool BYSYS>
 __device__ __forceinline__ int digit_of(uint64_t key, int val, const DigitSrc &s) {
-  if (s.by_system) return (system_of(val, s.nsys, s.off) >> s.shift) & 0xFF;
+  if (BYSYS) return (system_of(val, s.nsys, s.off) >> s.shift) & 0xFF;
   return (int)((key >> s.shift) & 0xFFull);
 }
 
@@ -75,23 +78,32 @@ __device__ __forceinline__ int block_incl_scan(int v, int *sh_wave, int *total) 
   return Op::op(pre, v);
 }
 
+template <bool BYSYS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__restrict__ keys,
                                                              const int *__restrict__ vals,
                                                              int n, DigitSrc src, int nblocks,
                                                              int *__restrict__ ghist) {
   __shared__ int hist[256];
   hist[threadIdx.x] = 0;
-  __syncthreads();
   const int base = blockIdx.x * kSortTile;
-#pragma unroll 4
+  // all of the tile's loads first, then the LDS counting
+  int dg[kSortRounds];
+#pragma unroll
   for (int r = 0; r < kSortRounds; r++) {
     const int i = base + r * kSortThreads + threadIdx.x;
-    if (i < n) atomicAdd(&hist[digit_of(keys[i], src.by_system ? vals[i] : 0, src)], 1);
+    const int ic = min(i, n - 1);   // unconditional loads: no branch between them
+    const int d = digit_of<BYSYS>(BYSYS ? 0ull : keys[ic], BYSYS ? vals[ic] : 0, src);
+    dg[r] = i < n ? d : -1;
   }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortRounds; r++)
+    if (dg[r] >= 0) atomicAdd(&hist[dg[r]], 1);
   __syncthreads();
   ghist[threadIdx.x * nblocks + blockIdx.x] = hist[threadIdx.x];
 }
 
+template <bool BYSYS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint64_t *__restrict__ keys, const int *__restrict__ vals, int n, DigitSrc src,
     int nblocks, const int *__restrict__ goff, uint64_t *__restrict__ keys_out,
@@ -125,17 +137,20 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
   const int wbase = blockIdx.x * kSortTile + w * (kSortRounds * kWave);
   uint64_t kk[kSortRounds];
   int vv[kSortRounds], dd[kSortRounds], rr[kSortRounds];
+  // the wave's 16 rounds of pairs are all loaded before the ranking starts
+#pragma unroll
+  for (int r = 0; r < kSortRounds; r++) {
+    const int i = wbase + r * kWave + lane;
+    kk[r] = i < n ? keys[i] : 0ull;
+    vv[r] = i < n ? vals[i] : 0;
+  }
 #pragma unroll
   for (int r = 0; r < kSortRounds; r++) {
     const int i = wbase + r * kWave + lane;
     const bool valid = i < n;
-    uint64_t key = 0;
-    int val = 0, d = 0;
-    if (valid) {
-      key = keys[i];
-      val = vals[i];
-      d = digit_of(key, val, src);
-    }
+    const uint64_t key = kk[r];
+    const int val = vv[r];
+    const int d = valid ? digit_of<BYSYS>(key, val, src) : 0;
     uint64_t peers = ballot64(valid);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
@@ -148,8 +163,6 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     wave_lds_sync();
     if (valid && rank == 0) wcnt[w][d] = before + __popcll(peers);
     wave_lds_sync();
-    kk[r] = key;
-    vv[r] = val;
     dd[r] = valid ? d : -1;
     rr[r] = before + rank;
   }
@@ -179,7 +192,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
   for (int j = tid; j < tile_n; j += kSortThreads) {
     const uint64_t key = skeys[j];
     const int val = svals[j];
-    const int d = digit_of(key, val, src);
+    const int d = digit_of<BYSYS>(key, val, src);
     const int dst = gbase[d] + (j - tstart[d]);
     keys_out[dst] = key;
     vals_out[dst] = val;
@@ -356,12 +369,14 @@ void radix_sort_pairs(Ctx &c, uint64_t *keys, int *vals, uint64_t *keys_tmp, int
   uint64_t *ks = keys, *kd = keys_tmp;
   int *vs = vals, *vd = vals_tmp;
   for (const Pass &p : passes) {
-    DigitSrc src{p.shift, p.by_sys, c.nsys, c.d_off};
-    hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kSortThreads), 0, c.stream, ks, vs, n,
-                       src, nblocks, c.sort_hist);
+    DigitSrc src{p.shift, c.nsys, c.d_off};
+    auto hist = p.by_sys ? k_radix_hist<true> : k_radix_hist<false>;
+    auto scatter = p.by_sys ? k_radix_scatter<true> : k_radix_scatter<false>;
+    hipLaunchKernelGGL(hist, dim3(nblocks), dim3(kSortThreads), 0, c.stream, ks, vs, n, src,
+                       nblocks, c.sort_hist);
     scan_exclusive_sum(c, c.sort_hist, c.sort_hist, hist_n, nullptr);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortThreads), 0, c.stream, ks, vs,
-                       n, src, nblocks, c.sort_hist, kd, vd);
+    hipLaunchKernelGGL(scatter, dim3(nblocks), dim3(kSortThreads), 0, c.stream, ks, vs, n, src,
+                       nblocks, c.sort_hist, kd, vd);
     std::swap(ks, kd);
     std::swap(vs, vd);
   }

@@ -586,44 +586,75 @@ void k_walk(WalkArgs a) {
 
 // Interaction statistics (code/computeForces.cu:567-609, :701-715): per valid
 // i-particle its group's counts; {direct_sum, direct_max, approx_sum,
-// approx_max} and 1/2 sum m phi.  One block, deterministic order.
-__global__ __launch_bounds__(1024) void k_walk_stats(const int2 *__restrict__ groups,
-                                                     const int2 *__restrict__ gcounts,
-                                                     const double *__restrict__ gpot,
-                                                     const DevCounters *dc,
-                                                     unsigned long long *out, double *pot,
-                                                     const int *run_if) {
-  if (run_if && *run_if == 0) return;
-  __shared__ unsigned long long s_ds[16], s_as[16];
-  __shared__ unsigned int s_dm[16], s_am[16];
-  __shared__ double s_p[16];
-  const int ng = dc->ngroups;
-  unsigned long long ds = 0, as = 0;
-  unsigned int dm = 0, am = 0;
-  double p = 0.0;
-  for (int g = threadIdx.x; g < ng; g += blockDim.x) {
-    const int2 c = gcounts[g];
-    const int np = groups[g].y;
-    ds += (unsigned long long)c.y * (unsigned long long)np;
-    as += (unsigned long long)c.x * (unsigned long long)np;
-    dm = max(dm, (unsigned int)c.y);
-    am = max(am, (unsigned int)c.x);
-    p += gpot[g];
-  }
-  ds = wave_sum(ds); as = wave_sum(as); p = wave_sum(p);
+// approx_max} and 1/2 sum m phi.  kWalkStatBlocks blocks reduce contiguous
+// group ranges, one block combines their partials: a fixed order, so the
+// FP64 potential sum is the same on every run.
+struct WStat {
+  unsigned long long ds, as;
+  unsigned int dm, am;
+  double p;
+};
+
+// 256 threads; the block's total in thread 0
+__device__ __forceinline__ WStat block_stats(WStat v, WStat *sh) {
+  v.ds = wave_sum(v.ds);
+  v.as = wave_sum(v.as);
+  v.p = wave_sum(v.p);
   for (int m = 32; m >= 1; m >>= 1) {
-    dm = max(dm, (unsigned int)__shfl_xor((int)dm, m, kWave));
-    am = max(am, (unsigned int)__shfl_xor((int)am, m, kWave));
+    v.dm = max(v.dm, (unsigned int)__shfl_xor((int)v.dm, m, kWave));
+    v.am = max(v.am, (unsigned int)__shfl_xor((int)v.am, m, kWave));
   }
   const int w = threadIdx.x >> 6;
-  if (lane_id() == 0) { s_ds[w] = ds; s_as[w] = as; s_dm[w] = dm; s_am[w] = am; s_p[w] = p; }
+  if (lane_id() == 0) sh[w] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
-      ds += s_ds[k]; as += s_as[k]; dm = max(dm, s_dm[k]); am = max(am, s_am[k]); p += s_p[k];
+    for (int k = 1; k < kWavesPerBlock; k++) {
+      v.ds += sh[k].ds; v.as += sh[k].as; v.p += sh[k].p;
+      v.dm = max(v.dm, sh[k].dm); v.am = max(v.am, sh[k].am);
     }
-    out[0] = ds; out[1] = dm; out[2] = as; out[3] = am;
-    *pot = 0.5 * p;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_walk_stats_part(const int2 *__restrict__ groups,
+                                                           const int2 *__restrict__ gcounts,
+                                                           const double *__restrict__ gpot,
+                                                           const DevCounters *dc, WStat *part,
+                                                           const int *run_if) {
+  if (run_if && *run_if == 0) return;
+  __shared__ WStat sh[kWavesPerBlock];
+  const int ng = dc->ngroups;
+  const int chunk = (ng + gridDim.x - 1) / gridDim.x;
+  const int g0 = blockIdx.x * chunk, g1 = min(ng, g0 + chunk);
+  WStat v = {0ull, 0ull, 0u, 0u, 0.0};
+  for (int g = g0 + threadIdx.x; g < g1; g += kBlock) {
+    const int2 c = gcounts[g];
+    const int np = groups[g].y;
+    v.ds += (unsigned long long)c.y * (unsigned long long)np;
+    v.as += (unsigned long long)c.x * (unsigned long long)np;
+    v.dm = max(v.dm, (unsigned int)c.y);
+    v.am = max(v.am, (unsigned int)c.x);
+    v.p += gpot[g];
+  }
+  v = block_stats(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_walk_stats_final(const WStat *__restrict__ part,
+                                                            unsigned long long *out, double *pot,
+                                                            const int *run_if) {
+  if (run_if && *run_if == 0) return;
+  __shared__ WStat sh[kWavesPerBlock];
+  WStat v = {0ull, 0ull, 0u, 0u, 0.0};
+  for (int k = threadIdx.x; k < kWalkStatBlocks; k += kBlock) {
+    const WStat q = part[k];
+    v.ds += q.ds; v.as += q.as; v.p += q.p;
+    v.dm = max(v.dm, q.dm); v.am = max(v.am, q.am);
+  }
+  v = block_stats(v, sh);
+  if (threadIdx.x == 0) {
+    out[0] = v.ds; out[1] = v.dm; out[2] = v.as; out[3] = v.am;
+    *pot = 0.5 * v.p;
   }
 }
 
@@ -681,8 +712,11 @@ void launch_walk(Ctx &c, const int *run_if) {
   else
     kern = c.kahan ? k_walk<true, false, false> : k_walk<false, false, false>;
   hipLaunchKernelGGL(kern, dim3(c.walk_blocks), dim3(kBlock), 0, c.stream, a);
-  hipLaunchKernelGGL(k_walk_stats, dim3(1), dim3(1024), 0, c.stream, c.groups, c.gcounts, c.gpot,
-                     c.dc, c.d_wstats, c.d_wpot, run_if);
+  WStat *part = reinterpret_cast<WStat *>(c.d_wstats + 4);
+  hipLaunchKernelGGL(k_walk_stats_part, dim3(kWalkStatBlocks), dim3(kBlock), 0, c.stream, c.groups,
+                     c.gcounts, c.gpot, c.dc, part, run_if);
+  hipLaunchKernelGGL(k_walk_stats_final, dim3(1), dim3(kBlock), 0, c.stream, part, c.d_wstats,
+                     c.d_wpot, run_if);
 }
 
 }  // namespace bt
