@@ -12,7 +12,7 @@ namespace bt {
 
 constexpr int kLevels = 32;   // level table slots (reference int2[32], code/Treecode.h:171)
 #ifndef BT_LEVEL_BLOCKS
-#define BT_LEVEL_BLOCKS 512
+#define BT_LEVEL_BLOCKS 2048
 #endif
 constexpr int kLevelBlocks = BT_LEVEL_BLOCKS;  // fixed grid of the per-level tree kernels
 constexpr int kFp64Levels = 4;     // cells at levels <= this combine moments in FP64

@@ -236,16 +236,28 @@ __global__ __launch_bounds__(kBlock) void k_level_count(LevelArgs a) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_level_scan(LevelArgs a) {
-  __shared__ int sh[kLevelBlocks];
-  for (int k = threadIdx.x; k < kLevelBlocks; k += kBlock) sh[k] = a.blk[k];
+  constexpr int kItems = (kLevelBlocks + kBlock - 1) / kBlock;
+  __shared__ int shw[kBlock / kWave];
+  __shared__ int s_run;
+  // thread t scans block sums [t * kItems, (t + 1) * kItems)
+  int v[kItems];
+  int own = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    const int i = threadIdx.x * kItems + k;
+    v[k] = i < kLevelBlocks ? a.blk[i] : 0;
+    own += v[k];
+  }
+  int incl = wave_incl_scan(own);
+  if (lane_id() == kWave - 1) shw[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  int pre = 0;
+  for (int w = 0; w < (int)(threadIdx.x >> 6); w++) pre += shw[w];
+  if (threadIdx.x == kBlock - 1) s_run = pre + incl;
+  int off = pre + incl - own;
   __syncthreads();
   if (threadIdx.x == 0) {
-    int run = 0;
-    for (int k = 0; k < kLevelBlocks; k++) {
-      const int v = sh[k];
-      sh[k] = run;
-      run += v;
-    }
+    const int run = s_run;
     DevCounters *dc = a.dc;
     const int start = a.L == 1 ? 0 : dc->lvl_start[a.L - 1] + dc->lvl_count[a.L - 1];
     int total = run;
@@ -257,8 +269,12 @@ __global__ __launch_bounds__(kBlock) void k_level_scan(LevelArgs a) {
     dc->lvl_count[a.L] = total;
     if (total > 0) dc->nlevels = a.L;
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < kLevelBlocks; k += kBlock) a.blk[k] = sh[k];
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    const int i = threadIdx.x * kItems + k;
+    if (i < kLevelBlocks) a.blk[i] = off;
+    off += v[k];
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_level_emit(LevelArgs a) {
