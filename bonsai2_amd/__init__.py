@@ -61,6 +61,7 @@ _SIGS = {
     "bt_set_walk_variant": [P, I],
     "bt_set_walk_lds_stack": [P, I],
     "bt_get_walk_stack_max": [P, P],
+    "bt_set_group_mode": [P, I],
     "bt_set_quadrupole": [P, I],
     "bt_step_graph": [P, F, I, I, I, I],
     "bt_autotune": [P, I, I, P, P, P],
@@ -185,6 +186,10 @@ class Treecode:
     def set_walk_lds_stack(self, entries):
         """Learned-angle walk: stack entries per wave kept in LDS (512..1024)."""
         self._ck(self._L.bt_set_walk_lds_stack(self._h, int(entries)), "set_walk_lds_stack")
+
+    def set_group_mode(self, mode):
+        """0 = makeGroups' PH-key groups (default), 1 = one group per leaf (tree order)."""
+        self._ck(self._L.bt_set_group_mode(self._h, int(mode)), "set_group_mode")
 
     def walk_stack_max(self):
         """Deepest traversal stack (entries) of any group in the last walk."""

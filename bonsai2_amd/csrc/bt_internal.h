@@ -66,7 +66,7 @@ struct DevCounters {
 struct GraphKey {
   float dt, eps, theta;
   int nleaf, level_split, ncrit;
-  int opening, kahan, quadrupole, walk_variant, stack_ints, blocks_per_cu, lds_stack;
+  int opening, kahan, quadrupole, walk_variant, stack_ints, blocks_per_cu, lds_stack, group_mode;
   float learn_shift, learn_theta_lo, learn_theta_hi;
   int check_ns;
   float check_p99;
@@ -138,6 +138,7 @@ struct Ctx {
   size_t stack_alloc = 0;
   int stack_ints = 16384;
   int lds_stack = kLdsStack;   // learned walk: stack entries per wave kept in LDS
+  int group_mode = 0;          // 0: PH-key groups (makeGroups); 1: one group per leaf
   int walk_blocks = 0;         // persistent grid (blocks of 256)
   int blocks_per_cu = 0;
   int kahan = 1;
@@ -206,6 +207,8 @@ void fill_int(Ctx &c, int *p, int value, int n);
 void launch_ph_keys(Ctx &c);
 void launch_gather_ph(Ctx &c);
 void launch_groups(Ctx &c);
+// groups = the tree's leaves, in tree order; the group order is the tree order
+void launch_leaf_groups(Ctx &c);
 
 // run_if: device flag; the walk (reset, traversal, statistics) does nothing
 // unless it is nonzero (nullptr = always run)

@@ -164,6 +164,14 @@ int phase_multipoles(Ctx &c) {
 }
 
 int phase_groups(Ctx &c) {
+  if (c.group_mode == 1) {
+    record(c, 5);
+    record(c, 6);
+    launch_leaf_groups(c);
+    record(c, 7);
+    BT_LAUNCH_CK();
+    return 0;
+  }
   launch_ph_keys(c);
   record(c, 5);
   radix_sort_pairs(c, c.ph_keys, c.perm, c.keyB, c.valB, c.n, 63, true);
@@ -182,6 +190,9 @@ int phase_forces(Ctx &c) {
   BT_LAUNCH_CK();
   return 0;
 }
+
+const char *const kGroupCapacityMsg =
+    "group capacity exceeded (leaf groups: a leaf holds more than 64 particles)";
 
 static uint64_t mix64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
@@ -357,6 +368,14 @@ int bt_set_walk_lds_stack(bt_ctx *h, int entries) {
   return 0;
 }
 
+int bt_set_group_mode(bt_ctx *h, int mode) {
+  if (!h) return BT_ERR_ARG;
+  if (mode != 0 && mode != 1) return fail(h->c, BT_ERR_ARG, "group mode must be 0 or 1");
+  if (h->c.group_mode != mode) h->c.have_groups = h->c.have_acc = false;
+  h->c.group_mode = mode;
+  return 0;
+}
+
 int bt_get_walk_stack_max(bt_ctx *h, int *entries) {
   if (!h || !entries) return BT_ERR_ARG;
   Ctx &c = h->c;
@@ -506,7 +525,7 @@ int bt_make_groups(bt_ctx *h, int level_split, int ncrit) {
   record(c, 4);
   if (phase_groups(c)) return BT_ERR_HIP;
   if (read_counters(c)) return BT_ERR_HIP;
-  if (c.hc.err & 2) return fail(c, BT_ERR_CAPACITY, "group capacity exceeded");
+  if (c.hc.err & 2) return fail(c, BT_ERR_CAPACITY, kGroupCapacityMsg);
   c.have_groups = true;
   c.have_acc = false;
   return 0;
@@ -607,7 +626,7 @@ int bt_step(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit) {
   if (read_counters(c)) return BT_ERR_HIP;
   c.have_tree = c.have_mp = c.have_groups = c.have_acc = false;
   if (c.hc.err & 1) return fail(c, BT_ERR_CAPACITY, "cell capacity exceeded");
-  if (c.hc.err & 2) return fail(c, BT_ERR_CAPACITY, "group capacity exceeded");
+  if (c.hc.err & 2) return fail(c, BT_ERR_CAPACITY, kGroupCapacityMsg);
   if (c.hc.err & 4) return fail(c, BT_ERR_STACK, "walk stack overflow");
   c.have_tree = c.have_mp = c.have_groups = c.have_acc = true;
   finish_timings(c, 9);
@@ -1145,6 +1164,7 @@ int bt_step_graph(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit, in
   key.stack_ints = c.stack_ints;
   key.blocks_per_cu = c.blocks_per_cu;
   key.lds_stack = c.lds_stack;
+  key.group_mode = c.group_mode;
   if (c.opening == 1) {
     key.learn_shift = c.learn_shift;
     key.learn_theta_lo = c.learn_theta_lo;

@@ -145,6 +145,53 @@ void launch_gather_ph(Ctx &c) {
                      c.n, c.perm, c.pos, c.id, c.pos_ph, c.id_ph);
 }
 
+// A new set of groups starts without the group error bit.
+__global__ void k_groups_init(DevCounters *dc) {
+  if (threadIdx.x == 0) dc->err &= ~2;
+}
+
+// Leaf groups: flag the first particle of every leaf (leaves tile the
+// particles in tree order).  A leaf of more than 64 particles (possible only
+// for a leaf capped at the deepest level) cannot be one wave's group: err 2.
+__global__ __launch_bounds__(kBlock) void k_leaf_starts(int cap_cells, const uint4 *__restrict__ cells,
+                                                        int *__restrict__ flag, DevCounters *dc) {
+  const int nc = min(dc->ncells, cap_cells);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < nc; i += gridDim.x * kBlock) {
+    const uint4 c = cells[i];
+    if (!cell_leaf(c)) continue;
+    flag[c.z] = 1;
+    if ((int)c.w - (int)c.z > kWave) atomicOr(&dc->err, 2);
+  }
+}
+
+// The group order of leaf groups is the tree order itself.
+__global__ __launch_bounds__(kBlock) void k_identity_order(int n, const float4 *__restrict__ pos,
+                                                           const int *__restrict__ id,
+                                                           int *__restrict__ perm,
+                                                           float4 *__restrict__ pos_g,
+                                                           int *__restrict__ id_g) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    perm[i] = i;
+    pos_g[i] = pos[i];
+    id_g[i] = id[i];
+  }
+}
+
+void launch_leaf_groups(Ctx &c) {
+  const int g = grid_for(c.n, kBlock, 8192);
+  hipLaunchKernelGGL(k_groups_init, dim3(1), dim3(64), 0, c.stream, c.dc);
+  hipLaunchKernelGGL(k_identity_order, dim3(g), dim3(kBlock), 0, c.stream, c.n, c.pos, c.id,
+                     c.perm, c.pos_ph, c.id_ph);
+  fill_int(c, c.gtmp0, 0, c.n);
+  hipLaunchKernelGGL(k_leaf_starts, dim3(grid_for(c.cap_cells, kBlock, 8192)), dim3(kBlock), 0,
+                     c.stream, c.cap_cells, c.cells, c.gtmp0, c.dc);
+  scan_exclusive_sum(c, c.gtmp0, c.gtmp1, c.n, &c.dc->ngroups);
+  hipLaunchKernelGGL(k_group_emit, dim3(g), dim3(kBlock), 0, c.stream, c.n, c.nsys, c.d_off,
+                     c.gtmp0, c.gtmp1, c.cap_groups, c.groups, c.group_sys, c.dc);
+  hipLaunchKernelGGL(k_group_sizes, dim3(grid_for(c.cap_groups, kBlock, 4096)), dim3(kBlock), 0,
+                     c.stream, c.n, c.cap_groups, c.groups, c.dc);
+}
+
 void launch_groups(Ctx &c) {
   const int ls = c.level_split < 1 ? 1 : c.level_split;
   uint64_t mask = 0;  // top ls octal digits of the 63-bit key (code/makeGroups.cu:259-266)
@@ -153,6 +200,7 @@ void launch_groups(Ctx &c) {
     if (i < ls) mask |= 7;
   }
   const int g = grid_for(c.n, kBlock, 8192);
+  hipLaunchKernelGGL(k_groups_init, dim3(1), dim3(64), 0, c.stream, c.dc);
   hipLaunchKernelGGL(k_run_starts, dim3(g), dim3(kBlock), 0, c.stream, c.n, c.nsys, c.d_off,
                      c.ph_keys, mask, c.gtmp0);
   scan_inclusive_max(c, c.gtmp0, c.gtmp1, c.n);

@@ -192,6 +192,19 @@ def make_groups(sorted_keys, level_split, ncrit, seg_id=None):
     return out[:ng].copy()
 
 
+def leaf_groups(cells):
+    """One group per leaf cell, ordered by first particle (tree order): the
+    leaf list of code/buildTree.cu:954-958 (collect_leaves, :807-850; unused
+    by the reference, whose list order comes from an atomic counter) as
+    groups.  -> int32 (nleaves, 2) of (pbeg, np)."""
+    cells = np.asarray(cells, np.uint32).reshape(-1, 4)
+    leaf = cells[:, 1] == 0xFFFFFFFF
+    beg = cells[leaf, 2].astype(np.int64)
+    end = cells[leaf, 3].astype(np.int64)
+    order = np.argsort(beg, kind="stable")
+    return np.stack([beg[order], end[order] - beg[order]], axis=1).astype(np.int32)
+
+
 def walk(groups, level_range, pos_ph, pos_tree, cells, size, mono, q0, q1, eps,
          acc_type=0, group_cells=None, cell_ls=None, group_inv_theta=None, counts=False,
          monopole=False):

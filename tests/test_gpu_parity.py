@@ -205,3 +205,50 @@ def test_energies_fp64(run, oracle):
     k_ref, w_ref = oracle.energies(pos, vel, acc_tree, run["eps"])
     assert abs(ek[0] - k_ref) <= 1e-10 * abs(k_ref)
     assert abs(ep[0] - w_ref) <= 1e-10 * abs(w_ref)
+
+
+@pytest.mark.parametrize("nleaf", [16, 64])
+def test_leaf_groups_match_oracle(bt, oracle, nleaf):
+    """bt_set_group_mode(1): one group per leaf (the leaf list of
+    code/buildTree.cu:954-958) in tree order.  Groups equal the oracle's
+    leaves; the walk over them matches the oracle's walk over the same groups
+    (counts within 1e-4, forces within 1e-5); input-order forces map back
+    through the ids; a step runs; mode 0 restores the PH groups."""
+    n, theta, eps = 30000, 0.5, 0.01
+    pos, vel = bt.clustered(n, seed=31)
+    t = bt.Treecode(eps, theta, 64)
+    try:
+        t.alloc(n)
+        t.ptcl_h2d(pos, vel)
+        t.set_group_mode(1)
+        t.buildTree(nleaf)
+        t.computeMultipoles()
+        t.makeGroups(5, 64)
+        ref = oracle.reference_pipeline(pos, nleaf, 64, theta, eps)
+        lg = oracle.leaf_groups(ref["tree"].cells)
+        assert np.array_equal(t.groups(), lg)
+        st = t.computeForces()
+        acc = t.acc(bt.ORDER_PH)
+        acc_o, st_o, _ = oracle.walk(lg, ref["tree"].level_range, ref["pos_tree"],
+                                     ref["pos_tree"], ref["tree"].cells, ref["size"], ref["mono"],
+                                     ref["q0"], ref["q1"], eps)
+        exp = (st_o[0] / n, st_o[1], st_o[2] / n, st_o[3])
+        for a, b in zip(st, exp):
+            assert abs(a - b) <= 1e-4 * max(1.0, abs(b)), (st, exp)
+        assert rel_rms(acc[:, :3], acc_o[:, :3]) <= 1e-5
+        # the group order is the tree order: input order through the tree ids
+        a_in = t.acc(bt.ORDER_INPUT)
+        back = np.empty_like(acc_o)
+        back[ref["tree"].order] = acc_o
+        assert rel_rms(a_in[:, :3], back[:, :3]) <= 1e-5
+        t.step(1.0 / 128, nleaf, 5, 64)
+        assert np.isfinite(t.acc(bt.ORDER_PH)).all()
+        t.set_group_mode(0)
+        t.buildTree(nleaf)
+        t.computeMultipoles()
+        t.makeGroups(5, 64)
+        p2 = t.ptcl_d2h(bt.ORDER_INPUT)[0]
+        ref2 = oracle.reference_pipeline(p2, nleaf, 64, theta, eps)
+        assert np.array_equal(t.groups(), ref2["groups"])
+    finally:
+        t.close()

@@ -240,6 +240,33 @@ def test_walk_theta_to_zero_is_direct_sum(oracle, bt):
     assert np.allclose(r["acc"][:, 3], d[:, 3], rtol=1e-5)
 
 
+@pytest.mark.parametrize("nleaf", [16, 64])
+def test_leaf_groups_tile_the_particles(oracle, bt, nleaf):
+    """Leaf groups (code/buildTree.cu:954-958): one group per leaf, in tree
+    order, tiling [0, n) with at most NLEAF particles each; a walk over them
+    at theta -> 0 is the direct sum, at theta 0.5 it stays within the
+    quadrupole error of the PH-group walk."""
+    pos, _ = bt.plummer(3000, seed=4)
+    box, _, _ = oracle.bounding_box(pos)
+    t = oracle.build_tree(pos, nleaf, box)
+    g = oracle.leaf_groups(t.cells)
+    nleaves = int((t.cells[:, 1] == 0xFFFFFFFF).sum())
+    assert len(g) == nleaves
+    assert g[0, 0] == 0 and g[-1, 0] + g[-1, 1] == len(pos)
+    assert (g[:-1, 0] + g[:-1, 1] == g[1:, 0]).all()
+    assert (g[:, 1] >= 1).all() and (g[:, 1] <= nleaf).all()
+    pos_tree = pos[t.order]
+    d = oracle.direct_fp64(pos_tree, np.arange(len(pos), dtype=np.int32), 0.01)
+    for theta, tol in ((1e-4, 1e-5), (0.5, 5e-3)):
+        size, mono, q0, q1, _ = oracle.multipoles(t.cells, pos_tree, theta)
+        acc, stats, _ = oracle.walk(g, t.level_range, pos_tree, pos_tree, t.cells, size, mono, q0,
+                                    q1, 0.01)
+        rel = np.linalg.norm(acc[:, :3] - d[:, :3], axis=1) / np.linalg.norm(d[:, :3], axis=1)
+        assert np.percentile(rel, 99) < tol, (theta, np.percentile(rel, 99))
+        if theta < 1e-3:
+            assert stats[2] == 0
+
+
 def test_walk_two_body_closed_form(oracle):
     m1, m2, r, eps = 0.7, 0.3, 0.4, 0.05
     pos = np.array([[0, 0, 0, m1], [r, 0, 0, m2]], np.float32)
