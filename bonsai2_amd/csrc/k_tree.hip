@@ -487,8 +487,7 @@ __global__ __launch_bounds__(kBlock) void k_multipole_level(MpArgs a) {
       Moments<float> m{};
       float3 mn = make_float3(1e10f, 1e10f, 1e10f), mx = make_float3(-1e10f, -1e10f, -1e10f);
       const float4 r0 = a.pos[pb];  // reference point: the first particle
-      for (int i = pb; i < pe; i++) {
-        const float4 p = a.pos[i];
+      auto add = [&](const float4 p) {
         mn.x = fminf(mn.x, p.x); mn.y = fminf(mn.y, p.y); mn.z = fminf(mn.z, p.z);
         mx.x = fmaxf(mx.x, p.x); mx.y = fmaxf(mx.y, p.y); mx.z = fmaxf(mx.z, p.z);
         const float dx = p.x - r0.x, dy = p.y - r0.y, dz = p.z - r0.z, w = p.w;
@@ -496,7 +495,17 @@ __global__ __launch_bounds__(kBlock) void k_multipole_level(MpArgs a) {
         m.sx += w * dx; m.sy += w * dy; m.sz += w * dz;
         m.xx += w * dx * dx; m.yy += w * dy * dy; m.zz += w * dz * dz;
         m.xy += w * dx * dy; m.xz += w * dx * dz; m.yz += w * dy * dz;
+      };
+      // particles in order, four loads issued together
+      int i = pb;
+      for (; i + 4 <= pe; i += 4) {
+        const float4 p0 = a.pos[i], p1 = a.pos[i + 1], p2 = a.pos[i + 2], p3 = a.pos[i + 3];
+        add(p0);
+        add(p1);
+        add(p2);
+        add(p3);
       }
+      for (; i < pe; i++) add(a.pos[i]);
       finish_moments<float>(a, c, r0.x, r0.y, r0.z, m, mn, mx);
     } else if (a.L <= kFp64Levels) {
       node_moments<double>(a, c, cd);
