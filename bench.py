@@ -316,6 +316,26 @@ def main():
                    "p50_rel_err": f50, "p99_rel_err": f99,
                    "interactions_per_particle": {"p2p": st_f[0], "p2c": st_f[2]},
                    "flops_per_launch": (20.0 * st_f[0] + 64.0 * st_f[2]) * n}
+        # the fixed angle as accurate as the learned one: the largest theta on a
+        # 0.025 grid whose p99 (same samples, same particles) is at or below the
+        # learned walk's p99
+        eq = {"theta": w["theta"], "walk_ms": compare["walk_ms"], "p99_rel_err": f99}
+        for th in np.arange(w["theta"] + 0.025, 0.801, 0.025):
+            t.set_params(theta=float(th))
+            t.computeMultipoles()
+            t.computeForces()
+            _, e99 = errors()
+            if e99 > p99:
+                break
+            ew = [t.timings()["walk"]]
+            t.computeForces()
+            ew.append(t.timings()["walk"])
+            eq = {"theta": round(float(th), 3), "walk_ms": float(min(ew)), "p99_rel_err": e99}
+        eq["walk_speedup"] = eq["walk_ms"] / float(np.median(lw))
+        compare["equal_p99"] = eq
+        t.set_params(theta=w["theta"])
+        t.computeMultipoles()
+        t.computeForces()
         if args.compare_steps > 0:
             tc0 = time.perf_counter()
             for _ in range(args.compare_steps):
