@@ -104,11 +104,11 @@ __global__ void k_energy_final(int nsys, int nb, const double *__restrict__ part
 
 // 1/sqrt in FP64: the hardware FP64 estimate v_rsq_f64 (relative error
 // about 2^-23) refined by NEWTON Newton steps (each squares the relative
-// error: ~3e-14 after one, below FP64 rounding after two).  r2 is clamped to
-// FLT_MIN so that a zero distance (no softening, the self pair) stays finite.
-template <int NEWTON>
+// error: ~3e-14 after one, below FP64 rounding after two).  CLAMP (no
+// softening): r2 is clamped to FLT_MIN so that the self pair stays finite.
+template <int NEWTON, bool CLAMP>
 __device__ __forceinline__ double rsqrt_fp64(double r2) {
-  const double x = fmax(r2, (double)FLT_MIN);
+  const double x = CLAMP ? fmax(r2, (double)FLT_MIN) : r2;
   double y = __builtin_amdgcn_rsq(x);
 #pragma unroll
   for (int k = 0; k < NEWTON; k++) y = y * (1.5 - 0.5 * x * y * y);
@@ -120,7 +120,8 @@ __device__ __forceinline__ double rsqrt_fp64(double r2) {
 // outside a target's system are skipped.  The kernel is bound by FP64 VALU
 // issue (about 20 FP64 instructions per pair), not by the LDS broadcasts:
 // two targets per thread, halving the LDS reads, measured no faster.
-template <int NEWTON, bool FOREST>
+// SOFT: eps > 0, so r2 > 0 for every pair and 1/sqrt needs no clamp.
+template <int NEWTON, bool FOREST, bool SOFT>
 __global__ __launch_bounds__(kBlock) void k_direct(int ns, const int *__restrict__ idx, int n,
                                                    int nsys, const int *__restrict__ off,
                                                    const float4 *__restrict__ pos, double eps2,
@@ -148,8 +149,8 @@ __global__ __launch_bounds__(kBlock) void k_direct(int ns, const int *__restrict
     for (int q = q0; q < q1; q++) {
       const double4 p = tile[q];
       const double dx = p.x - px, dy = p.y - py, dz = p.z - pz;
-      const double r2 = dx * dx + dy * dy + dz * dz + eps2;
-      const double ri = rsqrt_fp64<NEWTON>(r2);
+      const double r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, eps2)));
+      const double ri = rsqrt_fp64<NEWTON, !SOFT>(r2);
       const double mr = p.w * ri;
       const double mr3 = mr * (ri * ri);
       ax += mr3 * dx; ay += mr3 * dy; az += mr3 * dz; ph -= mr;
@@ -215,8 +216,13 @@ void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out, bool check) 
   // one Newton step (~3e-14) is ample; the measurement path keeps two
   const dim3 grid((ns + kDirectTargets - 1) / kDirectTargets, slices);
   void (*kern)(int, const int *, int, int, const int *, const float4 *, double, double4 *);
-  if (c.nsys > 1) kern = check ? k_direct<1, true> : k_direct<2, true>;
-  else kern = check ? k_direct<1, false> : k_direct<2, false>;
+  if (eps2 > 0.0) {
+    if (c.nsys > 1) kern = check ? k_direct<1, true, true> : k_direct<2, true, true>;
+    else kern = check ? k_direct<1, false, true> : k_direct<2, false, true>;
+  } else {
+    if (c.nsys > 1) kern = check ? k_direct<1, true, false> : k_direct<2, true, false>;
+    else kern = check ? k_direct<1, false, false> : k_direct<2, false, false>;
+  }
   hipLaunchKernelGGL(kern, grid, dim3(kBlock), 0, c.stream, ns, d_idx, c.n, c.nsys, c.d_off,
                      c.pos_ph, eps2, part);
   const int tpb = kBlock / kWave;   // targets per reduce block
