@@ -144,6 +144,10 @@ struct Treecode<float> {
   }
   void computeEnergies() { check(bt_compute_energies(ctx_, &ekin, &epot), "computeEnergies"); }
 
+  // The C-ABI context, for the calls of bonsai2_amd.h that extend the
+  // reference (learned opening angle, leaf groups, captured step, autotuner).
+  bt_ctx *handle() { return ctx_; }
+
   // (ax, ay, az, phi) per particle (reference d_ptclAcc), in input order by default
   std::vector<bonsai2::Particle4<float>> get_acc(int order = BT_ORDER_INPUT) {
     std::vector<bonsai2::Particle4<float>> a(nPtcl);
