@@ -1,7 +1,8 @@
 """GPU tests of the batched paths: a forest of independent systems in one
 context (one persistent walk over all systems' groups) against the oracle run
-on each system alone, the HIP-graph step against the eager step, and the
-autotuner.
+on each system alone (PH groups and leaf groups), the HIP-graph step against
+the eager step (fixed angle, and learned angle with its per-step check), and
+the autotuner.
 
 Bars: per system, forces within 1e-5 relative RMS of the oracle walk on that
 system alone and interaction counts equal; graph replays bit-identical to the
@@ -99,3 +100,73 @@ def test_autotune_picks_a_candidate_and_stays_correct(bt, oracle):
                                     THETA, EPS)
     t.close()
     assert rel_rms(acc[:, :3], ref["acc"][:, :3]) <= 1e-5
+
+
+def test_forest_leaf_groups_match_each_system_alone(bt, oracle):
+    """Leaf groups in a forest: every group lies in one system, and each
+    system's forces equal the oracle's walk over that system's leaves."""
+    pos, vel, off = bt.ensemble(nsys=4, seed=17, log2_min=10, log2_max=13)
+    n = int(off[-1])
+    t = bt.Treecode(EPS, THETA, 64)
+    try:
+        t.alloc(n, off)
+        t.ptcl_h2d(pos, vel)
+        t.set_group_mode(1)
+        t.buildTree(16)
+        t.computeMultipoles()
+        t.makeGroups(5, 64)
+        t.computeForces()
+        acc = t.acc(bt.ORDER_INPUT)
+        ids = t.ids(bt.ORDER_PH)
+        g = t.groups()
+    finally:
+        t.close()
+    # a group's particles all belong to one system
+    sys_of = np.searchsorted(off, ids, side="right") - 1
+    for b, m in g:
+        assert (sys_of[b:b + m] == sys_of[b]).all()
+    for k in range(4):
+        pk = pos[off[k]:off[k + 1]]
+        ref = oracle.reference_pipeline(pk, 16, 64, THETA, EPS)
+        lg = oracle.leaf_groups(ref["tree"].cells)
+        a_o, _, _ = oracle.walk(lg, ref["tree"].level_range, ref["pos_tree"], ref["pos_tree"],
+                                ref["tree"].cells, ref["size"], ref["mono"], ref["q0"], ref["q1"],
+                                EPS)
+        a_ref = np.empty_like(a_o)
+        a_ref[ref["tree"].order] = a_o
+        assert rel_rms(acc[off[k]:off[k + 1], :3], a_ref[:, :3]) <= 1e-5, k
+
+
+def test_graph_step_with_learned_check_in_a_forest(bt):
+    """A forest with the learned angle and its per-step check (the FP64
+    direct sum per system): graph replays equal eager steps bitwise."""
+    from bonsai2_amd import learn
+    params, budget, lo, hi = learn.load()
+    pos, vel, off = bt.ensemble(nsys=5, seed=19, log2_min=11, log2_max=13)
+    n = int(off[-1])
+    ts = []
+    for _ in range(2):
+        t = bt.Treecode(EPS, THETA, 64)
+        t.alloc(n, off)
+        t.ptcl_h2d(pos, vel)
+        t.buildTree(16)
+        t.computeMultipoles()
+        t.makeGroups(5, 64)
+        t.set_mlp_params(params)
+        t.set_opening(1, budget, lo, hi)
+        t.set_opening_check(128, 1e-3, 3)
+        t.computeForces()
+        ts.append(t)
+    eager, graph = ts
+    try:
+        for step in range(3):
+            eager.step(1.0 / 128, 16, 5, 64)
+            graph.step_graph(1.0 / 128, 16, 5, 64, nsteps=1)
+            assert np.array_equal(eager.ptcl_d2h(bt.ORDER_INPUT)[0],
+                                  graph.ptcl_d2h(bt.ORDER_INPUT)[0]), step
+            assert np.array_equal(eager.acc(bt.ORDER_INPUT), graph.acc(bt.ORDER_INPUT)), step
+            assert eager.opening_check() == graph.opening_check(), step
+        assert eager.opening_check()["checks"] == 4
+    finally:
+        eager.close()
+        graph.close()
