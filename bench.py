@@ -38,8 +38,8 @@ FP32_PEAK_TFLOPS = 157.3
 # 1.0x): profiles/hbm_traffic_halo_disk_2p24_pmc.md.  Counters cannot be read
 # from inside the bench run, so only the measured default configurations carry a
 # figure; every other configuration reports null.
-WALK_TRAFFIC_BYTES = {("halo_disk_2p24", "learned"): 4.082e9,
-                      ("halo_disk_2p24", "fixed"): 1.124e10}
+WALK_TRAFFIC_BYTES = {("halo_disk_2p24", "learned"): 4.095e9,
+                      ("halo_disk_2p24", "fixed"): 1.126e10}
 
 WORKLOADS = {
     "halo_disk_2p24": dict(n=1 << 24, ic="halo_disk", theta=0.4, eps=0.002),
