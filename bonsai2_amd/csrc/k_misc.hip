@@ -118,7 +118,7 @@ __device__ __forceinline__ double rsqrt_fp64(double r2) {
 // grid (ceil(ns/256), slices): each block sums a slice of the source index
 // range for 256 targets, the slice staged in LDS as FP64.  FOREST: sources
 // outside a target's system are skipped.  The kernel is bound by FP64 VALU
-// issue (about 20 FP64 instructions per pair), not by the LDS broadcasts:
+// issue (18 FP64 instructions per pair), not by the LDS broadcasts:
 // two targets per thread, halving the LDS reads, measured no faster.
 // SOFT: eps > 0, so r2 > 0 for every pair and 1/sqrt needs no clamp.
 template <int NEWTON, bool FOREST, bool SOFT>
