@@ -168,6 +168,8 @@ def main():
     ap.add_argument("--cpu-groups", type=int, default=32768)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kahan", action="store_true")
+    ap.add_argument("--group-mode", type=int, default=0, choices=(0, 1, 2),
+                    help="0 PH-key groups (makeGroups), 1 leaves, 2 tree cells <= ncrit")
     ap.add_argument("--walk-variant", type=int, default=0, choices=(0, 1),
                     help="0: j-data staged in LDS, 1: j-data by scalar loads")
     ap.add_argument("--opening", default="learned", choices=("learned", "fixed"))
@@ -214,6 +216,7 @@ def main():
     if args.no_kahan:
         t.set_walk_options(kahan=0)
     t.set_walk_variant(args.walk_variant)
+    t.set_group_mode(args.group_mode)
     t.alloc(n)
     t.ptcl_h2d(pos, vel)
     if args.monopole:
@@ -367,7 +370,8 @@ def main():
                                    budget, th_lo, th_hi, args.check_samples, w["theta"],
                                    args.p99_max)) if learned else "fixed theta",
                    "kahan": not args.no_kahan, "quadrupole": not args.monopole,
-                   "walk_jdata": "scalar" if args.walk_variant == 1 else "lds"},
+                   "walk_jdata": "scalar" if args.walk_variant == 1 else "lds",
+                   "groups": ("PH keys", "leaves", "tree cells")[args.group_mode]},
         "interactions_per_s": float((inter / (walk_ms * 1e-3)).mean()),
         "walk_ms": float(walk_ms.mean()),
         "force_evals_per_s_walk": float(n / (walk_ms.mean() * 1e-3)),

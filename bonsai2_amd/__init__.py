@@ -188,7 +188,8 @@ class Treecode:
         self._ck(self._L.bt_set_walk_lds_stack(self._h, int(entries)), "set_walk_lds_stack")
 
     def set_group_mode(self, mode):
-        """0 = makeGroups' PH-key groups (default), 1 = one group per leaf (tree order)."""
+        """0 = makeGroups' PH-key groups (default), 1 = one group per leaf, 2 = one
+        group per tree cell of <= ncrit particles (both in tree order)."""
         self._ck(self._L.bt_set_group_mode(self._h, int(mode)), "set_group_mode")
 
     def walk_stack_max(self):

@@ -138,7 +138,7 @@ struct Ctx {
   size_t stack_alloc = 0;
   int stack_ints = 16384;
   int lds_stack = kLdsStack;   // learned walk: stack entries per wave kept in LDS
-  int group_mode = 0;          // 0: PH-key groups (makeGroups); 1: one group per leaf
+  int group_mode = 0;   // 0: PH-key groups (makeGroups); 1: leaves; 2: tree cells <= ncrit
   int walk_blocks = 0;         // persistent grid (blocks of 256)
   int blocks_per_cu = 0;
   int kahan = 1;
@@ -207,7 +207,8 @@ void fill_int(Ctx &c, int *p, int value, int n);
 void launch_ph_keys(Ctx &c);
 void launch_gather_ph(Ctx &c);
 void launch_groups(Ctx &c);
-// groups = the tree's leaves, in tree order; the group order is the tree order
+// groups = tree cells (mode 1: the leaves; mode 2: cells of <= ncrit
+// particles), in tree order; the group order is the tree order
 void launch_leaf_groups(Ctx &c);
 
 // run_if: device flag; the walk (reset, traversal, statistics) does nothing

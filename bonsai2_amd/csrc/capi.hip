@@ -164,7 +164,7 @@ int phase_multipoles(Ctx &c) {
 }
 
 int phase_groups(Ctx &c) {
-  if (c.group_mode == 1) {
+  if (c.group_mode != 0) {
     record(c, 5);
     record(c, 6);
     launch_leaf_groups(c);
@@ -370,7 +370,7 @@ int bt_set_walk_lds_stack(bt_ctx *h, int entries) {
 
 int bt_set_group_mode(bt_ctx *h, int mode) {
   if (!h) return BT_ERR_ARG;
-  if (mode != 0 && mode != 1) return fail(h->c, BT_ERR_ARG, "group mode must be 0 or 1");
+  if (mode < 0 || mode > 2) return fail(h->c, BT_ERR_ARG, "group mode must be 0, 1 or 2");
   if (h->c.group_mode != mode) h->c.have_groups = h->c.have_acc = false;
   h->c.group_mode = mode;
   return 0;

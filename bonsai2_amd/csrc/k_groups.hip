@@ -150,17 +150,35 @@ __global__ void k_groups_init(DevCounters *dc) {
   if (threadIdx.x == 0) dc->err &= ~2;
 }
 
-// Leaf groups: flag the first particle of every leaf (leaves tile the
-// particles in tree order).  A leaf of more than 64 particles (possible only
-// for a leaf capped at the deepest level) cannot be one wave's group: err 2.
-__global__ __launch_bounds__(kBlock) void k_leaf_starts(int cap_cells, const uint4 *__restrict__ cells,
-                                                        int *__restrict__ flag, DevCounters *dc) {
+// Tree groups: flag the first particle of every group cell.  ncrit = 0: the
+// leaves (they tile the particles in tree order).  ncrit > 0: the cells of at
+// most ncrit particles whose parent holds more (level-1 cells have no parent
+// cell), and the leaves of more than ncrit particles; every particle lies in
+// exactly one of them.  A group of more than 64 particles (only a leaf capped
+// at the deepest level can be one) cannot be one wave's group: err 2.
+__global__ __launch_bounds__(kBlock) void k_tree_group_starts(int cap_cells, int ncrit,
+                                                              const uint4 *__restrict__ cells,
+                                                              int *__restrict__ flag,
+                                                              DevCounters *dc) {
   const int nc = min(dc->ncells, cap_cells);
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < nc; i += gridDim.x * kBlock) {
     const uint4 c = cells[i];
-    if (!cell_leaf(c)) continue;
+    const int np = (int)c.w - (int)c.z;
+    bool group;
+    if (ncrit == 0) {
+      group = cell_leaf(c);
+    } else {
+      const int level = (int)(c.x >> 27);
+      bool parent_splits = true;
+      if (level > 1) {
+        const uint4 p = cells[c.x & 0x07FFFFFFu];
+        parent_splits = (int)p.w - (int)p.z > ncrit;
+      }
+      group = (cell_leaf(c) || np <= ncrit) && parent_splits;
+    }
+    if (!group) continue;
     flag[c.z] = 1;
-    if ((int)c.w - (int)c.z > kWave) atomicOr(&dc->err, 2);
+    if (np > kWave) atomicOr(&dc->err, 2);
   }
 }
 
@@ -183,8 +201,9 @@ void launch_leaf_groups(Ctx &c) {
   hipLaunchKernelGGL(k_identity_order, dim3(g), dim3(kBlock), 0, c.stream, c.n, c.pos, c.id,
                      c.perm, c.pos_ph, c.id_ph);
   fill_int(c, c.gtmp0, 0, c.n);
-  hipLaunchKernelGGL(k_leaf_starts, dim3(grid_for(c.cap_cells, kBlock, 8192)), dim3(kBlock), 0,
-                     c.stream, c.cap_cells, c.cells, c.gtmp0, c.dc);
+  hipLaunchKernelGGL(k_tree_group_starts, dim3(grid_for(c.cap_cells, kBlock, 8192)), dim3(kBlock),
+                     0, c.stream, c.cap_cells, c.group_mode == 2 ? c.ncrit : 0, c.cells, c.gtmp0,
+                     c.dc);
   scan_exclusive_sum(c, c.gtmp0, c.gtmp1, c.n, &c.dc->ngroups);
   hipLaunchKernelGGL(k_group_emit, dim3(g), dim3(kBlock), 0, c.stream, c.n, c.nsys, c.d_off,
                      c.gtmp0, c.gtmp1, c.cap_groups, c.groups, c.group_sys, c.dc);

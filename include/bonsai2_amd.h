@@ -166,10 +166,12 @@ int bt_set_walk_lds_stack(bt_ctx *ctx, int entries);
 /* Groups of the walk (no reference use; code/buildTree.cu:954-958 collects the
  * leaf list d_leafList and leaves it unused): mode 0 (default) = makeGroups'
  * PH-key groups (code/makeGroups.cu); mode 1 = one group per leaf cell, in
- * tree order.  In mode 1 the "PH order" of bt_get_acc and the other per-group
- * arrays is the tree order (perm = identity), level_split and ncrit are not
- * used, and a leaf of more than 64 particles (a leaf capped at the deepest
- * level) returns BT_ERR_CAPACITY.  Takes effect at the next makeGroups. */
+ * tree order; mode 2 = one group per tree cell of at most ncrit particles
+ * whose parent holds more (and per leaf of more than ncrit), in tree order.
+ * In modes 1 and 2 the "PH order" of bt_get_acc and the other per-group
+ * arrays is the tree order (perm = identity), level_split is not used, and a
+ * leaf of more than 64 particles (a leaf capped at the deepest level) returns
+ * BT_ERR_CAPACITY.  Takes effect at the next makeGroups. */
 int bt_set_group_mode(bt_ctx *ctx, int mode);
 /* Deepest traversal stack of any group in the last walk (entries, LDS and
  * global parts together): the stack_ints a walk of this tree needs. */

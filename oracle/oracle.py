@@ -205,6 +205,23 @@ def leaf_groups(cells):
     return np.stack([beg[order], end[order] - beg[order]], axis=1).astype(np.int32)
 
 
+def tree_groups(cells, ncrit):
+    """One group per tree cell of at most ncrit particles whose parent holds
+    more (level-1 cells have no parent cell), and per leaf of more than ncrit,
+    ordered by first particle: the groups the octree itself defines.
+    -> int32 (ngroups, 2) of (pbeg, np)."""
+    cells = np.asarray(cells, np.uint32).reshape(-1, 4)
+    leaf = cells[:, 1] == 0xFFFFFFFF
+    np_ = cells[:, 3].astype(np.int64) - cells[:, 2].astype(np.int64)
+    level = cells[:, 0] >> 27
+    parent = (cells[:, 0] & 0x07FFFFFF).astype(np.int64)
+    parent_np = np.where(level > 1, np_[parent], ncrit + 1)
+    group = (leaf | (np_ <= ncrit)) & (parent_np > ncrit)
+    beg = cells[group, 2].astype(np.int64)
+    order = np.argsort(beg, kind="stable")
+    return np.stack([beg[order], np_[group][order]], axis=1).astype(np.int32)
+
+
 def walk(groups, level_range, pos_ph, pos_tree, cells, size, mono, q0, q1, eps,
          acc_type=0, group_cells=None, cell_ls=None, group_inv_theta=None, counts=False,
          monopole=False):

@@ -207,25 +207,27 @@ def test_energies_fp64(run, oracle):
     assert abs(ep[0] - w_ref) <= 1e-10 * abs(w_ref)
 
 
-@pytest.mark.parametrize("nleaf", [16, 64])
-def test_leaf_groups_match_oracle(bt, oracle, nleaf):
+@pytest.mark.parametrize("nleaf,mode", [(16, 1), (64, 1), (16, 2)])
+def test_leaf_groups_match_oracle(bt, oracle, nleaf, mode):
     """bt_set_group_mode(1): one group per leaf (the leaf list of
-    code/buildTree.cu:954-958) in tree order.  Groups equal the oracle's
-    leaves; the walk over them matches the oracle's walk over the same groups
-    (counts within 1e-4, forces within 1e-5); input-order forces map back
-    through the ids; a step runs; mode 0 restores the PH groups."""
+    code/buildTree.cu:954-958) in tree order; mode 2: one group per tree cell
+    of at most NCRIT particles.  Groups equal the oracle's; the walk over them
+    matches the oracle's walk over the same groups (counts within 1e-4,
+    forces within 1e-5); input-order forces map back through the ids; a step
+    runs; mode 0 restores the PH groups."""
     n, theta, eps = 30000, 0.5, 0.01
     pos, vel = bt.clustered(n, seed=31)
     t = bt.Treecode(eps, theta, 64)
     try:
         t.alloc(n)
         t.ptcl_h2d(pos, vel)
-        t.set_group_mode(1)
+        t.set_group_mode(mode)
         t.buildTree(nleaf)
         t.computeMultipoles()
         t.makeGroups(5, 64)
         ref = oracle.reference_pipeline(pos, nleaf, 64, theta, eps)
-        lg = oracle.leaf_groups(ref["tree"].cells)
+        lg = (oracle.leaf_groups(ref["tree"].cells) if mode == 1 else
+              oracle.tree_groups(ref["tree"].cells, 64))
         assert np.array_equal(t.groups(), lg)
         st = t.computeForces()
         acc = t.acc(bt.ORDER_PH)

@@ -267,6 +267,31 @@ def test_leaf_groups_tile_the_particles(oracle, bt, nleaf):
             assert stats[2] == 0
 
 
+@pytest.mark.parametrize("nleaf,ncrit", [(16, 64), (32, 32), (64, 48)])
+def test_tree_groups_are_the_topmost_small_cells(oracle, bt, nleaf, ncrit):
+    """Tree groups: from every leaf, climb while the parent holds <= ncrit
+    particles; the cells reached, ordered by first particle, are the groups
+    (an independent derivation of oracle.tree_groups).  They tile [0, N)."""
+    pos, _ = bt.clustered(5000, seed=6)
+    box, _, _ = oracle.bounding_box(pos)
+    t = oracle.build_tree(pos, nleaf, box)
+    cells = t.cells.astype(np.int64)
+    np_ = cells[:, 3] - cells[:, 2]
+    level = cells[:, 0] >> 27
+    parent = cells[:, 0] & 0x07FFFFFF
+    top = set()
+    for c in np.where(cells[:, 1] == 0xFFFFFFFF)[0]:
+        while level[c] > 1 and np_[parent[c]] <= ncrit:
+            c = parent[c]
+        top.add(int(c))
+    want = sorted((int(cells[c, 2]), int(np_[c])) for c in top)
+    g = oracle.tree_groups(t.cells, ncrit)
+    assert [tuple(x) for x in g.tolist()] == want
+    assert g[0, 0] == 0 and g[-1, 0] + g[-1, 1] == len(pos)
+    assert (g[:-1, 0] + g[:-1, 1] == g[1:, 0]).all()
+    assert (g[:, 1] <= max(ncrit, nleaf)).all()
+
+
 def test_walk_two_body_closed_form(oracle):
     m1, m2, r, eps = 0.7, 0.3, 0.4, 0.05
     pos = np.array([[0, 0, 0, m1], [r, 0, 0, m2]], np.float32)
