@@ -78,6 +78,10 @@ _SIGS = {
     "bt_ic_plummer": [I, ctypes.c_uint64, D, P, P],
     "bt_ic_clustered": [I, I, ctypes.c_uint64, D, D, D, P, P],
     "bt_ic_halo_disk": [I, ctypes.c_uint64, D, D, D, P, P],
+    "bt_host_alloc": [ctypes.POINTER(P), ctypes.c_size_t],
+    "bt_host_free": [P],
+    "bt_get_device_buffer": [P, I, ctypes.POINTER(P), P],
+    "bt_copy_device_buffer": [P, I, P, ctypes.c_int64],
     "bt_read_tipsy": [ctypes.c_char_p, P, P, P, I],
     "bt_write_tipsy": [ctypes.c_char_p, I, P, P, D, F],
 }
@@ -314,8 +318,10 @@ class Treecode:
             raise ValueError("particle count differs from alloc()")
         self._ck(self._L.bt_set_particles(self._h, _p(pos), _p(vel)), "ptcl_h2d")
 
-    def ptcl_d2h(self, order=ORDER_INPUT):
-        """ptcl_d2h() (code/Treecode.h:200-204) -> (pos, vel) in the given order."""
+    def ptcl_d2h(self, order=ORDER_TREE):
+        """ptcl_d2h() (code/Treecode.h:200-204) -> (pos, vel).  The default is the
+        reference's: the device arrays as they are (tree order after
+        buildTree, input order before the first build)."""
         pos = np.empty((self.nPtcl, 4), np.float32)
         vel = np.empty((self.nPtcl, 4), np.float32)
         self._ck(self._L.bt_get_particles(self._h, _p(pos), _p(vel), int(order)), "ptcl_d2h")
@@ -344,13 +350,16 @@ class Treecode:
         return tuple(float(x) for x in st)
 
     def moveParticles(self, dt):
+        """First half of a kick-drift-kick step (v += a dt/2, x += v dt); the
+        next force evaluation applies the second half kick."""
         self._ck(self._L.bt_move_particles(self._h, float(dt)), "moveParticles")
 
     def kick(self, dt_half):
         self._ck(self._L.bt_kick(self._h, float(dt_half)), "kick")
 
     def computeEnergies(self):
-        """FP64 (K, W) per system (arrays of length nsys)."""
+        """FP64 (K, W) per system (arrays of length nsys); evaluates the forces
+        first when the particles moved since the last evaluation."""
         ek = np.zeros(self.nsys, np.float64)
         ep = np.zeros(self.nsys, np.float64)
         self._ck(self._L.bt_compute_energies(self._h, _p(ek), _p(ep)), "computeEnergies")

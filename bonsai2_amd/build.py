@@ -26,6 +26,10 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
              "-Wno-unused-function", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(HERE, "..", "include")]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(HERE, "..", "include")]
+# what `hipconfig --cpp_config` prints: HIP's vector types (double4, ...) for
+# host-compiled code that includes include/Treecode.h
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIP_HOST_FLAGS = ["-D__HIP_PLATFORM_AMD__=", "-I" + os.path.join(ROCM, "include")]
 
 
 def _run(cmd):
@@ -61,7 +65,8 @@ def build(verbose=False):
     os.replace(tmp, OUT)
     # the command-line driver, linked against the library next to it
     exe = os.path.join(HERE, "treecode")
-    rc, cmd, out = _run(["g++"] + HOST_FLAGS + [os.path.join(CSRC, "treecode_main.cpp"), "-o",
+    rc, cmd, out = _run(["g++"] + HOST_FLAGS + HIP_HOST_FLAGS +
+                        [os.path.join(CSRC, "treecode_main.cpp"), "-o",
                                                  exe + ".tmp", "-L" + HERE, "-lbonsai2_amd",
                                                  "-Wl,-rpath,$ORIGIN"])
     if verbose or rc != 0:
@@ -69,15 +74,6 @@ def build(verbose=False):
     if rc != 0:
         raise RuntimeError("driver link failed")
     os.replace(exe + ".tmp", exe)
-    # the issue-rate probe behind the walk's roofline (a standalone program)
-    probe = os.path.join(HERE, "probe_issue")
-    rc, cmd, out = _run([HIPCC] + [f for f in HIP_FLAGS if f != "-fPIC"] +
-                        ["-I" + CSRC, os.path.join(CSRC, "probe_issue.hip"), "-o", probe + ".tmp"])
-    if verbose or rc != 0:
-        sys.stderr.write(cmd + "\n" + out)
-    if rc != 0:
-        raise RuntimeError("probe build failed")
-    os.replace(probe + ".tmp", probe)
     return OUT
 
 

@@ -58,7 +58,15 @@ struct DevCounters {
   int check_calls;  // checks drawn so far (seeds the sample indices)
   int fallback;     // 1: the last check failed and the fixed-angle walk ran
   int checks, fallbacks;
-  int pad[3];
+  // Sticky state, cleared only by the host (launch_clear_sticky) at the start
+  // of each API call that steps or walks: every error bit any phase raised
+  // since (err is cleared by each phase for itself), the deepest walk stack of
+  // every walk since, and the closing half kicks done since.  After an error
+  // the kick and drift kernels do nothing, so a failed step leaves the
+  // particles as after its first half kick and drift.
+  int err_sticky;
+  int stack_hw;
+  int steps_done;
 };
 
 // The values captured graphs bake into their kernel arguments: a graph is
@@ -170,6 +178,11 @@ struct Ctx {
   DevCounters *dc = nullptr;
   DevCounters hc;
 
+  // moveParticles' half kick still owed: the next computeForces closes the
+  // kick-drift-kick step with it (0 = none)
+  float pending_kick = 0.f;
+  int check_cap = 0;             // samples d_check / d_check_idx are sized for
+
   // state
   bool have_particles = false, have_tree = false, have_mp = false, have_groups = false,
        have_acc = false;
@@ -232,8 +245,10 @@ void launch_check_idx(Ctx &c, int ns);
 // fallback flag (p99 > check_p99) and counts checks and fallbacks
 void launch_learn_check(Ctx &c, int ns, const int *d_idx, const double *d_direct, double *out);
 
-void launch_kick(Ctx &c, float dt_half);
+// closing: the second half kick of a step (counted in steps_done)
+void launch_kick(Ctx &c, float dt_half, bool closing = false);
 void launch_drift(Ctx &c, float dt);
+void launch_clear_sticky(Ctx &c);
 void launch_energies(Ctx &c);
 int direct_slices(int ns);   // d_out of launch_direct: ns * (slices + 1) double4
 void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out, bool check = false);

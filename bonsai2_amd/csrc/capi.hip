@@ -229,20 +229,64 @@ int check_build_errors(Ctx &c) {
   return 0;
 }
 
+// Grows the cell arrays to their largest size (3n + 64 per system + 1024:
+// every level adds at most n cells, and a cell holds at least one particle
+// unless it is a node, whose children hold more than NLEAF).  Captured graphs
+// hold the old arrays and are dropped.
+int regrow_cells(Ctx &c) {
+  const long long cap = 3LL * c.n + 64LL * c.nsys + 1024;
+  if (cap > 0x1FFFFFFF || cap <= c.cap_cells)
+    return fail(c, BT_ERR_CAPACITY, "cell capacity exceeded");
+  (void)hipStreamSynchronize(c.stream);
+  release_graph(c);
+  free_cell_arrays(c);
+  if (alloc_cells(c, (int)cap)) return BT_ERR_HIP;
+  return 0;
+}
+
 // Build, growing the cell arrays once if they were too small.
 int build_checked(Ctx &c) {
   if (phase_build(c)) return BT_ERR_HIP;
   if (read_counters(c)) return BT_ERR_HIP;
   if (c.hc.err & 1) {
-    const long long cap = 3LL * c.n + 64LL * c.nsys + 1024;
-    if (cap > 0x1FFFFFFF) return fail(c, BT_ERR_CAPACITY, "cell capacity exceeded");
-    release_graph(c);
-    free_cell_arrays(c);
-    if (alloc_cells(c, (int)cap)) return BT_ERR_HIP;
+    const int rc = regrow_cells(c);
+    if (rc) return rc;
     launch_levels(c);
     BT_LAUNCH_CK();
     if (check_build_errors(c)) return BT_ERR_CAPACITY;
   }
+  return 0;
+}
+
+// The second half of a kick-drift-kick step, after the drift: the forces at
+// the new positions and the closing half kick.  Queued without host
+// synchronisation.
+int step_second_half(Ctx &c, float dt) {
+  if (phase_build(c) || phase_multipoles(c) || phase_groups(c) || phase_forces(c))
+    return BT_ERR_HIP;
+  launch_learned_check(c);
+  launch_kick(c, 0.5f * dt, true);
+  record(c, 9);
+  BT_LAUNCH_CK();
+  return 0;
+}
+
+// After the queued second half of a step: on a cell-capacity failure (the
+// walk and the closing kick were skipped, the particles hold the step's first
+// half kick and drift) grow the cells once and run the second half again.
+int finish_step(Ctx &c, float dt) {
+  if (read_counters(c)) return BT_ERR_HIP;
+  if (c.hc.err_sticky == 1) {
+    int rc = regrow_cells(c);
+    if (rc) return rc;
+    launch_clear_sticky(c);
+    rc = step_second_half(c, dt);
+    if (rc) return rc;
+    if (read_counters(c)) return BT_ERR_HIP;
+  }
+  if (c.hc.err_sticky & 1) return fail(c, BT_ERR_CAPACITY, "cell capacity exceeded");
+  if (c.hc.err_sticky & 2) return fail(c, BT_ERR_CAPACITY, kGroupCapacityMsg);
+  if (c.hc.err_sticky & 4) return fail(c, BT_ERR_STACK, "walk stack overflow");
   return 0;
 }
 
@@ -352,7 +396,9 @@ int bt_set_params(bt_ctx *h, float eps, float theta) {
 int bt_set_walk_options(bt_ctx *h, int kahan, int stack_ints, int blocks_per_cu) {
   if (!h) return BT_ERR_ARG;
   Ctx &c = h->c;
-  if (stack_ints < 256 || blocks_per_cu < 0 || blocks_per_cu > 64)
+  // the stack must hold the level-1 cells a walk starts from (<= 8); every
+  // push beyond its capacity is caught (BT_ERR_STACK)
+  if (stack_ints < 64 || blocks_per_cu < 0 || blocks_per_cu > 64)
     return fail(c, BT_ERR_ARG, "bad walk options");
   c.kahan = kahan ? 1 : 0;
   c.stack_ints = stack_ints;
@@ -380,7 +426,7 @@ int bt_get_walk_stack_max(bt_ctx *h, int *entries) {
   if (!h || !entries) return BT_ERR_ARG;
   Ctx &c = h->c;
   if (c.dc && read_counters(c)) return BT_ERR_HIP;
-  *entries = c.hc.stack_max;
+  *entries = c.hc.stack_hw;
   return 0;
 }
 
@@ -466,6 +512,7 @@ int bt_set_particles(bt_ctx *h, const float *pos4, const float *vel4) {
   BT_CK(hipMemcpyAsync(c.id, ids.data(), sizeof(int) * c.n, hipMemcpyHostToDevice, c.stream));
   BT_CK(hipStreamSynchronize(c.stream));
   c.have_particles = true;
+  c.pending_kick = 0.f;
   c.have_tree = c.have_mp = c.have_groups = c.have_acc = false;
   return 0;
 }
@@ -481,6 +528,7 @@ int bt_set_particles_device(bt_ctx *h, const void *dpos, const void *dvel) {
   BT_CK(hipMemcpyAsync(c.id, ids.data(), sizeof(int) * c.n, hipMemcpyHostToDevice, c.stream));
   BT_CK(hipStreamSynchronize(c.stream));
   c.have_particles = true;
+  c.pending_kick = 0.f;
   c.have_tree = c.have_mp = c.have_groups = c.have_acc = false;
   return 0;
 }
@@ -537,12 +585,16 @@ int bt_compute_forces(bt_ctx *h, int intcount, double stats[4]) {
   c.err.clear();
   if (!c.have_mp || !c.have_groups)
     return fail(c, BT_ERR_STATE, "computeMultipoles and makeGroups first");
+  launch_clear_sticky(c);
   record(c, 7);
   if (phase_forces(c)) return BT_ERR_HIP;
   launch_learned_check(c);
+  // a moveParticles before this evaluation owes its step's closing half kick
+  if (c.pending_kick != 0.f) launch_kick(c, c.pending_kick, true);
   BT_LAUNCH_CK();
   if (read_counters(c)) return BT_ERR_HIP;
-  if (c.hc.err & 4) return fail(c, BT_ERR_STACK, "walk stack overflow");
+  if (c.hc.err_sticky & 4) return fail(c, BT_ERR_STACK, "walk stack overflow");
+  c.pending_kick = 0.f;
   c.have_acc = true;
   if (intcount && stats) {
     unsigned long long w[4];
@@ -560,6 +612,7 @@ int bt_kick(bt_ctx *h, float dth) {
   if (!h) return BT_ERR_ARG;
   Ctx &c = h->c;
   if (!c.have_acc) return fail(c, BT_ERR_STATE, "computeForces first");
+  launch_clear_sticky(c);
   launch_kick(c, dth);
   BT_LAUNCH_CK();
   BT_CK(hipStreamSynchronize(c.stream));
@@ -570,11 +623,14 @@ int bt_move_particles(bt_ctx *h, float dt) {
   if (!h) return BT_ERR_ARG;
   Ctx &c = h->c;
   if (!c.have_acc) return fail(c, BT_ERR_STATE, "computeForces first");
+  launch_clear_sticky(c);
   launch_kick(c, 0.5f * dt);
   launch_drift(c, dt);
   BT_LAUNCH_CK();
   BT_CK(hipStreamSynchronize(c.stream));
-  // positions moved: the tree, moments, groups and forces are stale
+  // positions moved: the tree, moments, groups and forces are stale; the
+  // next force evaluation applies the step's second half kick
+  c.pending_kick = 0.5f * dt;
   c.have_tree = c.have_mp = c.have_groups = c.have_acc = false;
   return 0;
 }
@@ -582,7 +638,16 @@ int bt_move_particles(bt_ctx *h, float dt) {
 int bt_compute_energies(bt_ctx *h, double *ekin, double *epot) {
   if (!h) return BT_ERR_ARG;
   Ctx &c = h->c;
-  if (!c.have_acc) return fail(c, BT_ERR_STATE, "computeForces first (phi is needed)");
+  if (!c.have_particles) return fail(c, BT_ERR_STATE, "no particles");
+  if (!c.have_acc) {
+    // phi of the current positions (after moveParticles: the new forces also
+    // close the step's kick, so K and W are taken at the same time)
+    int rc = bt_build_tree(h, c.nleaf);
+    if (!rc) rc = bt_compute_multipoles(h);
+    if (!rc) rc = bt_make_groups(h, c.level_split, c.ncrit);
+    if (!rc) rc = bt_compute_forces(h, 0, nullptr);
+    if (rc) return rc;
+  }
   launch_energies(c);
   BT_LAUNCH_CK();
   std::vector<double> e(2 * (size_t)c.nsys);
@@ -615,19 +680,13 @@ int bt_step(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit) {
     if (!rc) rc = bt_compute_forces(h, 1, nullptr);
     if (rc) return rc;
   }
+  launch_clear_sticky(c);
   launch_kick(c, 0.5f * dt);
   launch_drift(c, dt);
-  if (phase_build(c) || phase_multipoles(c) || phase_groups(c) || phase_forces(c))
-    return BT_ERR_HIP;
-  launch_learned_check(c);
-  launch_kick(c, 0.5f * dt);
-  record(c, 9);
-  BT_LAUNCH_CK();
-  if (read_counters(c)) return BT_ERR_HIP;
   c.have_tree = c.have_mp = c.have_groups = c.have_acc = false;
-  if (c.hc.err & 1) return fail(c, BT_ERR_CAPACITY, "cell capacity exceeded");
-  if (c.hc.err & 2) return fail(c, BT_ERR_CAPACITY, kGroupCapacityMsg);
-  if (c.hc.err & 4) return fail(c, BT_ERR_STACK, "walk stack overflow");
+  int rc = step_second_half(c, dt);
+  if (!rc) rc = finish_step(c, dt);
+  if (rc) return rc;
   c.have_tree = c.have_mp = c.have_groups = c.have_acc = true;
   finish_timings(c, 9);
   return 0;
@@ -842,6 +901,74 @@ int bt_get_timings(bt_ctx *h, float ms[10]) {
   return 0;
 }
 
+
+// ---- memory of the reference's public members ------------------------------
+
+int bt_host_alloc(void **ptr, size_t bytes) {
+  if (!ptr) return BT_ERR_ARG;
+  *ptr = nullptr;
+  if (bytes == 0) return 0;
+  return hipHostMalloc(ptr, bytes, hipHostMallocDefault) == hipSuccess ? 0 : BT_ERR_HIP;
+}
+
+int bt_host_free(void *ptr) {
+  if (!ptr) return 0;
+  return hipHostFree(ptr) == hipSuccess ? 0 : BT_ERR_HIP;
+}
+
+namespace {
+
+// Address, element count and element size of one of the BT_BUF_* arrays.
+int device_buffer(Ctx &c, int which, void **ptr, int64_t *count, size_t *elem) {
+  if (c.dc && read_counters(c)) return BT_ERR_HIP;
+  const int64_t n = c.n, nc = c.hc.ncells, ng = c.hc.ngroups;
+  void *p = nullptr;
+  int64_t k = 0;
+  size_t e = 0;
+  switch (which) {
+    case BT_BUF_PTCL_POS: p = c.pos; k = n; e = sizeof(float4); break;
+    case BT_BUF_PTCL_VEL: p = c.vel; k = n; e = sizeof(float4); break;
+    case BT_BUF_PTCL_POS_TMP: p = c.pos_ph; k = n; e = sizeof(float4); break;
+    case BT_BUF_PTCL_ACC: p = c.acc; k = n; e = sizeof(float4); break;
+    case BT_BUF_DOMAIN: p = c.box; k = c.nsys; e = sizeof(float4); break;
+    case BT_BUF_LEVEL_BEGIDX: p = c.level_beg; k = kLevels; e = sizeof(int2); break;
+    case BT_BUF_CELL_DATA: p = c.cells; k = nc; e = sizeof(uint4); break;
+    case BT_BUF_CELL_SIZE: p = c.csize; k = nc; e = sizeof(float4); break;
+    case BT_BUF_CELL_MONOPOLE: p = c.cmono; k = nc; e = sizeof(float4); break;
+    case BT_BUF_CELL_QUAD0: p = c.cq0; k = nc; e = sizeof(float4); break;
+    case BT_BUF_CELL_QUAD1: p = c.cq1; k = nc; e = sizeof(float2); break;
+    case BT_BUF_GROUP_LIST: p = c.groups; k = ng; e = sizeof(int2); break;
+    case BT_BUF_PH_KEY: p = c.ph_keys; k = n; e = sizeof(uint64_t); break;
+    case BT_BUF_PH_PERM: p = c.perm; k = n; e = sizeof(int); break;
+    default: return fail(c, BT_ERR_ARG, "unknown device buffer");
+  }
+  if (ptr) *ptr = p;
+  if (count) *count = p ? k : 0;
+  if (elem) *elem = e;
+  return 0;
+}
+
+}  // namespace
+
+int bt_get_device_buffer(bt_ctx *h, int which, void **ptr, int64_t *count) {
+  if (!h) return BT_ERR_ARG;
+  return device_buffer(h->c, which, ptr, count, nullptr);
+}
+
+int bt_copy_device_buffer(bt_ctx *h, int which, void *host, int64_t count) {
+  if (!h || !host || count < 0) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  void *p = nullptr;
+  int64_t k = 0;
+  size_t e = 0;
+  const int rc = device_buffer(c, which, &p, &k, &e);
+  if (rc) return rc;
+  if (count > k) return fail(c, BT_ERR_ARG, "count exceeds the buffer");
+  if (count == 0) return 0;
+  BT_CK(hipMemcpyAsync(host, p, e * (size_t)count, hipMemcpyDeviceToHost, c.stream));
+  BT_CK(hipStreamSynchronize(c.stream));
+  return 0;
+}
 
 // ---- learned opening angle ------------------------------------------------
 
@@ -1105,10 +1232,15 @@ int bt_set_opening_check(bt_ctx *h, int nsamples, float p99_max, uint64_t seed) 
   Ctx &c = h->c;
   if (nsamples < 0 || nsamples > 1024 || !(p99_max > 0.f))
     return fail(c, BT_ERR_ARG, "samples must be in [0, 1024], p99_max > 0");
-  if (nsamples > 0) {
+  if (nsamples > c.check_cap) {
+    // captured steps hold the old buffers: drop them before the buffers go
+    (void)hipStreamSynchronize(c.stream);
+    release_graph(c);
+    c.check_cap = 0;
     if (dev_alloc(c, &c.d_check, (size_t)nsamples * (direct_slices(nsamples) + 1) * 4 + 4) ||
         dev_alloc(c, &c.d_check_idx, nsamples))
       return BT_ERR_HIP;
+    c.check_cap = nsamples;
   }
   c.check_ns = nsamples;
   c.check_p99 = p99_max;
@@ -1194,12 +1326,7 @@ int bt_step_graph(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit, in
       }
       launch_kick(c, 0.5f * dt);
       launch_drift(c, dt);
-      rc = phase_build(c);   // swaps the buffers on the host side
-      if (!rc) rc = phase_multipoles(c);
-      if (!rc) rc = phase_groups(c);
-      if (!rc) rc = phase_forces(c);
-      launch_learned_check(c);
-      launch_kick(c, 0.5f * dt);
+      rc = step_second_half(c, dt);   // the build swaps the buffers on the host side
       if (hipStreamEndCapture(c.stream, &c.graph[g]) != hipSuccess) rc = BT_ERR_HIP;
       if (!rc && hipGraphInstantiate(&c.graph_exec[g], c.graph[g], nullptr, nullptr, 0) !=
                      hipSuccess)
@@ -1215,17 +1342,42 @@ int bt_step_graph(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit, in
     c.graph_pos0 = pos0;
     std::memcpy(&c.graph_key, &key, sizeof(key));
   }
+  // errors of every replay stay on record (err_sticky); after one, the later
+  // replays leave the particles alone (their kicks and drifts do nothing)
+  launch_clear_sticky(c);
   for (int k = 0; k < nsteps; k++) {
     BT_CK(hipGraphLaunch(c.graph_exec[c.pos == c.graph_pos0 ? 0 : 1], c.stream));
     swap_particle_buffers(c);
   }
   if (read_counters(c)) return BT_ERR_HIP;
   c.have_tree = c.have_mp = c.have_groups = c.have_acc = false;
-  if (c.hc.err & 3) {
-    release_graph(c);
-    return fail(c, BT_ERR_CAPACITY, "cell or group capacity exceeded: run bt_step to regrow");
+  if (c.hc.err_sticky == 1) {
+    // cell capacity: steps_done steps completed, the next one stopped after
+    // its drift; grow the cells, finish that step, run the rest eagerly
+    const int done = c.hc.steps_done;
+    int rc = regrow_cells(c);
+    if (!rc) {
+      launch_clear_sticky(c);
+      rc = step_second_half(c, dt);
+    }
+    if (!rc) rc = finish_step(c, dt);
+    if (rc) return rc;
+    c.have_tree = c.have_mp = c.have_groups = c.have_acc = true;
+    for (int k = done + 1; k < nsteps; k++) {
+      rc = bt_step(h, dt, nleaf, level_split, ncrit);
+      if (rc) return rc;
+    }
+    return 0;
   }
-  if (c.hc.err & 4) return fail(c, BT_ERR_STACK, "walk stack overflow");
+  if (c.hc.err_sticky & 3) {
+    release_graph(c);
+    return fail(c, BT_ERR_CAPACITY, kGroupCapacityMsg);
+  }
+  if (c.hc.err_sticky & 4) {
+    c.err = "walk stack overflow in replay " + std::to_string(c.hc.steps_done + 1) + " of " +
+            std::to_string(nsteps) + " (the particles hold that step's first half kick and drift)";
+    return BT_ERR_STACK;
+  }
   c.have_tree = c.have_mp = c.have_groups = c.have_acc = true;
   return 0;
 }
@@ -1261,7 +1413,7 @@ int bt_autotune(bt_ctx *h, int level_split, int flags, double *table, int *ntabl
   auto row = [&](int nleaf, int ncrit, int bpc, float tot, float walk) {
     if (table) {
       double *r = table + 6 * nt;
-      r[0] = nleaf; r[1] = ncrit; r[2] = bpc; r[3] = tot; r[4] = walk; r[5] = c.hc.stack_max;
+      r[0] = nleaf; r[1] = ncrit; r[2] = bpc; r[3] = tot; r[4] = walk; r[5] = c.hc.stack_hw;
     }
     nt++;
   };
@@ -1284,7 +1436,7 @@ int bt_autotune(bt_ctx *h, int level_split, int flags, double *table, int *ntabl
     rc = timed_evaluation(h, bl, level_split, bc, &tot, &w);
     if (!rc) rc = timed_evaluation(h, bl, level_split, bc, &tot, &w);
     if (rc) { c.timing = timing; return rc; }
-    deepest = std::max(deepest, c.hc.stack_max);
+    deepest = std::max(deepest, c.hc.stack_hw);
     row(bl, bc, bpc, tot, w);
     if (w < best_w) { best_w = w; bb = bpc; }
   }

@@ -115,7 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_group_emit(int n, int nsys, const in
     if (!flag[i]) continue;
     const int g = slot[i];
     if (g >= cap) {
-      atomicOr(&dc->err, 2);
+      { atomicOr(&dc->err, 2); atomicOr(&dc->err_sticky, 2); }
       continue;
     }
     groups[g] = make_int2(i, 0);
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_tree_group_starts(int cap_cells, int
     }
     if (!group) continue;
     flag[c.z] = 1;
-    if (np > kWave) atomicOr(&dc->err, 2);
+    if (np > kWave) { atomicOr(&dc->err, 2); atomicOr(&dc->err_sticky, 2); }
   }
 }
 

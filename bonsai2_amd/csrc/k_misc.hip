@@ -27,9 +27,14 @@ constexpr int kEnergyBlocks = 64;
 constexpr int kDirectSlices = 64;
 constexpr int kDirectTargets = kBlock;         // targets per block of the direct sum
 
+// Both integrator kernels do nothing once an error is on record (err_sticky):
+// a step whose build or walk failed does not kick with invalid forces.
 __global__ __launch_bounds__(kBlock) void k_kick(int n, const int *__restrict__ perm,
                                                  const float4 *__restrict__ acc,
-                                                 float4 *__restrict__ vel, float dth) {
+                                                 float4 *__restrict__ vel, float dth,
+                                                 DevCounters *dc, int closing) {
+  if (dc->err_sticky) return;
+  if (closing && blockIdx.x == 0 && threadIdx.x == 0) dc->steps_done++;
   for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
     const int t = perm[k];
     const float4 a = acc[k];
@@ -42,7 +47,9 @@ __global__ __launch_bounds__(kBlock) void k_kick(int n, const int *__restrict__ 
 }
 
 __global__ __launch_bounds__(kBlock) void k_drift(int n, const float4 *__restrict__ vel,
-                                                  float4 *__restrict__ pos, float dt) {
+                                                  float4 *__restrict__ pos, float dt,
+                                                  const DevCounters *dc) {
+  if (dc->err_sticky) return;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const float4 v = vel[i];
     float4 p = pos[i];
@@ -50,6 +57,14 @@ __global__ __launch_bounds__(kBlock) void k_drift(int n, const float4 *__restric
     p.y += v.y * dt;
     p.z += v.z * dt;
     pos[i] = p;
+  }
+}
+
+__global__ void k_clear_sticky(DevCounters *dc) {
+  if (threadIdx.x == 0) {
+    dc->err_sticky = 0;
+    dc->stack_hw = 0;
+    dc->steps_done = 0;
   }
 }
 
@@ -181,14 +196,18 @@ __global__ __launch_bounds__(kBlock) void k_direct_reduce(int ns, int nslices,
 
 }  // namespace
 
-void launch_kick(Ctx &c, float dth) {
+void launch_kick(Ctx &c, float dth, bool closing) {
   hipLaunchKernelGGL(k_kick, dim3(grid_for(c.n, kBlock, 8192)), dim3(kBlock), 0, c.stream, c.n,
-                     c.perm, c.acc, c.vel, dth);
+                     c.perm, c.acc, c.vel, dth, c.dc, closing ? 1 : 0);
 }
 
 void launch_drift(Ctx &c, float dt) {
   hipLaunchKernelGGL(k_drift, dim3(grid_for(c.n, kBlock, 8192)), dim3(kBlock), 0, c.stream, c.n,
-                     c.vel, c.pos, dt);
+                     c.vel, c.pos, dt, c.dc);
+}
+
+void launch_clear_sticky(Ctx &c) {
+  hipLaunchKernelGGL(k_clear_sticky, dim3(1), dim3(64), 0, c.stream, c.dc);
 }
 
 void launch_energies(Ctx &c) {

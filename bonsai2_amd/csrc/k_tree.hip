@@ -263,6 +263,7 @@ __global__ __launch_bounds__(kBlock) void k_level_scan(LevelArgs a) {
     int total = run;
     if (start + total > a.cap_cells) {
       dc->err |= 1;
+      dc->err_sticky |= 1;
       total = 0;
     }
     dc->lvl_start[a.L] = start;

@@ -564,6 +564,7 @@ void k_walk(WalkArgs a) {
     }
     if (overflow) {
       atomicOr(&a.dc->err, 4);  // every lane, same bit
+      atomicOr(&a.dc->err_sticky, 4);
       na = 0;
       nd = 0;
     }
@@ -581,6 +582,7 @@ void k_walk(WalkArgs a) {
     a.gcounts[g] = overflow ? make_int2(-1, -1) : make_int2(n_app, n_dir);
     a.gpot[g] = gp;
     atomicMax(&a.dc->stack_max, lane == 0 ? deepest : 0);
+    atomicMax(&a.dc->stack_hw, lane == 0 ? deepest : 0);
   }
 }
 

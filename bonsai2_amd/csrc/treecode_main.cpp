@@ -110,7 +110,7 @@ int main(int argc, char **argv) {
     tree.computeMultipoles();
     tree.makeGroups(5, ngroup);
     const double t1 = now();
-    const bonsai2::double4_t st = tree.computeForces();
+    const double4 st = tree.computeForces();
     const double dtw = now() - t1;
     std::fprintf(stderr,
                  " direct: <n>= %g  max= %g  approx: <n>=%g max= %g :: perf= %g TFLOP/s \n",
@@ -136,7 +136,7 @@ int main(int argc, char **argv) {
                  n / 1e6 / (now() - t0));
     std::fprintf(stderr, " nLeaf= %d  nCrit= %d\n", tree.get_nLeaf(), tree.get_nCrit());
     if (!outfile.empty()) {
-      tree.ptcl_d2h();
+      tree.ptcl_d2h_input();
       if (bt_write_tipsy(outfile.c_str(), (int)n, &tree.h_ptclPos[0].data[0],
                          &tree.h_ptclVel[0].data[0], dt * steps, eps) != 0)
         return 5;

@@ -4,28 +4,51 @@
 // (code/Treecode.h:123-223, instantiated for float only,
 // code/TreecodeInstances.h:3) on top of the C ABI in bonsai2_amd.h, so a
 // caller written against the reference keeps its call sequence
-// (code/treecode.cu:61-165):
+// (code/treecode.cu:61-165) unchanged:
 //
-//   Treecode<float> tree(eps, theta);
+//   typedef Treecode<float> Tree;
+//   Tree tree(eps, theta);
 //   tree.alloc(n);
-//   ... fill tree.h_ptclPos[i] / tree.h_ptclVel[i] ...
+//   for (int i = 0; i < n; i++) {
+//     typename Tree::Particle pos, vel;
+//     pos.x() = ...; pos.mass() = ...;
+//     tree.h_ptclPos[i] = pos; tree.h_ptclVel[i] = vel;
+//   }
 //   tree.ptcl_h2d();
 //   tree.buildTree(nLeaf);
 //   tree.computeMultipoles();
 //   tree.makeGroups(5, nGroup);
-//   double4 interactions = tree.computeForces();
+//   const double4 interactions = tree.computeForces();
 //   tree.moveParticles();
 //   tree.computeEnergies();
 //
+// The vector types (double4, float4, int2, uint4) are HIP's, from
+// <hip/hip_vector_types.h>; a host compiler gets them with the flags
+// `hipconfig --cpp_config` prints.  Link libbonsai2_amd.so.
+//
+// Public members as in the reference: theta, eps2, grav_potential; the pinned
+// host arrays h_ptclPos / h_ptclVel (host_mem, code/cudamem.h:64-96) and
+// ptcl0; the device arrays d_ptclPos, d_ptclVel, d_ptclPos_tmp, d_ptclAcc,
+// d_domain, d_level_begIdx, d_cellDataList, d_cellSize, d_cellMonopole,
+// d_cellQuad0, d_cellQuad1, d_groupList as views that read the context's
+// current address (operator T*, d(), size(), d2h()).  ptcl_d2h() copies
+// d_ptclPos / d_ptclVel as they are: tree order after buildTree, as in the
+// reference (code/Treecode.h:200-204); ptcl_d2h_input() returns input order.
+//
 // Differences, all additive: errors throw bonsai2::Error instead of exit()
-// (code/cutil.h:772-780); moveParticles() integrates with the member dt
-// (the reference's is a stub, code/moveParticles.cu:3-7); computeEnergies()
-// fills ekin/epot (a stub in the reference, code/computeEnergies.cu:3-7);
-// d_ptclAcc is read with get_acc().  Host only: include with any C++11
-// compiler and link libbonsai2_amd.so.
+// (code/cutil.h:772-780); moveParticles() is the first half of a
+// kick-drift-kick step with the member dt, closed by the next force
+// evaluation (the reference's is a stub, code/moveParticles.cu:3-7);
+// computeEnergies() fills ekin / epot in FP64, evaluating the forces first
+// when the particles moved (a stub in the reference,
+// code/computeEnergies.cu:3-7).
 #pragma once
 
+#include <hip/hip_vector_types.h>
+
 #include <cmath>
+#include <cstddef>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -53,8 +76,64 @@ struct Particle4 {
   T mass() const { return data[3]; }
 };
 
-struct double4_t {
-  double x, y, z, w;
+// Pinned host array, host_mem<T> of code/cudamem.h:64-96.
+template <typename T>
+struct host_mem {
+  size_t n = 0;
+  T *ptr = nullptr;
+  host_mem() {}
+  ~host_mem() { free(); }
+  host_mem(const host_mem &) = delete;
+  host_mem &operator=(const host_mem &) = delete;
+  void alloc(size_t size) {
+    free();
+    void *p = nullptr;
+    if (bt_host_alloc(&p, size * sizeof(T)) != 0) throw Error(BT_ERR_HIP, "bt_host_alloc failed");
+    ptr = static_cast<T *>(p);
+    n = size;
+    for (size_t i = 0; i < n; i++) new (ptr + i) T();
+  }
+  void free() {
+    if (ptr) bt_host_free(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  void realloc(size_t size) {
+    if (size > n) alloc(size);
+  }
+  operator T *() { return ptr; }
+  const T &operator[](size_t i) const { return ptr[i]; }
+  T &operator[](size_t i) { return ptr[i]; }
+};
+
+// A device array of the context (cuda_mem<T>, code/cudamem.h:5-61), read
+// through the C ABI at each use: buildTree moves the particle arrays.
+template <typename T>
+class device_view {
+  bt_ctx *const *ctx_;
+  int which_;
+
+  void *query(int64_t *count) const {
+    void *p = nullptr;
+    if (bt_get_device_buffer(*ctx_, which_, &p, count) != 0)
+      throw Error(BT_ERR_STATE, std::string("device buffer: ") + bt_last_error(*ctx_));
+    return p;
+  }
+
+ public:
+  device_view(bt_ctx *const *ctx, int which) : ctx_(ctx), which_(which) {}
+  T *d() const { return static_cast<T *>(query(nullptr)); }
+  operator T *() const { return d(); }
+  size_t size() const {
+    int64_t k = 0;
+    query(&k);
+    return (size_t)k;
+  }
+  void d2h(T *host, int count = -1) const {
+    const int64_t k = count < 0 ? (int64_t)size() : (int64_t)count;
+    const int rc = bt_copy_device_buffer(*ctx_, which_, host, k);
+    if (rc != 0) throw Error(rc, std::string("d2h: ") + bt_last_error(*ctx_));
+  }
 };
 
 }  // namespace bonsai2
@@ -67,25 +146,41 @@ struct Treecode<float> {
   typedef bonsai2::Particle4<float> Particle;
 
   float theta, eps2;
-  float dt = 1.0f / 64;              // step for the no-argument moveParticles()
+  float dt = 1.0f / 64;              // step of the no-argument moveParticles()
   double grav_potential = 0.0;       // code/Treecode.h:162
   double ekin = 0.0, epot = 0.0;     // set by computeEnergies()
-  std::vector<Particle> h_ptclPos, h_ptclVel;  // host particle arrays (code/Treecode.h:142)
 
  private:
   bt_ctx *ctx_ = nullptr;
-  int nPtcl = 0, nCrit = 64, nLeaf = 16;
-  bool moved_ = false;
+  int nPtcl = 0, nCrit = 64, nLeaf = 16, levelSplit = 1;
 
   void check(int rc, const char *what) const {
     if (rc != 0) throw bonsai2::Error(rc, std::string(what) + ": " + bt_last_error(ctx_));
   }
+  void sync_params() {
+    check(bt_set_params(ctx_, eps2 > 0.f ? std::sqrt(eps2) : 0.f, theta), "set_params");
+  }
 
  public:
+  // code/Treecode.h:142-160
+  bonsai2::host_mem<Particle> h_ptclPos, h_ptclVel;
+  std::vector<Particle> ptcl0;
+  bonsai2::device_view<Particle> d_ptclPos{&ctx_, BT_BUF_PTCL_POS};
+  bonsai2::device_view<Particle> d_ptclVel{&ctx_, BT_BUF_PTCL_VEL};
+  bonsai2::device_view<Particle> d_ptclPos_tmp{&ctx_, BT_BUF_PTCL_POS_TMP};
+  bonsai2::device_view<Particle> d_ptclAcc{&ctx_, BT_BUF_PTCL_ACC};
+  bonsai2::device_view<float4> d_domain{&ctx_, BT_BUF_DOMAIN};
+  bonsai2::device_view<int2> d_level_begIdx{&ctx_, BT_BUF_LEVEL_BEGIDX};
+  bonsai2::device_view<uint4> d_cellDataList{&ctx_, BT_BUF_CELL_DATA};
+  bonsai2::device_view<float4> d_cellSize{&ctx_, BT_BUF_CELL_SIZE};
+  bonsai2::device_view<float4> d_cellMonopole{&ctx_, BT_BUF_CELL_MONOPOLE};
+  bonsai2::device_view<float4> d_cellQuad0{&ctx_, BT_BUF_CELL_QUAD0};
+  bonsai2::device_view<float2> d_cellQuad1{&ctx_, BT_BUF_CELL_QUAD1};
+  bonsai2::device_view<int2> d_groupList{&ctx_, BT_BUF_GROUP_LIST};
+
   int get_nPtcl() const { return nPtcl; }
   int get_nCrit() const { return nCrit; }
   int get_nLeaf() const { return nLeaf; }
-  bt_ctx *context() { return ctx_; }
 
   // code/Treecode.h:164-173
   Treecode(const float _eps = 0.01f, const float _theta = 0.75f, const int _ncrit = 64)
@@ -100,65 +195,68 @@ struct Treecode<float> {
   // code/Treecode.h:175-198
   void alloc(const int n) {
     nPtcl = n;
-    h_ptclPos.assign(n, Particle());
-    h_ptclVel.assign(n, Particle());
+    h_ptclPos.alloc(n);
+    h_ptclVel.alloc(n);
     check(bt_alloc(ctx_, n, 1, nullptr), "alloc");
   }
 
-  // code/Treecode.h:207-214
+  // code/Treecode.h:207-214 (also keeps the initial positions in ptcl0)
   void ptcl_h2d() {
-    float eps = 0.f;
-    if (eps2 > 0.f) eps = std::sqrt(eps2);
-    check(bt_set_params(ctx_, eps, theta), "set_params");
+    sync_params();
     check(bt_set_particles(ctx_, &h_ptclPos[0].data[0], &h_ptclVel[0].data[0]), "ptcl_h2d");
+    ptcl0.assign(h_ptclPos.ptr, h_ptclPos.ptr + nPtcl);
   }
 
-  // code/Treecode.h:200-204 (particles come back in input order)
+  // code/Treecode.h:200-204: d_ptclPos / d_ptclVel as they are (tree order
+  // after buildTree, input order before the first build)
   void ptcl_d2h() {
-    check(bt_get_particles(ctx_, &h_ptclPos[0].data[0], &h_ptclVel[0].data[0],
-                           BT_ORDER_INPUT),
+    check(bt_get_particles(ctx_, &h_ptclPos[0].data[0], &h_ptclVel[0].data[0], BT_ORDER_TREE),
           "ptcl_d2h");
+  }
+  // the same particles in the order ptcl_h2d gave them
+  void ptcl_d2h_input() {
+    check(bt_get_particles(ctx_, &h_ptclPos[0].data[0], &h_ptclVel[0].data[0], BT_ORDER_INPUT),
+          "ptcl_d2h_input");
   }
 
   void buildTree(const int _nLeaf = 16) {  // code/buildTree.cu:845
     nLeaf = _nLeaf;
-    if (moved_) ptcl_sync_params();
+    sync_params();   // theta and eps2 are public members the caller may change
     check(bt_build_tree(ctx_, _nLeaf), "buildTree");
   }
-  void computeMultipoles() { check(bt_compute_multipoles(ctx_), "computeMultipoles"); }
-  void makeGroups(int levelSplit = 1, const int _nCrit = 64) {  // code/makeGroups.cu:236
+  void computeMultipoles() {  // code/computeMultipoles.cu:194
+    sync_params();
+    check(bt_compute_multipoles(ctx_), "computeMultipoles");
+  }
+  void makeGroups(int _levelSplit = 1, const int _nCrit = 64) {  // code/makeGroups.cu:236
     nCrit = _nCrit;
-    check(bt_make_groups(ctx_, levelSplit, _nCrit), "makeGroups");
+    levelSplit = _levelSplit;
+    check(bt_make_groups(ctx_, _levelSplit, _nCrit), "makeGroups");
   }
   // code/computeForces.cu:622-727: {direct <n>, direct max, approx <n>, approx max}
-  bonsai2::double4_t computeForces(const bool INTCOUNT = true) {
+  double4 computeForces(const bool INTCOUNT = true) {
     double st[4] = {0, 0, 0, 0};
     check(bt_compute_forces(ctx_, INTCOUNT ? 1 : 0, st), "computeForces");
     check(bt_get_potential(ctx_, &grav_potential), "grav_potential");
-    return bonsai2::double4_t{st[0], st[1], st[2], st[3]};
+    return make_double4(st[0], st[1], st[2], st[3]);
   }
-  // half kick with the current forces, then drift (v += a dt/2, x += v dt)
-  void moveParticles() {
-    check(bt_move_particles(ctx_, dt), "moveParticles");
-    moved_ = true;
+  // first half of a kick-drift-kick step with dt; the next force evaluation
+  // applies the second half kick
+  void moveParticles() { check(bt_move_particles(ctx_, dt), "moveParticles"); }
+  void computeEnergies() {
+    check(bt_compute_energies(ctx_, &ekin, &epot), "computeEnergies");
+    check(bt_get_potential(ctx_, &grav_potential), "grav_potential");
   }
-  void computeEnergies() { check(bt_compute_energies(ctx_, &ekin, &epot), "computeEnergies"); }
 
   // The C-ABI context, for the calls of bonsai2_amd.h that extend the
   // reference (learned opening angle, leaf groups, captured step, autotuner).
   bt_ctx *handle() { return ctx_; }
+  bt_ctx *context() { return ctx_; }
 
-  // (ax, ay, az, phi) per particle (reference d_ptclAcc), in input order by default
-  std::vector<bonsai2::Particle4<float>> get_acc(int order = BT_ORDER_INPUT) {
-    std::vector<bonsai2::Particle4<float>> a(nPtcl);
+  // (ax, ay, az, phi) per particle (d_ptclAcc), in input order by default
+  std::vector<Particle> get_acc(int order = BT_ORDER_INPUT) {
+    std::vector<Particle> a(nPtcl);
     check(bt_get_acc(ctx_, &a[0].data[0], order), "get_acc");
     return a;
-  }
-
- private:
-  void ptcl_sync_params() {
-    float eps = 0.f;
-    if (eps2 > 0.f) eps = std::sqrt(eps2);
-    check(bt_set_params(ctx_, eps, theta), "set_params");
   }
 };

@@ -26,6 +26,7 @@
 #ifndef BONSAI2_AMD_H
 #define BONSAI2_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -90,16 +91,26 @@ int bt_make_groups(bt_ctx *ctx, int level_split, int ncrit);
  * (:701-715); grav_potential (:711) is returned by bt_get_potential(). */
 int bt_compute_forces(bt_ctx *ctx, int intcount, double stats[4]);
 /* moveParticles(), code/moveParticles.cu:3-7 (a stub in the reference):
- * half kick with the current accelerations, then drift: v += a*dt/2,
- * x += v*dt. */
+ * the first half of a kick-drift-kick leapfrog step, v += a*dt/2 with the
+ * current accelerations, then x += v*dt.  The step's second half kick is
+ * owed to the next force evaluation: the next bt_compute_forces (or the
+ * force evaluation bt_compute_energies makes) applies v += a_new*dt/2, so
+ * the reference's loop buildTree .. computeForces, moveParticles integrates
+ * a KDK leapfrog. */
 int bt_move_particles(bt_ctx *ctx, float dt);
 /* Half kick only (closes a kick-drift-kick step after new forces). */
 int bt_kick(bt_ctx *ctx, float dt_half);
 /* computeEnergies(), code/computeEnergies.cu:3-7 (a stub in the reference):
  * FP64 reductions K = 1/2 sum m v^2 and W = 1/2 sum m (phi + m/eps), per
- * system (arrays of nsys) when ekin/epot are not null. */
+ * system (arrays of nsys) when ekin/epot are not null.  When the forces are
+ * not current (after bt_move_particles, or none yet), it first evaluates them
+ * (buildTree, computeMultipoles, makeGroups, computeForces with the last
+ * NLEAF, levelSplit and NCRIT), which also applies a pending half kick. */
 int bt_compute_energies(bt_ctx *ctx, double *ekin, double *epot);
-/* One full kick-drift-kick step: forces are (re)computed as needed. */
+/* One full kick-drift-kick step: forces are (re)computed as needed.  If the
+ * cell arrays overflow, they are grown and the step's second half is run
+ * again.  On BT_ERR_STACK the particles hold the step's first half kick and
+ * drift (the closing kick is not applied with invalid forces). */
 int bt_step(bt_ctx *ctx, float dt, int nleaf, int level_split, int ncrit);
 
 /* ---- results ----------------------------------------------------------- */
@@ -149,7 +160,8 @@ int bt_direct_fp64(bt_ctx *ctx, int ns, const int *ph_idx, double *out4);
  * ph sort, groups, walk, integrate, energies}. */
 int bt_get_timings(bt_ctx *ctx, float ms[10]);
 /* Walk options: kahan = compensated accumulators (default 1);
- * stack_ints = per-wave traversal stack capacity (default 16384);
+ * stack_ints = per-wave traversal stack capacity (default 16384, at least 64;
+ *   a walk that needs more returns BT_ERR_STACK);
  * blocks_per_cu = persistent grid size (default 0 = automatic). */
 int bt_set_walk_options(bt_ctx *ctx, int kahan, int stack_ints, int blocks_per_cu);
 /* How the walk feeds interaction data to the lanes: 0 (default) = the wave
@@ -173,8 +185,9 @@ int bt_set_walk_lds_stack(bt_ctx *ctx, int entries);
  * leaf of more than 64 particles (a leaf capped at the deepest level) returns
  * BT_ERR_CAPACITY.  Takes effect at the next makeGroups. */
 int bt_set_group_mode(bt_ctx *ctx, int mode);
-/* Deepest traversal stack of any group in the last walk (entries, LDS and
- * global parts together): the stack_ints a walk of this tree needs. */
+/* Deepest traversal stack of any group in the walks of the last call that
+ * walked (entries, LDS and global parts together; all replays of a
+ * bt_step_graph): the stack_ints a walk of this tree needs. */
 int bt_get_walk_stack_max(bt_ctx *ctx, int *entries);
 /* code/Treecode.h:14-16 (#define QUADRUPOLE): 1 (default) = monopole +
  * quadrupole P2C (code/computeForces.cu:83-128, 64 flops); 0 = monopole-only
@@ -191,8 +204,11 @@ int bt_set_quadrupole(bt_ctx *ctx, int on);
  * band, the check's settings), and dropped when arrays are reallocated.
  * Needs forces (or one bt_step) first.  With the learned angle the per-step
  * FP64 check and its fixed-angle fallback are part of the graph (the device
- * decides).  A capacity overflow returns BT_ERR_CAPACITY: bt_step then
- * regrows the arrays. */
+ * decides).  Errors of every replay are kept (a sticky device word that only
+ * the host clears); after one, later replays leave the particles alone.  A
+ * cell-capacity overflow grows the cells, finishes the failed step and runs
+ * the remaining steps eagerly; a walk-stack overflow returns BT_ERR_STACK
+ * with the particles as after the failed step's first half kick and drift. */
 int bt_step_graph(bt_ctx *ctx, float dt, int nleaf, int level_split, int ncrit, int nsteps);
 /* Picks NLEAF, NCRIT (3 x 3 candidates, timed zero-dt steps), the persistent
  * walk grid (4, 8, 12, 16 blocks per CU) and the traversal stack depth
@@ -250,6 +266,38 @@ int bt_set_opening_check(bt_ctx *ctx, int nsamples, float p99_max, uint64_t seed
 /* {p50, p99, max of the last check, last call fell back (0/1), fallbacks,
  * checks} */
 int bt_get_opening_check(bt_ctx *ctx, double out[6]);
+
+/* ---- memory of the reference's public members ---------------------------- */
+
+/* Pinned host memory, the reference's host_mem<T> (code/cudamem.h:64-96,
+ * cudaMallocHost): hipHostMalloc / hipHostFree.  Needs no context. */
+int bt_host_alloc(void **ptr, size_t bytes);
+int bt_host_free(void *ptr);
+
+/* The device arrays behind the reference's public cuda_mem members
+ * (code/Treecode.h:142-160): current address and element count.  Particle
+ * arrays change address at every buildTree (the reference swaps its pointers
+ * too, code/buildTree.cu:929), so a pointer is valid until the next build. */
+enum {
+  BT_BUF_PTCL_POS = 0,     /* d_ptclPos: float4 (x,y,z,m), tree order after buildTree */
+  BT_BUF_PTCL_VEL = 1,     /* d_ptclVel: float4 (vx,vy,vz,m), same order */
+  BT_BUF_PTCL_POS_TMP = 2, /* d_ptclPos_tmp: float4, PH order after makeGroups */
+  BT_BUF_PTCL_ACC = 3,     /* d_ptclAcc: float4 (ax,ay,az,phi), PH order */
+  BT_BUF_DOMAIN = 4,       /* d_domain: float4 {cx, cy, cz, hsize} per system */
+  BT_BUF_LEVEL_BEGIDX = 5, /* d_level_begIdx: int2[32] */
+  BT_BUF_CELL_DATA = 6,    /* d_cellDataList: CellData uint4 */
+  BT_BUF_CELL_SIZE = 7,    /* d_cellSize: float4 (com, cellOp^2) */
+  BT_BUF_CELL_MONOPOLE = 8,/* d_cellMonopole: float4 (com, M) */
+  BT_BUF_CELL_QUAD0 = 9,   /* d_cellQuad0: float4 (xx, yy, zz, xy) */
+  BT_BUF_CELL_QUAD1 = 10,  /* d_cellQuad1: float2 (xz, yz) */
+  BT_BUF_GROUP_LIST = 11,  /* d_groupList: int2 (pbeg, np) */
+  BT_BUF_PH_KEY = 12,      /* sorted PH keys, uint64 (makeGroups' d_key) */
+  BT_BUF_PH_PERM = 13      /* PH index -> tree index, int (makeGroups' d_value) */
+};
+int bt_get_device_buffer(bt_ctx *ctx, int which, void **ptr, int64_t *count);
+/* Copies the first count elements of one of those arrays to host memory
+ * (cuda_mem<T>::d2h, code/cudamem.h:40-46); synchronises the stream. */
+int bt_copy_device_buffer(bt_ctx *ctx, int which, void *host, int64_t count);
 
 /* ---- initial conditions and I/O (host only, no device needed) ---------- */
 

@@ -16,7 +16,11 @@
 //                        hand-derived golden keys
 //   or_make_groups       closed-form chopping of sorted keys (tests)
 //   or_walk              theta->0 equals direct sum; two-body and quadrupole
-//                        closed forms; FP64 direct sum error bound
+//                        closed forms; FP64 direct sum error bound; its lists
+//                        (or_walk_lists) against an FP64 statement of the
+//                        opening rule: accepted cells are far enough by the
+//                        clamp distance, hold >= 3 particles, have an opened
+//                        parent, and with the opened leaves tile [0, N)
 //   or_direct_fp64       long-double brute force on tiny inputs
 //   or_plummer_ref       bit-exact against the reference's own plummer.h
 //                        compiled into oracle/_ref (when that builds)
@@ -987,6 +991,58 @@ void or_mlp_grad(int n, const float *params, const float *x, const float *t, int
     }
   }
   *loss = L / n;
+}
+
+// The interaction lists of one group (code/computeForces.cu:229-463, the
+// same traversal and opening rule as walk_impl): the cells it evaluates by
+// P2C (alist, breadth-first) and the leaves whose particles it evaluates by
+// P2P (leaves), with their counts in *na / *nl.  The lists are checked
+// against an FP64 statement of the opening rule in the tests.  mutate plants
+// a known mistake, so that the tests can show their check catches it:
+//   0 none; 1 no "< 3 particles" rule (:284-285); 2 the group is its centre
+//   point, not its box (no box half-sizes in ds, :30-54); 3 no s term in the
+//   opening radius (op = l / theta, needs cell_ls and inv_theta).
+void or_walk_lists(int g, const I2 *groups, const int *level_range, const F4 *pos_ph,
+                   const U4 *cells, const F4 *csize, const F2 *cell_ls, float inv_theta,
+                   int mutate, int *alist, int *na, int *leaves, int *nl) {
+  const int pbeg = groups[g].x, np = groups[g].y;
+  float mn[3] = {pos_ph[pbeg].x, pos_ph[pbeg].y, pos_ph[pbeg].z};
+  float mx[3] = {mn[0], mn[1], mn[2]};
+  for (int i = pbeg; i < pbeg + np; i++) {
+    mn[0] = std::min(mn[0], pos_ph[i].x); mx[0] = std::max(mx[0], pos_ph[i].x);
+    mn[1] = std::min(mn[1], pos_ph[i].y); mx[1] = std::max(mx[1], pos_ph[i].y);
+    mn[2] = std::min(mn[2], pos_ph[i].z); mx[2] = std::max(mx[2], pos_ph[i].z);
+  }
+  const float gcx = 0.5f * (mx[0] + mn[0]), gcy = 0.5f * (mx[1] + mn[1]),
+              gcz = 0.5f * (mx[2] + mn[2]);
+  float gsx = 0.5f * (mx[0] - mn[0]), gsy = 0.5f * (mx[1] - mn[1]), gsz = 0.5f * (mx[2] - mn[2]);
+  if (mutate == 2) gsx = gsy = gsz = 0.f;
+  std::vector<int> cur, nxt;
+  for (int c = level_range[2]; c <= level_range[3]; c++) cur.push_back(c);
+  int ka = 0, kl = 0;
+  while (!cur.empty()) {
+    nxt.clear();
+    for (int c : cur) {
+      const U4 cd = cells[c];
+      F4 cs = csize[c];
+      if (mutate == 3) {
+        const float op = cell_ls[c].x * inv_theta;
+        cs.w = op * op;
+      }
+      bool split = open_test(cs, gcx, gcy, gcz, gsx, gsy, gsz, nullptr, 0.f);
+      if (mutate != 1) split = split || ((int)cd.w - (int)cd.z < 3);
+      if (!split) {
+        alist[ka++] = c;
+      } else if (!cell_is_leaf(cd)) {
+        for (int k = 0; k < cell_n(cd); k++) nxt.push_back(cell_first(cd) + k);
+      } else {
+        leaves[kl++] = c;
+      }
+    }
+    cur.swap(nxt);
+  }
+  *na = ka;
+  *nl = kl;
 }
 
 // The walk with the learned per-pair opening angle (FP32 accumulation).

@@ -44,6 +44,8 @@ def lib():
         L.or_make_groups.restype = ctypes.c_int
         L.or_walk.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, P, P, P, ctypes.c_float,
                               ctypes.c_int, P, P, P, P, P, P, ctypes.c_int]
+        L.or_walk_lists.argtypes = [ctypes.c_int, P, P, P, P, P, P, ctypes.c_float,
+                                    ctypes.c_int, P, P, P, P]
         L.or_direct_fp64.argtypes = [ctypes.c_int, P, ctypes.c_int, P, ctypes.c_double, P]
         L.or_direct_ld.argtypes = [ctypes.c_int, P, ctypes.c_int, P, ctypes.c_double, P]
         L.or_energies.argtypes = [ctypes.c_int, P, P, P, ctypes.c_double, P, P]
@@ -127,7 +129,7 @@ def build_tree(pos, nleaf, box=None, offsets=None, boxes=None):
     offsets = np.ascontiguousarray(offsets, np.int32)
     boxes = np.ascontiguousarray(boxes, np.float32)
     nroot = len(offsets) - 1
-    ncell_max = max(16, n + 8 * nroot)
+    ncell_max = 3 * n + 64 * nroot + 1024   # the largest tree (levels add <= n cells each)
     order = np.zeros(n, np.int32)
     cells = np.zeros((ncell_max, 4), np.uint32)
     level_range = np.zeros(64, np.int32)
@@ -255,6 +257,27 @@ def walk(groups, level_range, pos_ph, pos_tree, cells, size, mono, q0, q1, eps,
     if counts:
         return acc, stats, float(pot[0]), cnt
     return acc, stats, float(pot[0])
+
+
+def walk_lists(g, groups, level_range, pos_ph, cells, size, ls=None, theta=None, mutate=0):
+    """Interaction lists of group g (code/computeForces.cu:229-463) ->
+    (P2C cells, leaves opened for P2P).  mutate plants a known mistake in the
+    opening rule (1: no < 3 particles rule, 2: group point instead of box,
+    3: no s term); used only to show the tests catch it."""
+    cells = np.ascontiguousarray(cells, np.uint32)
+    nc = len(cells)
+    alist = np.zeros(nc, np.int32)
+    leaves = np.zeros(nc, np.int32)
+    na = np.zeros(1, np.int32)
+    nl = np.zeros(1, np.int32)
+    it = np.float32(1.0 / float(np.float32(theta))) if theta is not None else np.float32(0)
+    lib().or_walk_lists(int(g), _ptr(np.ascontiguousarray(groups, np.int32)),
+                        _ptr(np.ascontiguousarray(level_range, np.int32).reshape(-1)),
+                        _ptr(_f4(pos_ph)), _ptr(cells), _ptr(np.ascontiguousarray(size, np.float32)),
+                        _ptr(None if ls is None else np.ascontiguousarray(ls, np.float32)),
+                        ctypes.c_float(it), int(mutate), _ptr(alist), _ptr(na), _ptr(leaves),
+                        _ptr(nl))
+    return alist[:na[0]].copy(), leaves[:nl[0]].copy()
 
 
 def direct_fp64(pos, idx, eps, longdouble=False):

@@ -145,3 +145,29 @@ def test_tipsy_errors(bt, tmp_path):
         fh.write(data[:-10])
     with pytest.raises(bt.BonsaiError):
         bt.read_tipsy(g)
+
+
+def test_reference_style_caller_compiles_against_treecode_header(tmp_path):
+    """tests/cpp/ref_caller.cpp follows code/treecode.cu:61-165 (double4
+    statistics, Tree::Particle, h_ptclPos/h_ptclVel, d_ptclAcc); it compiles
+    with a plain C++11 host compiler and the flags `hipconfig --cpp_config`
+    prints, and links against the library."""
+    import subprocess
+    from bonsai2_amd import build as bbuild
+    exe = str(tmp_path / "ref_caller")
+    cmd = (["g++", "-O1", "-std=c++11", "-Wall", "-Werror"] + bbuild.HIP_HOST_FLAGS +
+           ["-I" + os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cpp",
+                                                               "ref_caller.cpp"),
+            "-o", exe, "-L" + os.path.join(ROOT, "bonsai2_amd"), "-lbonsai2_amd"])
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout
+    # the header's double4 is HIP's vector type
+    src = '#include "Treecode.h"\nstatic_assert(sizeof(double4) == 32, "");\n' \
+          'double4 f(Treecode<float> &t) { return t.computeForces(); }\n'
+    p = tmp_path / "t.cpp"
+    p.write_text(src)
+    r = subprocess.run(["g++", "-std=c++11", "-fsyntax-only"] + bbuild.HIP_HOST_FLAGS +
+                       ["-I" + os.path.join(ROOT, "include"), str(p)],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout

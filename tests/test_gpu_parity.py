@@ -2,10 +2,12 @@
 
 Bars: bit-exact for the root box, geometric tree, PH keys, permutation and
 groups (integer/index work); multipoles within FP32 rounding of the oracle's
-FP64 per-cell sums; forces within 1e-5 relative RMS of the oracle walk with
-the same theta/NLEAF/NCRIT, interaction counts equal
-(to 1e-4 relative: an opening decision on the exact float boundary may flip
-when the GPU's moments differ from the oracle's in the last bit).
+FP64 per-cell sums; the walk's integer work -- every group's (n_approx,
+n_direct) and the statistics -- bit-exact against the oracle's walk over the
+GPU's own tree and moments; forces within 1e-5 relative RMS of the oracle
+walk with the same theta/NLEAF/NCRIT.  (Against the oracle's own FP64
+moments the counts agree to 1e-4: an opening decision exactly on the float
+boundary may flip where the moments differ in the last bit.)
 """
 import numpy as np
 import pytest
@@ -121,7 +123,36 @@ def test_forces_match_reference_walk(run):
     assert rms_phi <= 1e-5
 
 
-def test_interaction_counts(run):
+def gpu_tree_walk(oracle, t, eps, monopole=False, groups=None):
+    """The oracle's walk over the GPU's own tree, moments, particles and
+    groups -> (acc, stats, pot, per-group counts)."""
+    tr = t.tree()
+    pos_tree = t.ptcl_d2h(1)[0]
+    pos_ph = t.ptcl_d2h(2)[0] if groups is None else pos_tree
+    g = t.groups() if groups is None else groups
+    return oracle.walk(g, tr["level_range"], pos_ph, pos_tree, tr["cells"], tr["size"],
+                       tr["mono"], tr["q0"], tr["q1"], eps, counts=True, monopole=monopole)
+
+
+def assert_counts_exact(t, st, st_o, cnt_o, n):
+    assert np.array_equal(t.group_counts(), cnt_o)
+    assert tuple(st) == (st_o[0] / n, float(st_o[1]), st_o[2] / n, float(st_o[3]))
+    _, tot = t.walk_stats()
+    assert (tot[0], tot[1]) == (float(st_o[0]), float(st_o[2]))
+
+
+def test_interaction_counts_exact_on_gpu_tree(run, oracle):
+    """Per-group interaction counts and the statistics
+    (code/computeForces.cu:567-609, :701-715) equal the oracle's walk over
+    the same tree and moments exactly; forces within 1e-5."""
+    t = run["t"]
+    acc_o, st_o, pot_o, cnt_o = gpu_tree_walk(oracle, t, run["eps"])
+    assert_counts_exact(t, run["stats"], st_o, cnt_o, run["n"])
+    assert rel_rms(t.acc(2)[:, :3], acc_o[:, :3]) <= 1e-5
+    assert abs(t.grav_potential - pot_o) <= 1e-5 * abs(pot_o)
+
+
+def test_interaction_counts_oracle_moments(run):
     stats, ref = run["stats"], run["ref"]["stats"]
     n = run["n"]
     exp = (ref[0] / n, ref[1], ref[2] / n, ref[3])
@@ -151,22 +182,21 @@ def test_scalar_load_walk_matches(run):
 def test_monopole_only_walk_matches(run, oracle):
     """bt_set_quadrupole(0): the reference built without QUADRUPOLE
     (code/Treecode.h:14-16, code/computeForces.cu:198-225) against the oracle's
-    monopole walk on the same tree: counts equal, forces within 1e-5."""
+    monopole walk on the GPU's tree and moments: per-group counts equal,
+    forces within 1e-5."""
     t, ref = run["t"], run["ref"]
     t.set_quadrupole(False)
     try:
         st = t.computeForces()
         acc = t.acc(2)
+        cnt = t.group_counts()
     finally:
         t.set_quadrupole(True)
         t.computeForces()
-    acc_o, st_o, _ = oracle.walk(ref["groups"], ref["tree"].level_range, ref["pos_ph"],
-                                 ref["pos_tree"], ref["tree"].cells, ref["size"], ref["mono"],
-                                 ref["q0"], ref["q1"], run["eps"], monopole=True)
+    acc_o, st_o, _, cnt_o = gpu_tree_walk(oracle, t, run["eps"], monopole=True)
+    assert np.array_equal(cnt, cnt_o)
     n = run["n"]
-    exp = (st_o[0] / n, st_o[1], st_o[2] / n, st_o[3])
-    for a, b in zip(st, exp):
-        assert abs(a - b) <= 1e-4 * max(1.0, abs(b)), (st, exp)
+    assert tuple(st) == (st_o[0] / n, float(st_o[1]), st_o[2] / n, float(st_o[3]))
     assert rel_rms(acc[:, :3], acc_o[:, :3]) <= 1e-5
     # the quadrupole term is a real correction at this theta
     assert rel_rms(acc[:, :3], ref["acc"][:, :3]) > 1e-5
@@ -212,8 +242,8 @@ def test_leaf_groups_match_oracle(bt, oracle, nleaf, mode):
     """bt_set_group_mode(1): one group per leaf (the leaf list of
     code/buildTree.cu:954-958) in tree order; mode 2: one group per tree cell
     of at most NCRIT particles.  Groups equal the oracle's; the walk over them
-    matches the oracle's walk over the same groups (counts within 1e-4,
-    forces within 1e-5); input-order forces map back through the ids; a step
+    matches the oracle's walk over the same groups (counts exact on the GPU's
+    tree and moments, within 1e-4 on the oracle's; forces within 1e-5); input-order forces map back through the ids; a step
     runs; mode 0 restores the PH groups."""
     n, theta, eps = 30000, 0.5, 0.01
     pos, vel = bt.clustered(n, seed=31)
@@ -238,6 +268,10 @@ def test_leaf_groups_match_oracle(bt, oracle, nleaf, mode):
         for a, b in zip(st, exp):
             assert abs(a - b) <= 1e-4 * max(1.0, abs(b)), (st, exp)
         assert rel_rms(acc[:, :3], acc_o[:, :3]) <= 1e-5
+        # on the GPU's own tree and moments the counts are exact
+        acc_g, st_g, _, cnt_g = gpu_tree_walk(oracle, t, eps, groups=lg)
+        assert_counts_exact(t, st, st_g, cnt_g, n)
+        assert rel_rms(acc[:, :3], acc_g[:, :3]) <= 1e-5
         # the group order is the tree order: input order through the tree ids
         a_in = t.acc(bt.ORDER_INPUT)
         back = np.empty_like(acc_o)

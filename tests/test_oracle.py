@@ -381,3 +381,86 @@ def test_walk_stats_are_list_lengths(oracle, bt):
     assert st[0] == (cnt[:, 1] * npg).sum() and st[2] == (cnt[:, 0] * npg).sum()
     assert st[1] == cnt[:, 1].max() and st[3] == cnt[:, 0].max()
     assert np.isclose(pot, 0.5 * (r["pos_ph"][:, 3].astype(np.float64) * acc[:, 3]).sum())
+
+
+# ---- the opening rule, pinned independently of the oracle's own arithmetic ----
+
+def _cell_geometry_fp64(cells, pos_tree, theta):
+    """Per cell, from its particles in FP64: centre of mass, particle bbox,
+    opening radius op = l/theta + s with l = max(2 * max half extent, 1e-6)
+    and s = |bbox centre - com| (code/computeMultipoles.cu:168-177)."""
+    p = pos_tree[:, :3].astype(np.float64)
+    m = pos_tree[:, 3].astype(np.float64)
+    cm = np.concatenate([[0.0], np.cumsum(m)])
+    cx = np.concatenate([np.zeros((1, 3)), np.cumsum(p * m[:, None], axis=0)])
+    beg = cells[:, 2].astype(np.int64)
+    end = cells[:, 3].astype(np.int64)
+    M = cm[end] - cm[beg]
+    com = (cx[end] - cx[beg]) / M[:, None]
+    bmin = np.array([p[b:e].min(axis=0) for b, e in zip(beg, end)])
+    bmax = np.array([p[b:e].max(axis=0) for b, e in zip(beg, end)])
+    l = np.maximum(2.0 * (0.5 * (bmax - bmin)).max(axis=1), 1e-6)
+    s = np.linalg.norm(0.5 * (bmax + bmin) - com, axis=1)
+    return com, l / theta + s, l / theta, end - beg
+
+
+def _check_lists(alist, leaves, gmin, gmax, cells, com, op, npc, n, tol=1e-5):
+    """The rule as the walk must apply it, in FP64: ds = |com - clamp(com,
+    group box)| (code/computeForces.cu:30-54); a cell is accepted (P2C) iff
+    ds >= op and it holds >= 3 particles (:284-285); accepted cells are
+    maximal (their parent was opened); accepted cells and opened leaves tile
+    the particles exactly once.  Returns the list of violations."""
+    bad = []
+    def ds(c):
+        return np.linalg.norm(com[c] - np.clip(com[c], gmin, gmax), axis=-1)
+    a = np.asarray(alist, np.int64)
+    lf = np.asarray(leaves, np.int64)
+    if len(a):
+        if (npc[a] < 3).any():
+            bad.append("accepted cell with < 3 particles")
+        if (ds(a) < op[a] * (1 - tol)).any():
+            bad.append("accepted cell inside its opening radius")
+        level = cells[a, 0] >> 27
+        par = (cells[a, 0] & 0x07FFFFFF).astype(np.int64)[level > 1]
+        if len(par) and not ((ds(par) < op[par] * (1 + tol)) | (npc[par] < 3)).all():
+            bad.append("accepted cell whose parent was acceptable")
+    if len(lf) and not ((ds(lf) < op[lf] * (1 + tol)) | (npc[lf] < 3)).all():
+        bad.append("opened leaf outside its opening radius")
+    rng = np.concatenate([cells[a, 2:4], cells[lf, 2:4]]).astype(np.int64)
+    rng = rng[np.argsort(rng[:, 0])]
+    if not (rng[0, 0] == 0 and rng[-1, 1] == n and (rng[1:, 0] == rng[:-1, 1]).all()):
+        bad.append("lists do not tile the particles")
+    return bad
+
+
+def test_opening_rule_fp64_clamp_distance(oracle, bt):
+    """The oracle's interaction lists (or_walk_lists, the traversal of
+    code/computeForces.cu:229-463) obey an FP64 statement of the opening rule
+    computed from the particles alone.  Each planted mistake -- no < 3
+    particles rule, the group as a point, no s term -- breaks the check, and
+    the sample holds cells on both sides of l/theta + s that tell them apart."""
+    n, theta = 20000, 0.5
+    pos, _ = bt.plummer(n, seed=41)
+    r = _pipeline(oracle, pos, theta, 0.01)
+    cells = r["tree"].cells
+    com, op, op_nos, npc = _cell_geometry_fp64(cells, r["pos_tree"], theta)
+    rng = np.random.default_rng(3)
+    sel = rng.choice(len(r["groups"]), size=24, replace=False)
+    found = {1: False, 2: False, 3: False}
+    between = 0
+    for g in sel:
+        pb, npg = r["groups"][g]
+        gp = r["pos_ph"][pb:pb + npg, :3].astype(np.float64)
+        gmin, gmax = gp.min(axis=0), gp.max(axis=0)
+        a, lf = oracle.walk_lists(g, r["groups"], r["tree"].level_range, r["pos_ph"], cells,
+                                  r["size"])
+        assert _check_lists(a, lf, gmin, gmax, cells, com, op, npc, n) == [], g
+        d = np.linalg.norm(com - np.clip(com, gmin, gmax), axis=1)
+        between += int(((d >= op_nos) & (d < op)).sum())
+        for mut in (1, 2, 3):
+            am, lm = oracle.walk_lists(g, r["groups"], r["tree"].level_range, r["pos_ph"], cells,
+                                       r["size"], ls=r["ls"], theta=theta, mutate=mut)
+            if _check_lists(am, lm, gmin, gmax, cells, com, op, npc, n):
+                found[mut] = True
+    assert between > 0
+    assert all(found.values()), found
