@@ -1,7 +1,7 @@
 """Benchmark: one full gravity step (buildTree -> computeMultipoles -> makeGroups
 -> computeForces -> kick-drift-kick) per timed step on one MI355X.
 
-Default workload: N = 2^24 Plummer halo (mass 0.8) + exponential disk
+Headline workload: N = 2^24 Plummer halo (mass 0.8) + exponential disk
 (mass 0.2, scale length 0.3, sech^2 height 0.03), float32, eps = 0.002,
 NLEAF 16, NCRIT 64, levelSplit 5, dt = 1/128, the learned per-pair opening
 angle (bonsai2_amd/theta_mlp.json) with every step re-checked against 256
@@ -9,15 +9,18 @@ FP64 direct sums and falling back to fixed theta = 0.4 if their p99 exceeds
 1e-3.  Synthetic seeded input, resident on the device before timing starts.
 
 Prints ONE JSON line.  value = force evaluations per second of the whole step
-(N / step time).  Beside it, on the same final particle state: the fixed
-theta = 0.4 walk and step (the comparison of the metric), and both walks'
-p50/p99 relative force errors against FP64 direct sums.  The walk kernel's
-roofline uses the reference's flop count (20 per P2P, 64 per P2C;
-code/treecode.cu:150-158) over the walk's own duration, taken with HIP events
-on the stream the kernels run on.  cpu_baseline = the OpenMP C++ oracle (a
-transcription of the reference path, fixed theta) on the host cores: build,
-multipoles and groups in full, the walk on a seeded sample of the groups,
-scaled to all groups.
+(N / step time of the eager steps, each with its HIP-event phase times); the
+same step replayed from its HIP graph is reported beside it.  Also in the
+line, on the same final particle state: the fixed theta = 0.4 walk and step
+(the comparison of the metric), both walks' p50/p99 relative force errors
+against FP64 direct sums, the walk kernel's roofline (the reference's flop
+count, 20 per P2P and 64 per P2C, code/treecode.cu:150-158, over the walk's
+own HIP-event duration), the HBM fraction of the build, sort and multipole
+phases (algorithmic bytes over their HIP-event times), the plain fixed-angle
+walk on a Plummer sphere of 2^24, every other configuration of the
+benchmark (C1-C4) with its errors, drift and timings, and the CPU baseline:
+the OpenMP C++ oracle (a transcription of the reference path, fixed theta)
+on the host cores, median of 3.
 """
 import argparse
 import json
@@ -33,20 +36,16 @@ sys.path.insert(0, ROOT)
 METRIC = ("force-evals/s at N=2^24, p99 rel force err ≤1e-3 (learned vs fixed θ); "
           "interactions/s")
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per walk launch from two rocprofv3 --pmc passes (FETCH_SIZE, then
-# WRITE_SIZE) scaled by a 1 GiB device copy (FETCH_SIZE reads 0.5x, WRITE_SIZE
-# 1.0x): profiles/hbm_traffic_halo_disk_2p24_pmc.md.  Counters cannot be read
-# from inside the bench run, so only the measured default configurations carry a
-# figure; every other configuration reports null.
-WALK_TRAFFIC_BYTES = {("halo_disk_2p24", "learned"): 4.095e9,
-                      ("halo_disk_2p24", "fixed"): 1.126e10}
+HBM_PEAK_GBS = 8000.0
+SEED = 19810614
 
 WORKLOADS = {
     "halo_disk_2p24": dict(n=1 << 24, ic="halo_disk", theta=0.4, eps=0.002),
     "clustered_2p24": dict(n=1 << 24, ic="clustered", theta=0.5, eps=0.002),
     "ensemble_256": dict(n=0, ic="ensemble", theta=0.5, eps=0.01),
-    "plummer_2p24": dict(n=1 << 24, ic="plummer", theta=0.5, eps=0.002),
+    "plummer_2p24": dict(n=1 << 24, ic="plummer", theta=0.4, eps=0.002),
     "plummer_2p22": dict(n=1 << 22, ic="plummer", theta=0.5, eps=0.005),
+    "plummer_2p16": dict(n=1 << 16, ic="plummer", theta=0.4, eps=0.01),
     "plummer_2p20": dict(n=1 << 20, ic="plummer", theta=0.5, eps=0.005),
 }
 
@@ -59,46 +58,209 @@ def make_ic(bt, ic, n, seed):
     return bt.halo_disk(n, seed=seed)
 
 
-def cpu_baseline(pos, nleaf, ncrit, theta, eps, level_split, sample_groups, seed=5):
-    from oracle import oracle as O
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+# ---- host description and the CPU baseline -------------------------------------
+
+def host_info():
+    cpus = sorted(os.sched_getaffinity(0))
+    model, cores = "unknown", set()
+    try:
+        cur = {}
+        with open("/proc/cpuinfo") as f:
+            for line in list(f) + [""]:
+                if not line.strip():
+                    if cur.get("processor") is not None and int(cur["processor"]) in cpus:
+                        cores.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                        model = cur.get("model name", model)
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+    except OSError:
+        pass
+    return dict(model=model, logical_cpus=len(cpus), physical_cores=len(cores) or None,
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+                omp_proc_bind=os.environ.get("OMP_PROC_BIND"))
+
+
+def oracle_pipeline_times(O, pos, nleaf, ncrit, theta, eps, level_split, sample_groups, seed):
+    """One oracle pass: build, multipoles, groups in full, the walk on a
+    seeded sample of groups (all when sample_groups is None) -> seconds."""
     t0 = time.perf_counter()
     box, _, _ = O.bounding_box(pos)
     tree = O.build_tree(pos, nleaf, box)
     pos_tree = pos[tree.order]
-    t1 = time.perf_counter()
     size, mono, q0, q1, _ = O.multipoles(tree.cells, pos_tree, theta)
-    t2 = time.perf_counter()
     keys = O.ph_keys(pos_tree, box)
     perm = O.sort_keys(keys)
     pos_ph = pos_tree[perm]
     groups = O.make_groups(keys[perm], level_split, ncrit)
-    t3 = time.perf_counter()
+    t1 = time.perf_counter()
+    if sample_groups is None or sample_groups >= len(groups):
+        sel = np.arange(len(groups))
+    else:
+        rng = np.random.default_rng(seed)
+        sel = np.sort(rng.choice(len(groups), size=sample_groups, replace=False))
+    O.walk(groups[sel], tree.level_range, pos_ph, pos_tree, tree.cells, size, mono, q0, q1, eps)
+    t2 = time.perf_counter()
+    return t1 - t0, t2 - t1, len(sel), len(groups)
+
+
+def cpu_baseline(bt, head_pos, head_w, args):
+    """The OpenMP oracle on the host, median of 3: C1 (Plummer 2^16) in full,
+    and the headline particles with the walk on a seeded sample of groups,
+    scaled to all groups."""
+    from oracle import oracle as O
+    O.lib()
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    info = host_info()
+    w1 = WORKLOADS["plummer_2p16"]
+    p1, _ = make_ic(bt, "plummer", w1["n"], SEED)
+    c1 = []
+    for _ in range(3):
+        b, wk, _, _ = oracle_pipeline_times(O, p1, 16, 64, w1["theta"], w1["eps"], 5, None, 5)
+        c1.append(b + wk)
+    head = []
+    for k in range(3):
+        b, wk, ns, ng = oracle_pipeline_times(O, head_pos, args.nleaf, args.ncrit, head_w["theta"],
+                                              head_w["eps"], args.level_split, args.cpu_groups, 5)
+        head.append((b + wk * ng / ns, b, wk, ns, ng))
+    head.sort()
+    tot, b, wk, ns, ng = head[1]
+    n = len(head_pos)
+    return dict(
+        value=n / tot, unit="force-evals/s", cores=int(info["omp_num_threads"] or
+                                                       info["logical_cpus"]),
+        kind="port",
+        sample="headline particles (N=%d, fixed theta %g): build, multipoles, groups in full "
+               "(%.1f s), walk on %d of %d groups (seeded, %.1f%%, %.1f s) scaled x%.1f; "
+               "median of 3" % (n, head_w["theta"], b, ns, ng, 100.0 * ns / ng, wk, ng / ns),
+        host=info,
+        runs_s=[h[0] for h in head],
+        c1_full={"n": w1["n"], "theta": w1["theta"], "eps": w1["eps"],
+                 "step_s_median": float(np.median(c1)), "runs_s": c1,
+                 "force_evals_per_s": w1["n"] / float(np.median(c1))})
+
+
+# ---- accuracy -----------------------------------------------------------------
+
+def rel_errors(t, bt, idx):
+    d = t.direct_fp64(idx)
+    a = t.acc(bt.ORDER_PH)[idx, :3].astype(np.float64)
+    e = np.linalg.norm(a - d[:, :3], axis=1) / np.linalg.norm(d[:, :3], axis=1)
+    return float(np.percentile(e, 50)), float(np.percentile(e, 99))
+
+
+def sample_idx(n, k, seed):
     rng = np.random.default_rng(seed)
-    ns = min(len(groups), sample_groups)
-    sel = np.sort(rng.choice(len(groups), size=ns, replace=False))
-    O.walk(groups[sel], tree.level_range, pos_ph, pos_tree, tree.cells, size, mono, q0, q1,
-           eps)
-    t4 = time.perf_counter()
-    frac = ns / len(groups)
-    t_walk = (t4 - t3) / frac
-    total = (t3 - t0) + t_walk
-    return dict(value=len(pos) / total, unit="force-evals/s",
-                cores=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
-                kind="port",
-                sample="full build/multipoles/groups (%.1f s); walk on %d of %d groups "
-                       "(seeded, %.2f%%, %.1f s) scaled x%.0f" % (
-                           t3 - t0, ns, len(groups), 100 * frac, t4 - t3, 1 / frac),
-                walk_s_scaled=t_walk, build_s=t1 - t0, multipoles_s=t2 - t1, groups_s=t3 - t2)
+    return np.sort(rng.choice(n, size=min(n, k), replace=False)).astype(np.int32)
 
 
-def bench_ensemble(args, bt):
-    """256 independent Plummer systems, N_k = round(2^U[10, 16]), in one context:
-    autotuned NLEAF / NCRIT / walk grid / stack, the step captured in a HIP
-    graph and replayed; aggregate interactions/s and per-system energy drift."""
-    seed = 19810614
-    pos, vel, off = bt.ensemble(256, seed=seed)
-    n = int(off[-1])
+def new_tree(bt, pos, vel, w, ncrit=64, nleaf=16, level_split=5, offsets=None):
+    t = bt.Treecode(w["eps"], w["theta"], ncrit)
+    t.alloc(len(pos), offsets)
+    t.ptcl_h2d(pos, vel)
+    t.buildTree(nleaf)
+    t.computeMultipoles()
+    t.makeGroups(level_split, ncrit)
+    t.computeForces()
+    return t
+
+
+def walk_rate(t, reps=3):
+    ws = []
+    for _ in range(reps):
+        st = t.computeForces()
+        ws.append(t.timings()["walk"])
+    _, tot = t.walk_stats()
+    w = float(np.median(ws))
+    fl = 20.0 * tot[0] + 64.0 * tot[1]
+    return dict(walk_ms=w, flops_per_launch=fl, tflops=fl / (w * 1e-3) / 1e12,
+                frac=fl / (w * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                interactions_per_particle={"p2p": st[0], "p2c": st[2]},
+                interactions_per_s=(tot[0] + tot[1]) / (w * 1e-3))
+
+
+# ---- the configurations -------------------------------------------------------------
+
+def config_c1(bt):
+    """C1: Plummer 2^16, theta 0.4, eps 0.01: one force evaluation and one
+    leapfrog step at dt = 1/64, both against the FP64 direct sum over all
+    particles."""
+    w = WORKLOADS["plummer_2p16"]
+    pos, vel = make_ic(bt, "plummer", w["n"], SEED)
+    t = new_tree(bt, pos, vel, w)
+    allidx = np.arange(w["n"], dtype=np.int32)
+    r = {"n": w["n"], "theta": w["theta"], "eps": w["eps"]}
+    r.update(walk_rate(t))
+    p50, p99 = rel_errors(t, bt, allidx)
+    ek0, ep0 = t.computeEnergies()
+    t.step(1.0 / 64, 16, 5, 64)
+    q50, q99 = rel_errors(t, bt, allidx)
+    ek1, ep1 = t.computeEnergies()
+    r.update(p50_rel_err=p50, p99_rel_err=p99, after_step={"p50_rel_err": q50, "p99_rel_err": q99,
+             "dt": 1.0 / 64}, error_samples=w["n"], step_ms=sum(t.timings().values()),
+             rel_energy_change=float(abs((ek1 + ep1) - (ek0 + ep0)) / abs(ek0 + ep0)))
+    t.close()
+    return r
+
+
+def config_c2(bt):
+    """C2: Plummer 2^22, theta 0.5, eps 0.005: 64 leapfrog steps at dt = 1/128
+    (captured step, replayed), |dE/E| in FP64, errors on 4096 samples."""
+    w = WORKLOADS["plummer_2p22"]
+    pos, vel = make_ic(bt, "plummer", w["n"], SEED + 2)
+    t = new_tree(bt, pos, vel, w)
+    r = {"n": w["n"], "theta": w["theta"], "eps": w["eps"]}
+    r.update(walk_rate(t))
+    ek0, ep0 = t.computeEnergies()
+    t.step_graph(1.0 / 128, 16, 5, 64, nsteps=1)   # capture + first replay
+    t0 = time.perf_counter()
+    t.step_graph(1.0 / 128, 16, 5, 64, nsteps=63)
+    ms = (time.perf_counter() - t0) * 1e3 / 63
+    ek1, ep1 = t.computeEnergies()
+    p50, p99 = rel_errors(t, bt, sample_idx(w["n"], 4096, SEED + 3))
+    r.update(steps=64, dt=1.0 / 128, graph_ms_per_step=ms,
+             rel_energy_drift=float(abs((ek1 + ep1) - (ek0 + ep0)) / abs(ek0 + ep0)),
+             p50_rel_err=p50, p99_rel_err=p99, error_samples=4096)
+    t.close()
+    return r
+
+
+def config_c3(bt):
+    """C3: clustered 2^24 (64 Plummer subclusters), theta 0.5, eps 0.002:
+    compensated against plain FP32 accumulation, errors on 8192 samples."""
+    w = WORKLOADS["clustered_2p24"]
+    pos, vel = make_ic(bt, "clustered", w["n"], SEED + 4)
+    t = new_tree(bt, pos, vel, w)
+    idx = sample_idx(w["n"], 8192, SEED + 5)
+    r = {"n": w["n"], "theta": w["theta"], "eps": w["eps"], "error_samples": 8192}
+    accs = {}
+    for kahan in (1, 0):
+        t.set_walk_options(kahan=kahan)
+        rate = walk_rate(t)
+        p50, p99 = rel_errors(t, bt, idx)
+        accs[kahan] = t.acc(bt.ORDER_PH)[:, :3].astype(np.float64)
+        rate.update(p50_rel_err=p50, p99_rel_err=p99)
+        r["kahan" if kahan else "plain"] = rate
+    r["rel_rms_difference"] = float(np.sqrt(((accs[1] - accs[0]) ** 2).sum() /
+                                            (accs[1] ** 2).sum()))
+    r["nlevels"] = t.counts()["nlevels"]
+    t.set_walk_options(kahan=1)
+    t.close()
+    return r
+
+
+def config_c4(bt, args):
+    """C4: 256 independent Plummer systems, N_k = round(2^U[10, 16]), theta
+    0.5, eps 0.01: autotuned NLEAF / NCRIT / walk grid / stack, 32 steps of
+    the captured step, aggregate interactions/s and per-system drift."""
     w = WORKLOADS["ensemble_256"]
+    pos, vel, off = bt.ensemble(256, seed=SEED)
+    n = int(off[-1])
     t = bt.Treecode(w["eps"], w["theta"], 64)
     t.alloc(n, off)
     t.ptcl_h2d(pos, vel)
@@ -111,67 +273,102 @@ def bench_ensemble(args, bt):
     t.makeGroups(args.level_split, nc)
     t.computeForces()
     ek0, ep0 = t.computeEnergies()
-    steps = args.steps
-    # eager steps for reference (and warm-up), then the captured step
     te0 = time.perf_counter()
-    for _ in range(args.warmup):
-        t.step(args.dt, nl, args.level_split, nc)
-    ms_eager = (time.perf_counter() - te0) * 1e3 / max(args.warmup, 1)
-    st, tot = t.walk_stats()
-    t.step_graph(args.dt, nl, args.level_split, nc, nsteps=1)   # capture + first replay
+    for _ in range(2):
+        t.step(1.0 / 128, nl, args.level_split, nc)
+    ms_eager = (time.perf_counter() - te0) * 1e3 / 2
+    t.step_graph(1.0 / 128, nl, args.level_split, nc, nsteps=1)
     t0 = time.perf_counter()
-    t.step_graph(args.dt, nl, args.level_split, nc, nsteps=steps)
-    ms_step = (time.perf_counter() - t0) * 1e3 / steps
-    st2, tot2 = t.walk_stats()
+    t.step_graph(1.0 / 128, nl, args.level_split, nc, nsteps=29)
+    ms_step = (time.perf_counter() - t0) * 1e3 / 29
+    st, tot = t.walk_stats()
     ek1, ep1 = t.computeEnergies()
-    e0 = ek0 + ep0
-    e1 = ek1 + ep1
-    drift = np.abs(e1 - e0) / np.abs(e0)
-    inter = tot2[0] + tot2[1]
-    out = {
-        "metric": "interactions/s (ensemble of 256 systems, one persistent walk per step)",
-        "value": inter / (ms_step * 1e-3),
-        "unit": "interactions/s",
-        "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": ms_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (256 seeded Plummer systems, resident in HBM)",
-        "config": {"workload": "ensemble_256", "n": n, "nsys": 256, "theta": w["theta"],
-                   "eps": w["eps"], "dt": args.dt, "level_split": args.level_split,
-                   "autotuned": best, "hip_graph": True},
-        "force_evals_per_s": n / (ms_step * 1e-3),
-        "eager_ms_per_step": ms_eager,
-        "autotune_s": tune_s,
-        "autotune_table": table,
-        "interactions_per_particle": {"p2p": st2[0], "p2c": st2[2]},
-        "energy_drift": {"median": float(np.median(drift)), "p90": float(np.percentile(drift, 90)),
-                         "max": float(drift.max()), "steps": args.warmup + steps + 1},
-        "system_sizes": {"min": int(np.diff(off).min()), "max": int(np.diff(off).max()),
-                         "total": n},
-        "cpu_baseline": None,
-    }
+    drift = np.abs((ek1 + ep1) - (ek0 + ep0)) / np.abs(ek0 + ep0)
+    p50, p99 = rel_errors(t, bt, sample_idx(n, 4096, SEED + 6))
+    r = {"n": n, "nsys": 256, "theta": w["theta"], "eps": w["eps"], "steps": 32,
+         "autotuned": best, "autotune_s": tune_s, "graph_ms_per_step": ms_step,
+         "eager_ms_per_step": ms_eager,
+         "interactions_per_s": (tot[0] + tot[1]) / (ms_step * 1e-3),
+         "force_evals_per_s": n / (ms_step * 1e-3),
+         "interactions_per_particle": {"p2p": st[0], "p2c": st[2]},
+         "energy_drift": {"median": float(np.median(drift)),
+                          "p90": float(np.percentile(drift, 90)), "max": float(drift.max())},
+         "p50_rel_err": p50, "p99_rel_err": p99, "error_samples": 4096,
+         "system_sizes": {"min": int(np.diff(off).min()), "max": int(np.diff(off).max())}}
     t.close()
-    print(json.dumps(out))
+    return r
 
+
+def plain_walk(bt):
+    """The plain fixed-angle walk at N = 2^24 Plummer (theta 0.4, eps
+    0.002): its HIP-event duration and the fraction of FP32 vector peak."""
+    w = WORKLOADS["plummer_2p24"]
+    pos, vel = make_ic(bt, "plummer", w["n"], SEED + 7)
+    t = new_tree(bt, pos, vel, w)
+    r = {"workload": "plummer_2p24", "n": w["n"], "theta": w["theta"], "eps": w["eps"],
+         "kernel": "k_walk (fixed theta)"}
+    r.update(walk_rate(t, reps=5))
+    r["roofline"] = {"bound": "valu", "achieved": r["tflops"], "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": r["frac"]}
+    p50, p99 = rel_errors(t, bt, sample_idx(w["n"], 4096, SEED + 8))
+    r.update(p50_rel_err=p50, p99_rel_err=p99, error_samples=4096)
+    t.close()
+    return r
+
+
+# ---- HBM fractions of the non-walk phases ------------------------------------------------
+
+def hbm_phases(t, phase, n, sort_passes):
+    """Algorithmic bytes of the build, sort and multipole phases over their
+    HIP-event times (bytes per kernel as listed in DESIGN.md)."""
+    tr = t.tree(multipoles=False)
+    cells = tr["cells"].astype(np.int64)
+    node = cells[:, 1] != 0xFFFFFFFF
+    nsplit = int((cells[node, 3] - cells[node, 2]).sum())
+    ncells = len(cells)
+    ngroups = t.counts()["ngroups"]
+    b = {"bbox_keys": 44.0 * n,
+         "sort": 24.0 * n * sort_passes,
+         "levels": 32.0 * nsplit + 16.0 * ncells,
+         "multipoles": 16.0 * n + 112.0 * ncells,
+         "ph_keys": 28.0 * n,
+         "ph_sort": 24.0 * n * sort_passes,
+         "groups": 36.0 * n + 8.0 * n + 8.0 * ngroups}
+    out = {}
+    for k, v in b.items():
+        ms = phase.get(k, 0.0)
+        gbs = v / (ms * 1e-3) / 1e9 if ms > 0 else None
+        out[k] = {"bytes": v, "ms": ms, "GB_s": gbs,
+                  "frac": gbs / HBM_PEAK_GBS if gbs else None}
+    out["sort_passes"] = sort_passes
+    return out
+
+
+# ---- the headline ------------------------------------------------------------------------
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="halo_disk_2p24", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="halo_disk_2p24",
+                    choices=[k for k in sorted(WORKLOADS) if k != "ensemble_256"])
     ap.add_argument("--n", type=int, default=0, help="override the particle count")
     ap.add_argument("--nleaf", type=int, default=16)
     ap.add_argument("--ncrit", type=int, default=64)
     ap.add_argument("--level-split", type=int, default=5)
     ap.add_argument("--dt", type=float, default=1.0 / 128)
     ap.add_argument("--samples", type=int, default=4096)
-    ap.add_argument("--cpu-groups", type=int, default=32768)
+    ap.add_argument("--cpu-groups", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip C1-C4 and the plain Plummer walk")
     ap.add_argument("--no-kahan", action="store_true")
     ap.add_argument("--group-mode", type=int, default=0, choices=(0, 1, 2),
                     help="0 PH-key groups (makeGroups), 1 leaves, 2 tree cells <= ncrit")
-    ap.add_argument("--walk-variant", type=int, default=0, choices=(0, 1),
-                    help="0: j-data staged in LDS, 1: j-data by scalar loads")
+    ap.add_argument("--walk-variant", type=int, default=0, choices=(0, 1, 2),
+                    help="0: two i-particles per lane, LDS j-data; 1: scalar-load j-data; "
+                         "2: one i-particle per lane, LDS j-data")
     ap.add_argument("--opening", default="learned", choices=("learned", "fixed"))
     ap.add_argument("--mlp", default=None, help="trained network (default bonsai2_amd/theta_mlp.json)")
     ap.add_argument("--check-samples", type=int, default=256)
@@ -181,8 +378,6 @@ def main():
                          "of known size, for scaling HBM byte counters)")
     ap.add_argument("--monopole", action="store_true",
                     help="monopole-only P2C (the reference without QUADRUPOLE; 20 flops each)")
-    ap.add_argument("--kahan-compare", action="store_true",
-                    help="also report the fixed-theta walk with and without compensated sums")
     ap.add_argument("--compare-steps", type=int, default=3,
                     help="timed fixed-theta steps for the comparison (0 = walk only)")
     args = ap.parse_args()
@@ -202,15 +397,12 @@ def main():
         del src, dst
 
     import bonsai2_amd as bt
-    if args.workload == "ensemble_256":
-        bench_ensemble(args, bt)
-        return
     w = dict(WORKLOADS[args.workload])
     if args.n:
         w["n"] = args.n
     n = w["n"]
-    seed = 19810614
-    pos, vel = make_ic(bt, w["ic"], n, seed)
+    pos, vel = make_ic(bt, w["ic"], n, SEED)
+    log("headline %s n=%d" % (args.workload, n))
 
     t = bt.Treecode(w["eps"], w["theta"], args.ncrit)
     if args.no_kahan:
@@ -229,7 +421,7 @@ def main():
         params, budget, th_lo, th_hi = learn.load(args.mlp or learn.DEFAULT_PATH)
         t.set_mlp_params(params)
         t.set_opening(1, budget, th_lo, th_hi)
-        t.set_opening_check(args.check_samples, args.p99_max, seed)
+        t.set_opening_check(args.check_samples, args.p99_max, SEED)
     # first force evaluation (not timed), then warmup steps
     t.buildTree(args.nleaf)
     t.computeMultipoles()
@@ -253,6 +445,12 @@ def main():
         inter.append(tot[0] + tot[1])
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / args.steps
+    # the same step replayed from its HIP graph (one host synchronisation)
+    t.step_graph(args.dt, args.nleaf, args.level_split, args.ncrit, nsteps=1)
+    tg0 = time.perf_counter()
+    t.step_graph(args.dt, args.nleaf, args.level_split, args.ncrit, nsteps=args.steps)
+    ms_graph = (time.perf_counter() - tg0) * 1e3 / args.steps
+    log("headline: %.2f ms/step eager, %.2f ms/step graph" % (ms_step, ms_graph))
 
     walk_ms = np.array(walk_ms)
     flops = np.array(flops)
@@ -260,47 +458,20 @@ def main():
     achieved = float((flops / (walk_ms * 1e-3)).mean() / 1e12)
     chk = t.opening_check() if learned else None
     # accuracy against the FP64 direct sum on seeded samples (outside the timed region)
-    rng = np.random.default_rng(seed + 1)
-    idx = np.sort(rng.choice(n, size=min(n, args.samples), replace=False)).astype(np.int32)
-    d = t.direct_fp64(idx)
-
-    def errors():
-        a = t.acc(bt.ORDER_PH)[idx, :3].astype(np.float64)
-        e = np.linalg.norm(a - d[:, :3], axis=1) / np.linalg.norm(d[:, :3], axis=1)
-        return tuple(float(x) for x in np.percentile(e, [50, 99]))
-
-    p50, p99 = errors()
+    t.computeForces()
+    idx = sample_idx(n, args.samples, SEED + 1)
+    p50, p99 = rel_errors(t, bt, idx)
     ek, ep = t.computeEnergies()
     e_end = float(ek.sum() + ep.sum())
     c = t.counts()
     st_main, _ = t.walk_stats()
-    kahan_cmp = None
-    if args.kahan_compare:
-        # the fixed-theta walk on the final state, compensated and plain FP32 sums
-        if learned:
-            t.set_opening_check(0, args.p99_max, seed)
-            t.set_opening(0)
-        res = {}
-        accs = {}
-        for kahan in (1, 0):
-            t.set_walk_options(kahan=kahan)
-            t.computeForces()
-            accs[kahan] = t.acc(bt.ORDER_PH)[:, :3].astype(np.float64)
-            k50, k99 = errors()
-            res["kahan" if kahan else "plain"] = {"p50_rel_err": k50, "p99_rel_err": k99,
-                                                  "walk_ms": t.timings()["walk"]}
-        res["rel_rms_difference"] = float(np.sqrt(((accs[1] - accs[0]) ** 2).sum() /
-                                                  (accs[1] ** 2).sum()))
-        t.set_walk_options(kahan=0 if args.no_kahan else 1)
-        if learned:
-            t.set_opening(1, budget, th_lo, th_hi)
-            t.set_opening_check(args.check_samples, args.p99_max, seed)
-        kahan_cmp = res
+    phase_ms = {k: float(np.mean([p[k] for p in phase])) for k in phase[0]}
+    hbm = hbm_phases(t, phase_ms, n, sort_passes=bt.SORT_PASSES)
     compare = None
     if learned:
         # the metric's comparison on the same particles and tree: the fixed
         # theta = 0.4 walk (and, with --compare-steps, its full step)
-        t.set_opening_check(0, args.p99_max, seed)
+        t.set_opening_check(0, args.p99_max, SEED)
         lw = []
         for _ in range(3):
             t.computeForces()
@@ -312,13 +483,15 @@ def main():
         for _ in range(3):
             st_f = t.computeForces()
             fw.append(t.timings()["walk"])
-        f50, f99 = errors()
+        f50, f99 = rel_errors(t, bt, idx)
         compare = {"theta": w["theta"], "walk_ms": float(np.median(fw)),
                    "learned_walk_ms": float(np.median(lw)),
                    "walk_speedup": float(np.median(fw) / np.median(lw)),
                    "p50_rel_err": f50, "p99_rel_err": f99,
                    "interactions_per_particle": {"p2p": st_f[0], "p2c": st_f[2]},
                    "flops_per_launch": (20.0 * st_f[0] + 64.0 * st_f[2]) * n}
+        compare["roofline_frac"] = compare["flops_per_launch"] / (compare["walk_ms"] * 1e-3) / \
+            1e12 / FP32_PEAK_TFLOPS
         # the fixed angle as accurate as the learned one: the largest theta on a
         # 0.025 grid whose p99 (same samples, same particles) is at or below the
         # learned walk's p99
@@ -327,7 +500,7 @@ def main():
             t.set_params(theta=float(th))
             t.computeMultipoles()
             t.computeForces()
-            _, e99 = errors()
+            _, e99 = rel_errors(t, bt, idx)
             if e99 > p99:
                 break
             ew = [t.timings()["walk"]]
@@ -347,8 +520,9 @@ def main():
             compare["ms_per_step"] = ms_f
             compare["value"] = n / (ms_f * 1e-3)
             compare["step_speedup"] = ms_f / ms_step
+    t.close()
+    log("headline done")
 
-    phase_ms = {k: float(np.mean([p[k] for p in phase])) for k in phase[0]}
     out = {
         "metric": METRIC,
         "value": n / (ms_step * 1e-3),
@@ -370,8 +544,10 @@ def main():
                                    budget, th_lo, th_hi, args.check_samples, w["theta"],
                                    args.p99_max)) if learned else "fixed theta",
                    "kahan": not args.no_kahan, "quadrupole": not args.monopole,
-                   "walk_jdata": "scalar" if args.walk_variant == 1 else "lds",
+                   "walk_variant": args.walk_variant,
                    "groups": ("PH keys", "leaves", "tree cells")[args.group_mode]},
+        "graph_ms_per_step": ms_graph,
+        "value_graph": n / (ms_graph * 1e-3),
         "interactions_per_s": float((inter / (walk_ms * 1e-3)).mean()),
         "walk_ms": float(walk_ms.mean()),
         "force_evals_per_s_walk": float(n / (walk_ms.mean() * 1e-3)),
@@ -381,14 +557,15 @@ def main():
         "energy": {"kinetic": float(ek.sum()), "potential": float(ep.sum()),
                    "E_start": e_start, "E_end": e_end,
                    "rel_drift": abs(e_end - e_start) / abs(e_start),
-                   "steps_integrated": args.warmup + args.steps},
+                   "steps_integrated": args.warmup + 2 * args.steps + 1},
         "tree": {"ncells": c["ncells"], "nlevels": c["nlevels"], "ngroups": c["ngroups"]},
         "phase_ms": phase_ms,
+        "phase_hbm": hbm,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": (WALK_TRAFFIC_BYTES.get((args.workload, args.opening))
-                                 if not args.n and args.nleaf == 16 and args.ncrit == 64
-                                 else None),
+                     # HBM bytes per launch need two rocprofv3 --pmc passes, which
+                     # cannot run inside this process: profiles/ holds them
+                     "traffic": None,
                      "kernel": "k_walk<learned>" if learned else "k_walk",
                      "flops_per_launch": float(flops.mean()),
                      "avg_launch_ms": float(walk_ms.mean())},
@@ -399,14 +576,22 @@ def main():
     if learned:
         out["opening_check"] = chk
         out["fixed_theta"] = compare
-    if kahan_cmp is not None:
-        out["kahan_compare"] = kahan_cmp
+    if not args.no_configs:
+        configs = {"C5_halo_disk_2p24": {"see": "headline"}}
+        for name, fn in (("C1_plummer_2p16", lambda: config_c1(bt)),
+                         ("C2_plummer_2p22", lambda: config_c2(bt)),
+                         ("C3_clustered_2p24", lambda: config_c3(bt)),
+                         ("C4_ensemble_256", lambda: config_c4(bt, args))):
+            log(name)
+            configs[name] = fn()
+        out["configs"] = configs
+        log("plain walk")
+        out["plain_walk"] = plain_walk(bt)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(pos, args.nleaf, args.ncrit, w["theta"], w["eps"],
-                                           args.level_split, args.cpu_groups)
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(bt, pos, w, args)
     else:
         out["cpu_baseline"] = None
-    t.close()
     print(json.dumps(out))
 
 

@@ -21,6 +21,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbonsai2_amd.so")
 
 ORDER_INPUT, ORDER_TREE, ORDER_PH = 0, 1, 2
+# radix passes per sort of the 63-bit keys (bytes per pass: 24 per key-value pair)
+SORT_PASSES = 8
 P = ctypes.c_void_p
 I = ctypes.c_int
 F = ctypes.c_float
@@ -78,6 +80,15 @@ _SIGS = {
     "bt_ic_plummer": [I, ctypes.c_uint64, D, P, P],
     "bt_ic_clustered": [I, I, ctypes.c_uint64, D, D, D, P, P],
     "bt_ic_halo_disk": [I, ctypes.c_uint64, D, D, D, P, P],
+    "bt_create_f64": [ctypes.POINTER(P), D, D, I],
+    "bt_set_params_f64": [P, D, D],
+    "bt_set_particles_f64": [P, P, P],
+    "bt_get_particles_f64": [P, P, P, I],
+    "bt_get_acc_f64": [P, P, I],
+    "bt_get_tree_f64": [P, P, P, P, P],
+    "bt_get_boxes_f64": [P, P],
+    "bt_move_particles_f64": [P, D],
+    "bt_step_f64": [P, D, I, I, I],
     "bt_host_alloc": [ctypes.POINTER(P), ctypes.c_size_t],
     "bt_host_free": [P],
     "bt_get_device_buffer": [P, I, ctypes.POINTER(P), P],
@@ -122,21 +133,33 @@ def _f4(a):
     return a
 
 
-class Treecode:
-    """Drop-in mirror of Treecode<float> (code/Treecode.h:164-221).
+def _d4(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim != 2 or a.shape[1] != 4:
+        raise ValueError("expected an (n, 4) float64 array")
+    return a
 
-    Treecode(eps=0.01, theta=0.75, ncrit=64) as the reference constructor.
+
+class Treecode:
+    """Drop-in mirror of Treecode<real_t> (code/Treecode.h:164-221).
+
+    Treecode(eps=0.01, theta=0.75, ncrit=64) as the reference constructor;
+    fp64=True makes Treecode<double> (code/Particle4.h:106-125): particles,
+    moments, walk and integration in FP64, arrays of float64.
     """
 
-    def __init__(self, eps=0.01, theta=0.75, ncrit=64):
+    def __init__(self, eps=0.01, theta=0.75, ncrit=64, fp64=False):
         self._L = lib()
         h = P()
-        rc = self._L.bt_create(ctypes.byref(h), float(eps), float(theta), int(ncrit))
+        self.fp64 = bool(fp64)
+        create = self._L.bt_create_f64 if self.fp64 else self._L.bt_create
+        rc = create(ctypes.byref(h), float(eps), float(theta), int(ncrit))
         if rc != 0:
             raise BonsaiError("bt_create failed (%d): no HIP device or runtime error" % rc)
         self._h = h
-        self.eps = float(np.float32(eps))
-        self.theta = float(np.float32(theta))
+        self._rt = np.float64 if self.fp64 else np.float32
+        self.eps = float(self._rt(eps))
+        self.theta = float(self._rt(theta))
         self.nCrit = int(ncrit)
         self.nLeaf = 16
         self.nPtcl = 0
@@ -174,10 +197,11 @@ class Treecode:
 
     def set_params(self, eps=None, theta=None):
         if eps is not None:
-            self.eps = float(np.float32(eps))
+            self.eps = float(self._rt(eps))
         if theta is not None:
-            self.theta = float(np.float32(theta))
-        self._ck(self._L.bt_set_params(self._h, self.eps, self.theta), "set_params")
+            self.theta = float(self._rt(theta))
+        fn = self._L.bt_set_params_f64 if self.fp64 else self._L.bt_set_params
+        self._ck(fn(self._h, self.eps, self.theta), "set_params")
 
     def set_walk_options(self, kahan=1, stack_ints=16384, blocks_per_cu=0):
         self._ck(self._L.bt_set_walk_options(self._h, int(kahan), int(stack_ints),
@@ -312,19 +336,23 @@ class Treecode:
 
     # -- particles ---------------------------------------------------------
     def ptcl_h2d(self, pos, vel):
-        """ptcl_h2d() (code/Treecode.h:207-214): pos/vel are (n,4) float32."""
-        pos, vel = _f4(pos), _f4(vel)
+        """ptcl_h2d() (code/Treecode.h:207-214): pos/vel are (n,4) float32
+        (float64 for an fp64 context)."""
+        cv = _d4 if self.fp64 else _f4
+        pos, vel = cv(pos), cv(vel)
         if len(pos) != self.nPtcl or len(vel) != self.nPtcl:
             raise ValueError("particle count differs from alloc()")
-        self._ck(self._L.bt_set_particles(self._h, _p(pos), _p(vel)), "ptcl_h2d")
+        fn = self._L.bt_set_particles_f64 if self.fp64 else self._L.bt_set_particles
+        self._ck(fn(self._h, _p(pos), _p(vel)), "ptcl_h2d")
 
     def ptcl_d2h(self, order=ORDER_TREE):
         """ptcl_d2h() (code/Treecode.h:200-204) -> (pos, vel).  The default is the
         reference's: the device arrays as they are (tree order after
         buildTree, input order before the first build)."""
-        pos = np.empty((self.nPtcl, 4), np.float32)
-        vel = np.empty((self.nPtcl, 4), np.float32)
-        self._ck(self._L.bt_get_particles(self._h, _p(pos), _p(vel), int(order)), "ptcl_d2h")
+        pos = np.empty((self.nPtcl, 4), self._rt)
+        vel = np.empty((self.nPtcl, 4), self._rt)
+        fn = self._L.bt_get_particles_f64 if self.fp64 else self._L.bt_get_particles
+        self._ck(fn(self._h, _p(pos), _p(vel), int(order)), "ptcl_d2h")
         return pos, vel
 
     # -- the gravity step (reference method names) ----------------------------
@@ -352,7 +380,8 @@ class Treecode:
     def moveParticles(self, dt):
         """First half of a kick-drift-kick step (v += a dt/2, x += v dt); the
         next force evaluation applies the second half kick."""
-        self._ck(self._L.bt_move_particles(self._h, float(dt)), "moveParticles")
+        fn = self._L.bt_move_particles_f64 if self.fp64 else self._L.bt_move_particles
+        self._ck(fn(self._h, float(dt)), "moveParticles")
 
     def kick(self, dt_half):
         self._ck(self._L.bt_kick(self._h, float(dt_half)), "kick")
@@ -366,8 +395,8 @@ class Treecode:
         return ek, ep
 
     def step(self, dt, nLeaf=16, levelSplit=5, nCrit=64):
-        self._ck(self._L.bt_step(self._h, float(dt), int(nLeaf), int(levelSplit), int(nCrit)),
-                 "step")
+        fn = self._L.bt_step_f64 if self.fp64 else self._L.bt_step
+        self._ck(fn(self._h, float(dt), int(nLeaf), int(levelSplit), int(nCrit)), "step")
 
     # -- results -------------------------------------------------------------
     def get_nPtcl(self):
@@ -393,8 +422,9 @@ class Treecode:
         return dict(zip(keys, (int(x) for x in c)))
 
     def acc(self, order=ORDER_PH):
-        a = np.empty((self.nPtcl, 4), np.float32)
-        self._ck(self._L.bt_get_acc(self._h, _p(a), int(order)), "get_acc")
+        a = np.empty((self.nPtcl, 4), self._rt)
+        fn = self._L.bt_get_acc_f64 if self.fp64 else self._L.bt_get_acc
+        self._ck(fn(self._h, _p(a), int(order)), "get_acc")
         return a
 
     def tree(self, multipoles=True):
@@ -403,12 +433,18 @@ class Treecode:
         cells = np.empty((nc, 4), np.uint32)
         lr = np.empty((32, 2), np.int32)
         if multipoles:
-            size = np.empty((nc, 4), np.float32)
-            mono = np.empty((nc, 4), np.float32)
-            q0 = np.empty((nc, 4), np.float32)
-            q1 = np.empty((nc, 2), np.float32)
-            self._ck(self._L.bt_get_tree(self._h, _p(cells), _p(lr), _p(size), _p(mono),
-                                         _p(q0), _p(q1)), "get_tree")
+            size = np.empty((nc, 4), self._rt)
+            mono = np.empty((nc, 4), self._rt)
+            q0 = np.empty((nc, 4), self._rt)
+            q1 = np.empty((nc, 2), self._rt)
+            if self.fp64:
+                self._ck(self._L.bt_get_tree(self._h, _p(cells), _p(lr), None, None, None, None),
+                         "get_tree")
+                self._ck(self._L.bt_get_tree_f64(self._h, _p(size), _p(mono), _p(q0), _p(q1)),
+                         "get_tree_f64")
+            else:
+                self._ck(self._L.bt_get_tree(self._h, _p(cells), _p(lr), _p(size), _p(mono),
+                                             _p(q0), _p(q1)), "get_tree")
             return dict(cells=cells, level_range=lr, size=size, mono=mono, q0=q0, q1=q1)
         self._ck(self._L.bt_get_tree(self._h, _p(cells), _p(lr), None, None, None, None),
                  "get_tree")
@@ -444,8 +480,9 @@ class Treecode:
         return ids
 
     def boxes(self):
-        b = np.empty((self.nsys, 4), np.float32)
-        self._ck(self._L.bt_get_boxes(self._h, _p(b)), "get_boxes")
+        b = np.empty((self.nsys, 4), self._rt)
+        fn = self._L.bt_get_boxes_f64 if self.fp64 else self._L.bt_get_boxes
+        self._ck(fn(self._h, _p(b)), "get_boxes")
         return b
 
     def direct_fp64(self, ph_idx):

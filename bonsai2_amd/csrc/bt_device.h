@@ -62,6 +62,55 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
   return v;
 }
 
+// Vector types of the real type: Treecode<float> (float4 particles and cell
+// data) and Treecode<double> (double4).  A context allocates its particle and
+// cell arrays with the element size of its real type; kernels view them
+// through as4 / as2.
+template <typename R> struct RealVec;
+template <> struct RealVec<float> { typedef float4 v4; typedef float2 v2; };
+template <> struct RealVec<double> { typedef double4 v4; typedef double2 v2; };
+template <typename R> using V4 = typename RealVec<R>::v4;
+template <typename R> using V2 = typename RealVec<R>::v2;
+
+template <typename R>
+__host__ __device__ __forceinline__ V4<R> make_v4(R x, R y, R z, R w) {
+  V4<R> v;
+  v.x = x; v.y = y; v.z = z; v.w = w;
+  return v;
+}
+template <typename R>
+__host__ __device__ __forceinline__ V2<R> make_v2(R x, R y) {
+  V2<R> v;
+  v.x = x; v.y = y;
+  return v;
+}
+template <typename R, typename P>
+__host__ __device__ __forceinline__ V4<R> *as4(P *p) { return reinterpret_cast<V4<R> *>(p); }
+template <typename R, typename P>
+__host__ __device__ __forceinline__ V2<R> *as2(P *p) { return reinterpret_cast<V2<R> *>(p); }
+
+// min / max of the real type, and over the wave
+template <typename R>
+__device__ __forceinline__ R rmin(R a, R b) { return fmin(a, b); }
+template <typename R>
+__device__ __forceinline__ R rmax(R a, R b) { return fmax(a, b); }
+template <>
+__device__ __forceinline__ float rmin(float a, float b) { return fminf(a, b); }
+template <>
+__device__ __forceinline__ float rmax(float a, float b) { return fmaxf(a, b); }
+template <typename R>
+__device__ __forceinline__ R wave_min_r(R v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = rmin(v, __shfl_xor(v, m, kWave));
+  return v;
+}
+template <typename R>
+__device__ __forceinline__ R wave_max_r(R v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = rmax(v, __shfl_xor(v, m, kWave));
+  return v;
+}
+
 // System (root) of a particle index: offsets[0..nsys] partition [0, n).
 __device__ __forceinline__ int system_of(int i, int nsys, const int *__restrict__ off) {
   int lo = 0, hi = nsys - 1;

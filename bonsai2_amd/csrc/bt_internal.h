@@ -72,7 +72,7 @@ struct DevCounters {
 // The values captured graphs bake into their kernel arguments: a graph is
 // replayed only while all of them are unchanged (compared bytewise).
 struct GraphKey {
-  float dt, eps, theta;
+  double dt, eps, theta;
   int nleaf, level_split, ncrit;
   int opening, kahan, quadrupole, walk_variant, stack_ints, blocks_per_cu, lds_stack, group_mode;
   float learn_shift, learn_theta_lo, learn_theta_hi;
@@ -83,9 +83,13 @@ struct GraphKey {
 
 struct Ctx {
   int dev = 0;
+  // Treecode<double>: particle and cell arrays hold double4 / double2 (twice
+  // the float4 / float2 element count is allocated) and the step computes in
+  // FP64; otherwise Treecode<float>
+  bool fp64 = false;
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  float eps = 0.01f, theta = 0.75f;
+  double eps = 0.01, theta = 0.75;   // a float context holds float values
   int ncrit = 64, nleaf = 16, level_split = 1;
 
   // sizes
@@ -180,7 +184,7 @@ struct Ctx {
 
   // moveParticles' half kick still owed: the next computeForces closes the
   // kick-drift-kick step with it (0 = none)
-  float pending_kick = 0.f;
+  double pending_kick = 0.0;
   int check_cap = 0;             // samples d_check / d_check_idx are sized for
 
   // state
@@ -246,8 +250,8 @@ void launch_check_idx(Ctx &c, int ns);
 void launch_learn_check(Ctx &c, int ns, const int *d_idx, const double *d_direct, double *out);
 
 // closing: the second half kick of a step (counted in steps_done)
-void launch_kick(Ctx &c, float dt_half, bool closing = false);
-void launch_drift(Ctx &c, float dt);
+void launch_kick(Ctx &c, double dt_half, bool closing = false);
+void launch_drift(Ctx &c, double dt);
 void launch_clear_sticky(Ctx &c);
 void launch_energies(Ctx &c);
 int direct_slices(int ns);   // d_out of launch_direct: ns * (slices + 1) double4

@@ -84,17 +84,22 @@ void free_cell_arrays(Ctx &c) {
   dev_free(&c.cwalk);
 }
 
+// float4 / float2 elements per value of the context's real type (2 for
+// Treecode<double>, whose arrays hold double4 / double2)
+size_t rw(const Ctx &c) { return c.fp64 ? 2 : 1; }
+
 int alloc_cells(Ctx &c, int cap) {
   int rc = 0;
   c.cap_cells = cap;
-  rc |= dev_alloc(c, &c.cells, cap);
-  rc |= dev_alloc(c, &c.csize, cap);
-  rc |= dev_alloc(c, &c.cmono, cap);
-  rc |= dev_alloc(c, &c.cq0, cap);
-  rc |= dev_alloc(c, &c.cq1, cap);
-  rc |= dev_alloc(c, &c.cls, cap);
-  rc |= dev_alloc(c, &c.cscr, (size_t)cap * 6);
-  rc |= dev_alloc(c, &c.cwalk, (size_t)cap * 3);
+  const size_t w = rw(c), n = (size_t)cap;
+  rc |= dev_alloc(c, &c.cells, n);
+  rc |= dev_alloc(c, &c.csize, n * w);
+  rc |= dev_alloc(c, &c.cmono, n * w);
+  rc |= dev_alloc(c, &c.cq0, n * w);
+  rc |= dev_alloc(c, &c.cq1, n * w);
+  rc |= dev_alloc(c, &c.cls, n * w);
+  rc |= dev_alloc(c, &c.cscr, n * 6 * w);
+  rc |= dev_alloc(c, &c.cwalk, n * 3 * w);
   rc |= dev_alloc(c, &c.child_bnd, (size_t)cap * 8 + 8 * (size_t)c.nsys);
   return rc ? BT_ERR_HIP : 0;
 }
@@ -157,7 +162,7 @@ int phase_build(Ctx &c) {
 
 int phase_multipoles(Ctx &c) {
   launch_multipoles(c);
-  launch_sys_scale(c);
+  if (!c.fp64) launch_sys_scale(c);   // the learned angle's scales (Treecode<float> only)
   record(c, 4);
   BT_LAUNCH_CK();
   return 0;
@@ -261,11 +266,11 @@ int build_checked(Ctx &c) {
 // The second half of a kick-drift-kick step, after the drift: the forces at
 // the new positions and the closing half kick.  Queued without host
 // synchronisation.
-int step_second_half(Ctx &c, float dt) {
+int step_second_half(Ctx &c, double dt) {
   if (phase_build(c) || phase_multipoles(c) || phase_groups(c) || phase_forces(c))
     return BT_ERR_HIP;
   launch_learned_check(c);
-  launch_kick(c, 0.5f * dt, true);
+  launch_kick(c, 0.5 * dt, true);
   record(c, 9);
   BT_LAUNCH_CK();
   return 0;
@@ -274,7 +279,7 @@ int step_second_half(Ctx &c, float dt) {
 // After the queued second half of a step: on a cell-capacity failure (the
 // walk and the closing kick were skipped, the particles hold the step's first
 // half kick and drift) grow the cells once and run the second half again.
-int finish_step(Ctx &c, float dt) {
+int finish_step(Ctx &c, double dt) {
   if (read_counters(c)) return BT_ERR_HIP;
   if (c.hc.err_sticky == 1) {
     int rc = regrow_cells(c);
@@ -310,13 +315,18 @@ bool valid_nleaf(int nleaf) {
 
 extern "C" {
 
-int bt_create(bt_ctx **out, float eps, float theta, int ncrit) {
+}  // extern "C"
+
+namespace {
+
+int create_ctx(bt_ctx **out, double eps, double theta, int ncrit, bool fp64) {
   if (!out) return BT_ERR_ARG;
   *out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BT_ERR_NODEVICE;
   bt_ctx *h = new bt_ctx();
   Ctx &c = h->c;
+  c.fp64 = fp64;
   c.eps = eps;
   c.theta = theta;
   c.ncrit = ncrit;
@@ -346,6 +356,26 @@ int bt_create(bt_ctx **out, float eps, float theta, int ncrit) {
   (void)hipMemcpy(c.level_beg, z.data(), sizeof(int2) * kLevels, hipMemcpyHostToDevice);
   *out = h;
   return 0;
+}
+
+// The real type of a data entry point must match the context's.
+template <typename T>
+int check_real(Ctx &c) {
+  if (c.fp64 == (sizeof(T) == 8)) return 0;
+  return fail(c, BT_ERR_STATE, c.fp64 ? "Treecode<double> context: use the _f64 entry point"
+                                      : "Treecode<float> context: use the float entry point");
+}
+
+}  // namespace
+
+extern "C" {
+
+int bt_create(bt_ctx **out, float eps, float theta, int ncrit) {
+  return create_ctx(out, eps, theta, ncrit, false);
+}
+
+int bt_create_f64(bt_ctx **out, double eps, double theta, int ncrit) {
+  return create_ctx(out, eps, theta, ncrit, true);
 }
 
 int bt_destroy(bt_ctx *h) {
@@ -385,11 +415,18 @@ int bt_set_stream(bt_ctx *h, void *s) {
   return 0;
 }
 
-int bt_set_params(bt_ctx *h, float eps, float theta) {
+int bt_set_params(bt_ctx *h, float eps, float theta) { return bt_set_params_f64(h, eps, theta); }
+
+int bt_set_params_f64(bt_ctx *h, double eps, double theta) {
   if (!h) return BT_ERR_ARG;
-  h->c.eps = eps;
-  if (h->c.theta != theta) h->c.have_mp = false;  // theta is baked into cellOp^2
-  h->c.theta = theta;
+  Ctx &c = h->c;
+  if (!c.fp64) {   // a float context holds float values
+    eps = (float)eps;
+    theta = (float)theta;
+  }
+  c.eps = eps;
+  if (c.theta != theta) c.have_mp = false;  // theta is baked into cellOp^2
+  c.theta = theta;
   return 0;
 }
 
@@ -432,7 +469,7 @@ int bt_get_walk_stack_max(bt_ctx *h, int *entries) {
 
 int bt_set_walk_variant(bt_ctx *h, int variant) {
   if (!h) return BT_ERR_ARG;
-  if (variant < 0 || variant > 1) return fail(h->c, BT_ERR_ARG, "walk variant must be 0 or 1");
+  if (variant < 0 || variant > 2) return fail(h->c, BT_ERR_ARG, "walk variant must be 0, 1 or 2");
   h->c.walk_variant = variant;
   return 0;
 }
@@ -475,19 +512,20 @@ int bt_alloc(bt_ctx *h, int n, int nsys, const int *offsets) {
     free_cell_arrays(c);
     c.cap_n = n;
     int rc = 0;
-    rc |= dev_alloc(c, &c.pos, n); rc |= dev_alloc(c, &c.vel, n);
-    rc |= dev_alloc(c, &c.pos2, n); rc |= dev_alloc(c, &c.vel2, n);
+    const size_t w = rw(c), nw = (size_t)n * w;
+    rc |= dev_alloc(c, &c.pos, nw); rc |= dev_alloc(c, &c.vel, nw);
+    rc |= dev_alloc(c, &c.pos2, nw); rc |= dev_alloc(c, &c.vel2, nw);
     rc |= dev_alloc(c, &c.id, n); rc |= dev_alloc(c, &c.id2, n);
     rc |= dev_alloc(c, &c.keyB, n); rc |= dev_alloc(c, &c.valA, n);
     rc |= dev_alloc(c, &c.valB, n); rc |= dev_alloc(c, &c.geo_keys, n);
-    rc |= dev_alloc(c, &c.pos_ph, n); rc |= dev_alloc(c, &c.perm, n);
+    rc |= dev_alloc(c, &c.pos_ph, nw); rc |= dev_alloc(c, &c.perm, n);
     rc |= dev_alloc(c, &c.id_ph, n); rc |= dev_alloc(c, &c.ph_keys, n);
-    rc |= dev_alloc(c, &c.acc, (size_t)n + 64);  // + spare slots for idle walk lanes
+    rc |= dev_alloc(c, &c.acc, ((size_t)n + 64) * w);  // + spare slots for idle walk lanes
     c.cap_groups = n;
     rc |= dev_alloc(c, &c.groups, n); rc |= dev_alloc(c, &c.group_sys, n);
     rc |= dev_alloc(c, &c.gtmp0, n); rc |= dev_alloc(c, &c.gtmp1, n);
     rc |= dev_alloc(c, &c.gcounts, n); rc |= dev_alloc(c, &c.gpot, n);
-    rc |= dev_alloc(c, &c.box, nsys); rc |= dev_alloc(c, &c.root_cells, nsys);
+    rc |= dev_alloc(c, &c.box, (size_t)nsys * w); rc |= dev_alloc(c, &c.root_cells, nsys);
     rc |= dev_alloc(c, &c.sys_scale, nsys);
     rc |= dev_alloc(c, &c.d_off, nsys + 1);
     rc |= dev_alloc(c, &c.d_energy, 2 * (size_t)nsys * 65);
@@ -495,42 +533,52 @@ int bt_alloc(bt_ctx *h, int n, int nsys, const int *offsets) {
     if (alloc_cells(c, n + 1024 * nsys + 64)) return BT_ERR_HIP;
   }
   c.bbox_blocks = std::max(1, std::min(1024, (c.max_sys_n + 16383) / 16384));
-  if (dev_alloc(c, &c.bbox_part, 2 * (size_t)nsys * c.bbox_blocks)) return BT_ERR_HIP;
+  if (dev_alloc(c, &c.bbox_part, 2 * (size_t)nsys * c.bbox_blocks * rw(c))) return BT_ERR_HIP;
   BT_CK(hipMemcpy(c.d_off, off.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice));
   c.have_particles = c.have_tree = c.have_mp = c.have_groups = c.have_acc = false;
   return 0;
 }
 
-int bt_set_particles(bt_ctx *h, const float *pos4, const float *vel4) {
-  if (!h || !pos4 || !vel4) return BT_ERR_ARG;
-  Ctx &c = h->c;
+}  // extern "C"
+
+namespace {
+
+// Copies n particles in (input order), from host or device memory; bytes =
+// n values of the context's 4-vector type.
+int put_particles(Ctx &c, const void *pos4, const void *vel4, hipMemcpyKind kind) {
   if (c.n <= 0) return fail(c, BT_ERR_STATE, "bt_alloc first");
-  BT_CK(hipMemcpyAsync(c.pos, pos4, sizeof(float4) * c.n, hipMemcpyHostToDevice, c.stream));
-  BT_CK(hipMemcpyAsync(c.vel, vel4, sizeof(float4) * c.n, hipMemcpyHostToDevice, c.stream));
+  const size_t bytes = sizeof(float4) * rw(c) * (size_t)c.n;
+  BT_CK(hipMemcpyAsync(c.pos, pos4, bytes, kind, c.stream));
+  BT_CK(hipMemcpyAsync(c.vel, vel4, bytes, kind, c.stream));
   std::vector<int> ids(c.n);
   for (int i = 0; i < c.n; i++) ids[i] = i;
   BT_CK(hipMemcpyAsync(c.id, ids.data(), sizeof(int) * c.n, hipMemcpyHostToDevice, c.stream));
   BT_CK(hipStreamSynchronize(c.stream));
   c.have_particles = true;
-  c.pending_kick = 0.f;
+  c.pending_kick = 0.0;
   c.have_tree = c.have_mp = c.have_groups = c.have_acc = false;
   return 0;
 }
 
+}  // namespace
+
+extern "C" {
+
+int bt_set_particles(bt_ctx *h, const float *pos4, const float *vel4) {
+  if (!h || !pos4 || !vel4) return BT_ERR_ARG;
+  if (check_real<float>(h->c)) return BT_ERR_STATE;
+  return put_particles(h->c, pos4, vel4, hipMemcpyHostToDevice);
+}
+
+int bt_set_particles_f64(bt_ctx *h, const double *pos4, const double *vel4) {
+  if (!h || !pos4 || !vel4) return BT_ERR_ARG;
+  if (check_real<double>(h->c)) return BT_ERR_STATE;
+  return put_particles(h->c, pos4, vel4, hipMemcpyHostToDevice);
+}
+
 int bt_set_particles_device(bt_ctx *h, const void *dpos, const void *dvel) {
   if (!h || !dpos || !dvel) return BT_ERR_ARG;
-  Ctx &c = h->c;
-  if (c.n <= 0) return fail(c, BT_ERR_STATE, "bt_alloc first");
-  BT_CK(hipMemcpyAsync(c.pos, dpos, sizeof(float4) * c.n, hipMemcpyDeviceToDevice, c.stream));
-  BT_CK(hipMemcpyAsync(c.vel, dvel, sizeof(float4) * c.n, hipMemcpyDeviceToDevice, c.stream));
-  std::vector<int> ids(c.n);
-  for (int i = 0; i < c.n; i++) ids[i] = i;
-  BT_CK(hipMemcpyAsync(c.id, ids.data(), sizeof(int) * c.n, hipMemcpyHostToDevice, c.stream));
-  BT_CK(hipStreamSynchronize(c.stream));
-  c.have_particles = true;
-  c.pending_kick = 0.f;
-  c.have_tree = c.have_mp = c.have_groups = c.have_acc = false;
-  return 0;
+  return put_particles(h->c, dpos, dvel, hipMemcpyDeviceToDevice);
 }
 
 int bt_build_tree(bt_ctx *h, int nleaf) {
@@ -590,11 +638,11 @@ int bt_compute_forces(bt_ctx *h, int intcount, double stats[4]) {
   if (phase_forces(c)) return BT_ERR_HIP;
   launch_learned_check(c);
   // a moveParticles before this evaluation owes its step's closing half kick
-  if (c.pending_kick != 0.f) launch_kick(c, c.pending_kick, true);
+  if (c.pending_kick != 0.0) launch_kick(c, c.pending_kick, true);
   BT_LAUNCH_CK();
   if (read_counters(c)) return BT_ERR_HIP;
   if (c.hc.err_sticky & 4) return fail(c, BT_ERR_STACK, "walk stack overflow");
-  c.pending_kick = 0.f;
+  c.pending_kick = 0.0;
   c.have_acc = true;
   if (intcount && stats) {
     unsigned long long w[4];
@@ -619,18 +667,20 @@ int bt_kick(bt_ctx *h, float dth) {
   return 0;
 }
 
-int bt_move_particles(bt_ctx *h, float dt) {
+int bt_move_particles(bt_ctx *h, float dt) { return bt_move_particles_f64(h, dt); }
+
+int bt_move_particles_f64(bt_ctx *h, double dt) {
   if (!h) return BT_ERR_ARG;
   Ctx &c = h->c;
   if (!c.have_acc) return fail(c, BT_ERR_STATE, "computeForces first");
   launch_clear_sticky(c);
-  launch_kick(c, 0.5f * dt);
+  launch_kick(c, 0.5 * dt);
   launch_drift(c, dt);
   BT_LAUNCH_CK();
   BT_CK(hipStreamSynchronize(c.stream));
   // positions moved: the tree, moments, groups and forces are stale; the
   // next force evaluation applies the step's second half kick
-  c.pending_kick = 0.5f * dt;
+  c.pending_kick = 0.5 * dt;
   c.have_tree = c.have_mp = c.have_groups = c.have_acc = false;
   return 0;
 }
@@ -664,6 +714,10 @@ int bt_compute_energies(bt_ctx *h, double *ekin, double *epot) {
 // One kick-drift-kick step (forces first if none are current), queued without
 // host synchronisation until the end.
 int bt_step(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit) {
+  return bt_step_f64(h, dt, nleaf, level_split, ncrit);
+}
+
+int bt_step_f64(bt_ctx *h, double dt, int nleaf, int level_split, int ncrit) {
   if (!h) return BT_ERR_ARG;
   Ctx &c = h->c;
   c.err.clear();
@@ -681,7 +735,7 @@ int bt_step(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit) {
     if (rc) return rc;
   }
   launch_clear_sticky(c);
-  launch_kick(c, 0.5f * dt);
+  launch_kick(c, 0.5 * dt);
   launch_drift(c, dt);
   c.have_tree = c.have_mp = c.have_groups = c.have_acc = false;
   int rc = step_second_half(c, dt);
@@ -694,64 +748,89 @@ int bt_step(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit) {
 
 // ---- results --------------------------------------------------------------
 
-int bt_get_acc(bt_ctx *h, float *acc4, int order) {
-  if (!h || !acc4) return BT_ERR_ARG;
-  Ctx &c = h->c;
+}  // extern "C"
+
+namespace {
+
+// acc (PH order on the device) in the requested order; T = the context's real type
+template <typename T>
+int get_acc_t(Ctx &c, T *acc4, int order) {
+  typedef typename RealVec<T>::v4 V;
+  if (check_real<T>(c)) return BT_ERR_STATE;
   if (!c.have_acc) return fail(c, BT_ERR_STATE, "no forces");
-  std::vector<float4> a(c.n);
-  BT_CK(hipMemcpyAsync(a.data(), c.acc, sizeof(float4) * c.n, hipMemcpyDeviceToHost, c.stream));
+  if (order != BT_ORDER_PH && order != BT_ORDER_TREE && order != BT_ORDER_INPUT)
+    return fail(c, BT_ERR_ARG, "bad order");
+  std::vector<V> a(c.n);
+  BT_CK(hipMemcpyAsync(a.data(), c.acc, sizeof(V) * c.n, hipMemcpyDeviceToHost, c.stream));
+  V *o = reinterpret_cast<V *>(acc4);
   if (order == BT_ORDER_PH) {
     BT_CK(hipStreamSynchronize(c.stream));
-    std::memcpy(acc4, a.data(), sizeof(float4) * c.n);
+    std::memcpy(o, a.data(), sizeof(V) * c.n);
     return 0;
   }
   std::vector<int> map(c.n);
-  if (order == BT_ORDER_TREE)
-    BT_CK(hipMemcpyAsync(map.data(), c.perm, sizeof(int) * c.n, hipMemcpyDeviceToHost, c.stream));
-  else if (order == BT_ORDER_INPUT)
-    BT_CK(hipMemcpyAsync(map.data(), c.id_ph, sizeof(int) * c.n, hipMemcpyDeviceToHost, c.stream));
-  else
-    return fail(c, BT_ERR_ARG, "bad order");
+  BT_CK(hipMemcpyAsync(map.data(), order == BT_ORDER_TREE ? c.perm : c.id_ph,
+                       sizeof(int) * c.n, hipMemcpyDeviceToHost, c.stream));
   BT_CK(hipStreamSynchronize(c.stream));
-  float4 *o = reinterpret_cast<float4 *>(acc4);
   for (int k = 0; k < c.n; k++) o[map[k]] = a[k];
   return 0;
 }
 
-int bt_get_particles(bt_ctx *h, float *pos4, float *vel4, int order) {
-  if (!h) return BT_ERR_ARG;
-  Ctx &c = h->c;
+template <typename T>
+int get_particles_t(Ctx &c, T *pos4, T *vel4, int order) {
+  typedef typename RealVec<T>::v4 V;
+  if (check_real<T>(c)) return BT_ERR_STATE;
   if (!c.have_particles) return fail(c, BT_ERR_STATE, "no particles");
-  std::vector<float4> p(c.n), v(c.n);
-  std::vector<int> id(c.n);
-  BT_CK(hipMemcpyAsync(p.data(), c.pos, sizeof(float4) * c.n, hipMemcpyDeviceToHost, c.stream));
-  BT_CK(hipMemcpyAsync(v.data(), c.vel, sizeof(float4) * c.n, hipMemcpyDeviceToHost, c.stream));
+  if (order != BT_ORDER_PH && order != BT_ORDER_TREE && order != BT_ORDER_INPUT)
+    return fail(c, BT_ERR_ARG, "bad order");
+  if (order == BT_ORDER_PH && !c.have_groups)
+    return fail(c, BT_ERR_STATE, "no PH order before makeGroups");
+  std::vector<V> p(c.n), v(c.n);
+  std::vector<int> id(c.n), perm;
+  BT_CK(hipMemcpyAsync(p.data(), c.pos, sizeof(V) * c.n, hipMemcpyDeviceToHost, c.stream));
+  BT_CK(hipMemcpyAsync(v.data(), c.vel, sizeof(V) * c.n, hipMemcpyDeviceToHost, c.stream));
   BT_CK(hipMemcpyAsync(id.data(), c.id, sizeof(int) * c.n, hipMemcpyDeviceToHost, c.stream));
-  std::vector<int> perm;
   if (order == BT_ORDER_PH) {
-    if (!c.have_groups) return fail(c, BT_ERR_STATE, "no PH order before makeGroups");
     perm.resize(c.n);
     BT_CK(hipMemcpyAsync(perm.data(), c.perm, sizeof(int) * c.n, hipMemcpyDeviceToHost,
                          c.stream));
   }
   BT_CK(hipStreamSynchronize(c.stream));
-  float4 *po = reinterpret_cast<float4 *>(pos4);
-  float4 *vo = reinterpret_cast<float4 *>(vel4);
-  for (int i = 0; i < c.n; i++) {
-    int dst = i;
-    if (order == BT_ORDER_INPUT) dst = id[i];
-    if (order == BT_ORDER_PH) continue;
-    if (po) po[dst] = p[i];
-    if (vo) vo[dst] = v[i];
+  V *po = reinterpret_cast<V *>(pos4);
+  V *vo = reinterpret_cast<V *>(vel4);
+  for (int k = 0; k < c.n; k++) {
+    // tree order: the arrays as they are; input order: through the ids;
+    // PH order: slot k holds tree particle perm[k]
+    const int src = order == BT_ORDER_PH ? perm[k] : k;
+    const int dst = order == BT_ORDER_INPUT ? id[k] : k;
+    if (po) po[dst] = p[src];
+    if (vo) vo[dst] = v[src];
   }
-  if (order == BT_ORDER_PH)
-    for (int k = 0; k < c.n; k++) {
-      if (po) po[k] = p[perm[k]];
-      if (vo) vo[k] = v[perm[k]];
-    }
-  if (order != BT_ORDER_INPUT && order != BT_ORDER_TREE && order != BT_ORDER_PH)
-    return fail(c, BT_ERR_ARG, "bad order");
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bt_get_acc(bt_ctx *h, float *acc4, int order) {
+  if (!h || !acc4) return BT_ERR_ARG;
+  return get_acc_t(h->c, acc4, order);
+}
+
+int bt_get_acc_f64(bt_ctx *h, double *acc4, int order) {
+  if (!h || !acc4) return BT_ERR_ARG;
+  return get_acc_t(h->c, acc4, order);
+}
+
+int bt_get_particles(bt_ctx *h, float *pos4, float *vel4, int order) {
+  if (!h) return BT_ERR_ARG;
+  return get_particles_t(h->c, pos4, vel4, order);
+}
+
+int bt_get_particles_f64(bt_ctx *h, double *pos4, double *vel4, int order) {
+  if (!h) return BT_ERR_ARG;
+  return get_particles_t(h->c, pos4, vel4, order);
 }
 
 int bt_get_walk_stats(bt_ctx *h, double stats[4], double totals2[2]) {
@@ -799,6 +878,7 @@ int bt_get_tree(bt_ctx *h, uint32_t *cells4, int *level_range64, float *size4, f
   if (!h) return BT_ERR_ARG;
   Ctx &c = h->c;
   if (!c.have_tree) return fail(c, BT_ERR_STATE, "no tree");
+  if ((size4 || mono4 || q0_4 || q1_2) && check_real<float>(c)) return BT_ERR_STATE;
   if (read_counters(c)) return BT_ERR_HIP;
   const size_t nc = (size_t)c.hc.ncells;
   if (cells4) BT_CK(hipMemcpy(cells4, c.cells, sizeof(uint4) * nc, hipMemcpyDeviceToHost));
@@ -813,9 +893,24 @@ int bt_get_tree(bt_ctx *h, uint32_t *cells4, int *level_range64, float *size4, f
   return 0;
 }
 
+int bt_get_tree_f64(bt_ctx *h, double *size4, double *mono4, double *q0_4, double *q1_2) {
+  if (!h) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  if (check_real<double>(c)) return BT_ERR_STATE;
+  if (!c.have_mp) return fail(c, BT_ERR_STATE, "no multipoles");
+  if (read_counters(c)) return BT_ERR_HIP;
+  const size_t nc = (size_t)c.hc.ncells;
+  if (size4) BT_CK(hipMemcpy(size4, c.csize, sizeof(double4) * nc, hipMemcpyDeviceToHost));
+  if (mono4) BT_CK(hipMemcpy(mono4, c.cmono, sizeof(double4) * nc, hipMemcpyDeviceToHost));
+  if (q0_4) BT_CK(hipMemcpy(q0_4, c.cq0, sizeof(double4) * nc, hipMemcpyDeviceToHost));
+  if (q1_2) BT_CK(hipMemcpy(q1_2, c.cq1, sizeof(double2) * nc, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 int bt_get_cell_ls(bt_ctx *h, float *ls2) {
   if (!h || !ls2) return BT_ERR_ARG;
   Ctx &c = h->c;
+  if (check_real<float>(c)) return BT_ERR_STATE;
   if (!c.have_mp) return fail(c, BT_ERR_STATE, "no multipoles");
   if (read_counters(c)) return BT_ERR_HIP;
   BT_CK(hipMemcpy(ls2, c.cls, sizeof(float2) * (size_t)c.hc.ncells, hipMemcpyDeviceToHost));
@@ -869,8 +964,18 @@ int bt_get_ids(bt_ctx *h, int *ids, int order) {
 int bt_get_boxes(bt_ctx *h, float *boxes4) {
   if (!h || !boxes4) return BT_ERR_ARG;
   Ctx &c = h->c;
+  if (check_real<float>(c)) return BT_ERR_STATE;
   if (!c.have_tree) return fail(c, BT_ERR_STATE, "no tree");
   BT_CK(hipMemcpy(boxes4, c.box, sizeof(float4) * c.nsys, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int bt_get_boxes_f64(bt_ctx *h, double *boxes4) {
+  if (!h || !boxes4) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  if (check_real<double>(c)) return BT_ERR_STATE;
+  if (!c.have_tree) return fail(c, BT_ERR_STATE, "no tree");
+  BT_CK(hipMemcpy(boxes4, c.box, sizeof(double4) * c.nsys, hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -925,18 +1030,19 @@ int device_buffer(Ctx &c, int which, void **ptr, int64_t *count, size_t *elem) {
   void *p = nullptr;
   int64_t k = 0;
   size_t e = 0;
+  const size_t r4 = sizeof(float4) * rw(c), r2 = sizeof(float2) * rw(c);
   switch (which) {
-    case BT_BUF_PTCL_POS: p = c.pos; k = n; e = sizeof(float4); break;
-    case BT_BUF_PTCL_VEL: p = c.vel; k = n; e = sizeof(float4); break;
-    case BT_BUF_PTCL_POS_TMP: p = c.pos_ph; k = n; e = sizeof(float4); break;
-    case BT_BUF_PTCL_ACC: p = c.acc; k = n; e = sizeof(float4); break;
-    case BT_BUF_DOMAIN: p = c.box; k = c.nsys; e = sizeof(float4); break;
+    case BT_BUF_PTCL_POS: p = c.pos; k = n; e = r4; break;
+    case BT_BUF_PTCL_VEL: p = c.vel; k = n; e = r4; break;
+    case BT_BUF_PTCL_POS_TMP: p = c.pos_ph; k = n; e = r4; break;
+    case BT_BUF_PTCL_ACC: p = c.acc; k = n; e = r4; break;
+    case BT_BUF_DOMAIN: p = c.box; k = c.nsys; e = r4; break;
     case BT_BUF_LEVEL_BEGIDX: p = c.level_beg; k = kLevels; e = sizeof(int2); break;
     case BT_BUF_CELL_DATA: p = c.cells; k = nc; e = sizeof(uint4); break;
-    case BT_BUF_CELL_SIZE: p = c.csize; k = nc; e = sizeof(float4); break;
-    case BT_BUF_CELL_MONOPOLE: p = c.cmono; k = nc; e = sizeof(float4); break;
-    case BT_BUF_CELL_QUAD0: p = c.cq0; k = nc; e = sizeof(float4); break;
-    case BT_BUF_CELL_QUAD1: p = c.cq1; k = nc; e = sizeof(float2); break;
+    case BT_BUF_CELL_SIZE: p = c.csize; k = nc; e = r4; break;
+    case BT_BUF_CELL_MONOPOLE: p = c.cmono; k = nc; e = r4; break;
+    case BT_BUF_CELL_QUAD0: p = c.cq0; k = nc; e = r4; break;
+    case BT_BUF_CELL_QUAD1: p = c.cq1; k = nc; e = r2; break;
     case BT_BUF_GROUP_LIST: p = c.groups; k = ng; e = sizeof(int2); break;
     case BT_BUF_PH_KEY: p = c.ph_keys; k = n; e = sizeof(uint64_t); break;
     case BT_BUF_PH_PERM: p = c.perm; k = n; e = sizeof(int); break;
@@ -1130,6 +1236,7 @@ int bt_learn_collect(bt_ctx *h, int nsel, const int *gsel, float theta_lo, float
   Ctx &c = h->c;
   c.err.clear();
   if (!(theta_lo > 0.f) || !(theta_hi > theta_lo)) return fail(c, BT_ERR_ARG, "bad theta band");
+  if (c.fp64) return fail(c, BT_ERR_STATE, "training pairs come from Treecode<float> contexts");
   if (!c.have_mp || !c.have_groups) return fail(c, BT_ERR_STATE, "computeMultipoles and makeGroups first");
   if (read_counters(c)) return BT_ERR_HIP;
   const int ng = c.hc.ngroups;
@@ -1215,6 +1322,7 @@ int bt_set_opening(bt_ctx *h, int mode, float budget, float theta_lo, float thet
   Ctx &c = h->c;
   if (mode != 0 && mode != 1) return fail(c, BT_ERR_ARG, "opening mode must be 0 or 1");
   if (mode == 1) {
+    if (c.fp64) return fail(c, BT_ERR_STATE, "the learned angle runs on Treecode<float> only");
     if (!c.have_mlp) return fail(c, BT_ERR_STATE, "no network parameters");
     if (!c.quadrupole) return fail(c, BT_ERR_STATE, "the learned angle needs quadrupoles");
     if (!(budget > 0.f) || !(theta_lo > 0.f) || !(theta_hi > theta_lo))
@@ -1324,7 +1432,7 @@ int bt_step_graph(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit, in
         rc = BT_ERR_HIP;
         break;
       }
-      launch_kick(c, 0.5f * dt);
+      launch_kick(c, 0.5 * (double)dt);
       launch_drift(c, dt);
       rc = step_second_half(c, dt);   // the build swaps the buffers on the host side
       if (hipStreamEndCapture(c.stream, &c.graph[g]) != hipSuccess) rc = BT_ERR_HIP;

@@ -47,22 +47,23 @@ __device__ __forceinline__ uint64_t hilbert_key21(int x, int y, int z) {
   return key;
 }
 
+template <typename R>
 __global__ __launch_bounds__(kBlock) void k_ph_keys(int n, int nsys, const int *__restrict__ off,
-                                                    const float4 *__restrict__ box,
-                                                    const float4 *__restrict__ pos,
+                                                    const V4<R> *__restrict__ box,
+                                                    const V4<R> *__restrict__ pos,
                                                     uint64_t *__restrict__ keys,
                                                     int *__restrict__ vals) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const int r = nsys > 1 ? system_of(i, nsys, off) : 0;
-    const float4 b = box[r];
-    const float4 p = pos[i];
+    const V4<R> b = box[r];
+    const V4<R> p = pos[i];
     int xc, yc, zc;
     {
 #pragma clang fp contract(off)
-      // code/makeGroups.cu:123-131, evaluated left to right in float
-      const float inv = 0.5f / b.w;
-      const float bx = b.x - b.w, by = b.y - b.w, bz = b.z - b.w;
-      const float scale = 2097152.0f;  // 1 << 21
+      // code/makeGroups.cu:123-131, evaluated left to right in real_t
+      const R inv = R(0.5) / b.w;
+      const R bx = b.x - b.w, by = b.y - b.w, bz = b.z - b.w;
+      const R scale = R(2097152);  // 1 << 21
       xc = (int)((p.x - bx) * inv * scale);
       yc = (int)((p.y - by) * inv * scale);
       zc = (int)((p.z - bz) * inv * scale);
@@ -72,10 +73,11 @@ __global__ __launch_bounds__(kBlock) void k_ph_keys(int n, int nsys, const int *
   }
 }
 
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gather_ph(int n, const int *__restrict__ perm,
-                                                      const float4 *__restrict__ pos,
+                                                      const T *__restrict__ pos,
                                                       const int *__restrict__ id,
-                                                      float4 *__restrict__ pos_ph,
+                                                      T *__restrict__ pos_ph,
                                                       int *__restrict__ id_ph) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const int j = perm[i];
@@ -136,13 +138,23 @@ __global__ __launch_bounds__(kBlock) void k_group_sizes(int n, int cap, int2 *__
 }  // namespace
 
 void launch_ph_keys(Ctx &c) {
-  hipLaunchKernelGGL(k_ph_keys, dim3(grid_for(c.n, kBlock, 8192)), dim3(kBlock), 0, c.stream,
-                     c.n, c.nsys, c.d_off, c.box, c.pos, c.ph_keys, c.perm);
+  const dim3 g(grid_for(c.n, kBlock, 8192));
+  if (c.fp64)
+    hipLaunchKernelGGL(k_ph_keys<double>, g, dim3(kBlock), 0, c.stream, c.n, c.nsys, c.d_off,
+                       as4<double>(c.box), as4<double>(c.pos), c.ph_keys, c.perm);
+  else
+    hipLaunchKernelGGL(k_ph_keys<float>, g, dim3(kBlock), 0, c.stream, c.n, c.nsys, c.d_off, c.box,
+                       c.pos, c.ph_keys, c.perm);
 }
 
 void launch_gather_ph(Ctx &c) {
-  hipLaunchKernelGGL(k_gather_ph, dim3(grid_for(c.n, kBlock, 8192)), dim3(kBlock), 0, c.stream,
-                     c.n, c.perm, c.pos, c.id, c.pos_ph, c.id_ph);
+  const dim3 g(grid_for(c.n, kBlock, 8192));
+  if (c.fp64)
+    hipLaunchKernelGGL(k_gather_ph<double4>, g, dim3(kBlock), 0, c.stream, c.n, c.perm,
+                       as4<double>(c.pos), c.id, as4<double>(c.pos_ph), c.id_ph);
+  else
+    hipLaunchKernelGGL(k_gather_ph<float4>, g, dim3(kBlock), 0, c.stream, c.n, c.perm, c.pos,
+                       c.id, c.pos_ph, c.id_ph);
 }
 
 // A new set of groups starts without the group error bit.
@@ -183,10 +195,11 @@ __global__ __launch_bounds__(kBlock) void k_tree_group_starts(int cap_cells, int
 }
 
 // The group order of leaf groups is the tree order itself.
-__global__ __launch_bounds__(kBlock) void k_identity_order(int n, const float4 *__restrict__ pos,
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_identity_order(int n, const T *__restrict__ pos,
                                                            const int *__restrict__ id,
                                                            int *__restrict__ perm,
-                                                           float4 *__restrict__ pos_g,
+                                                           T *__restrict__ pos_g,
                                                            int *__restrict__ id_g) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     perm[i] = i;
@@ -198,8 +211,12 @@ __global__ __launch_bounds__(kBlock) void k_identity_order(int n, const float4 *
 void launch_leaf_groups(Ctx &c) {
   const int g = grid_for(c.n, kBlock, 8192);
   hipLaunchKernelGGL(k_groups_init, dim3(1), dim3(64), 0, c.stream, c.dc);
-  hipLaunchKernelGGL(k_identity_order, dim3(g), dim3(kBlock), 0, c.stream, c.n, c.pos, c.id,
-                     c.perm, c.pos_ph, c.id_ph);
+  if (c.fp64)
+    hipLaunchKernelGGL(k_identity_order<double4>, dim3(g), dim3(kBlock), 0, c.stream, c.n,
+                       as4<double>(c.pos), c.id, c.perm, as4<double>(c.pos_ph), c.id_ph);
+  else
+    hipLaunchKernelGGL(k_identity_order<float4>, dim3(g), dim3(kBlock), 0, c.stream, c.n, c.pos,
+                       c.id, c.perm, c.pos_ph, c.id_ph);
   fill_int(c, c.gtmp0, 0, c.n);
   hipLaunchKernelGGL(k_tree_group_starts, dim3(grid_for(c.cap_cells, kBlock, 8192)), dim3(kBlock),
                      0, c.stream, c.cap_cells, c.group_mode == 2 ? c.ncrit : 0, c.cells, c.gtmp0,

@@ -29,16 +29,17 @@ constexpr int kDirectTargets = kBlock;         // targets per block of the direc
 
 // Both integrator kernels do nothing once an error is on record (err_sticky):
 // a step whose build or walk failed does not kick with invalid forces.
+template <typename R>
 __global__ __launch_bounds__(kBlock) void k_kick(int n, const int *__restrict__ perm,
-                                                 const float4 *__restrict__ acc,
-                                                 float4 *__restrict__ vel, float dth,
+                                                 const V4<R> *__restrict__ acc,
+                                                 V4<R> *__restrict__ vel, R dth,
                                                  DevCounters *dc, int closing) {
   if (dc->err_sticky) return;
   if (closing && blockIdx.x == 0 && threadIdx.x == 0) dc->steps_done++;
   for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
     const int t = perm[k];
-    const float4 a = acc[k];
-    float4 v = vel[t];
+    const V4<R> a = acc[k];
+    V4<R> v = vel[t];
     v.x += a.x * dth;
     v.y += a.y * dth;
     v.z += a.z * dth;
@@ -46,13 +47,14 @@ __global__ __launch_bounds__(kBlock) void k_kick(int n, const int *__restrict__ 
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_drift(int n, const float4 *__restrict__ vel,
-                                                  float4 *__restrict__ pos, float dt,
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_drift(int n, const V4<R> *__restrict__ vel,
+                                                  V4<R> *__restrict__ pos, R dt,
                                                   const DevCounters *dc) {
   if (dc->err_sticky) return;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const float4 v = vel[i];
-    float4 p = pos[i];
+    const V4<R> v = vel[i];
+    V4<R> p = pos[i];
     p.x += v.x * dt;
     p.y += v.y * dt;
     p.z += v.z * dt;
@@ -81,16 +83,17 @@ __device__ __forceinline__ double block_sum_d(double v, double *sh) {
 }
 
 // grid (kEnergyBlocks, nsys): partial[(r*B + b)*2 + {0,1}] = (K, W) partial sums
+template <typename R>
 __global__ __launch_bounds__(kBlock) void k_energy_partial(
-    const int *__restrict__ off, const float4 *__restrict__ pos, const float4 *__restrict__ vel,
-    const float4 *__restrict__ pos_ph, const float4 *__restrict__ acc, double inv_eps,
+    const int *__restrict__ off, const V4<R> *__restrict__ pos, const V4<R> *__restrict__ vel,
+    const V4<R> *__restrict__ pos_ph, const V4<R> *__restrict__ acc, double inv_eps,
     double *__restrict__ part) {
   __shared__ double sh[4];
   const int r = blockIdx.y;
   const int b = off[r], e = off[r + 1];
   double k = 0.0, w = 0.0;
   for (int i = b + blockIdx.x * kBlock + threadIdx.x; i < e; i += gridDim.x * kBlock) {
-    const float4 p = pos[i], v = vel[i];
+    const V4<R> p = pos[i], v = vel[i];
     const double m = p.w;
     k += 0.5 * m * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
     const double mp = pos_ph[i].w;
@@ -136,10 +139,10 @@ __device__ __forceinline__ double rsqrt_fp64(double r2) {
 // issue (18 FP64 instructions per pair), not by the LDS broadcasts:
 // two targets per thread, halving the LDS reads, measured no faster.
 // SOFT: eps > 0, so r2 > 0 for every pair and 1/sqrt needs no clamp.
-template <int NEWTON, bool FOREST, bool SOFT>
+template <int NEWTON, bool FOREST, bool SOFT, typename R>
 __global__ __launch_bounds__(kBlock) void k_direct(int ns, const int *__restrict__ idx, int n,
                                                    int nsys, const int *__restrict__ off,
-                                                   const float4 *__restrict__ pos, double eps2,
+                                                   const V4<R> *__restrict__ pos, double eps2,
                                                    double4 *__restrict__ part) {
   __shared__ double4 tile[kBlock];
   const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -147,14 +150,14 @@ __global__ __launch_bounds__(kBlock) void k_direct(int ns, const int *__restrict
   const int k = active ? idx[t] : 0;
   const int r = FOREST ? system_of(k, nsys, off) : 0;
   const int lo = FOREST ? off[r] : 0, hi = FOREST ? off[r + 1] : n;
-  const float4 pt = pos[k];
+  const V4<R> pt = pos[k];
   const double px = pt.x, py = pt.y, pz = pt.z;
   const int chunk = (n + gridDim.y - 1) / gridDim.y;
   const int s0 = blockIdx.y * chunk, s1 = min(n, s0 + chunk);
   double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
   for (int j0 = s0; j0 < s1; j0 += kBlock) {
     const int j = j0 + threadIdx.x;
-    const float4 pj = j < s1 ? pos[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const V4<R> pj = j < s1 ? pos[j] : make_v4<R>(R(0), R(0), R(0), R(0));
     tile[threadIdx.x] = make_double4(pj.x, pj.y, pj.z, pj.w);
     __syncthreads();
     const int cnt = min(kBlock, s1 - j0);
@@ -196,14 +199,24 @@ __global__ __launch_bounds__(kBlock) void k_direct_reduce(int ns, int nslices,
 
 }  // namespace
 
-void launch_kick(Ctx &c, float dth, bool closing) {
-  hipLaunchKernelGGL(k_kick, dim3(grid_for(c.n, kBlock, 8192)), dim3(kBlock), 0, c.stream, c.n,
-                     c.perm, c.acc, c.vel, dth, c.dc, closing ? 1 : 0);
+void launch_kick(Ctx &c, double dth, bool closing) {
+  const dim3 g(grid_for(c.n, kBlock, 8192));
+  if (c.fp64)
+    hipLaunchKernelGGL(k_kick<double>, g, dim3(kBlock), 0, c.stream, c.n, c.perm,
+                       as4<double>(c.acc), as4<double>(c.vel), dth, c.dc, closing ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_kick<float>, g, dim3(kBlock), 0, c.stream, c.n, c.perm, c.acc, c.vel,
+                       (float)dth, c.dc, closing ? 1 : 0);
 }
 
-void launch_drift(Ctx &c, float dt) {
-  hipLaunchKernelGGL(k_drift, dim3(grid_for(c.n, kBlock, 8192)), dim3(kBlock), 0, c.stream, c.n,
-                     c.vel, c.pos, dt, c.dc);
+void launch_drift(Ctx &c, double dt) {
+  const dim3 g(grid_for(c.n, kBlock, 8192));
+  if (c.fp64)
+    hipLaunchKernelGGL(k_drift<double>, g, dim3(kBlock), 0, c.stream, c.n, as4<double>(c.vel),
+                       as4<double>(c.pos), dt, c.dc);
+  else
+    hipLaunchKernelGGL(k_drift<float>, g, dim3(kBlock), 0, c.stream, c.n, c.vel, c.pos, (float)dt,
+                       c.dc);
 }
 
 void launch_clear_sticky(Ctx &c) {
@@ -211,10 +224,15 @@ void launch_clear_sticky(Ctx &c) {
 }
 
 void launch_energies(Ctx &c) {
-  const double inv_eps = c.eps > 0.f ? 1.0 / (double)c.eps : 0.0;
+  const double inv_eps = c.eps > 0.0 ? 1.0 / c.eps : 0.0;
   double *part = c.d_energy + 2 * c.nsys;
-  hipLaunchKernelGGL(k_energy_partial, dim3(kEnergyBlocks, c.nsys), dim3(kBlock), 0, c.stream,
-                     c.d_off, c.pos, c.vel, c.pos_ph, c.acc, inv_eps, part);
+  if (c.fp64)
+    hipLaunchKernelGGL(k_energy_partial<double>, dim3(kEnergyBlocks, c.nsys), dim3(kBlock), 0,
+                       c.stream, c.d_off, as4<double>(c.pos), as4<double>(c.vel),
+                       as4<double>(c.pos_ph), as4<double>(c.acc), inv_eps, part);
+  else
+    hipLaunchKernelGGL(k_energy_partial<float>, dim3(kEnergyBlocks, c.nsys), dim3(kBlock), 0,
+                       c.stream, c.d_off, c.pos, c.vel, c.pos_ph, c.acc, inv_eps, part);
   hipLaunchKernelGGL(k_energy_final, dim3(grid_for(c.nsys, 64, 1024)), dim3(64), 0, c.stream,
                      c.nsys, kEnergyBlocks, part, c.d_energy);
 }
@@ -229,21 +247,32 @@ int direct_slices(int ns) {
 // d_out must hold ns * (direct_slices(ns) + 1) double4: the result, then the partials.
 void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out, bool check) {
   double4 *part = reinterpret_cast<double4 *>(d_out) + ns;
-  const double eps2 = (double)c.eps * (double)c.eps;
+  // the softening the step uses: float(eps) squared in FP64 (Treecode<float>),
+  // the FP64 eps squared (Treecode<double>)
+  const double eps2 = c.eps * c.eps;
   const int slices = direct_slices(ns);
   // the per-step check of a learned walk needs relative errors of ~1e-6:
   // one Newton step (~3e-14) is ample; the measurement path keeps two
   const dim3 grid((ns + kDirectTargets - 1) / kDirectTargets, slices);
-  void (*kern)(int, const int *, int, int, const int *, const float4 *, double, double4 *);
-  if (eps2 > 0.0) {
-    if (c.nsys > 1) kern = check ? k_direct<1, true, true> : k_direct<2, true, true>;
-    else kern = check ? k_direct<1, false, true> : k_direct<2, false, true>;
+  const bool soft = eps2 > 0.0, forest = c.nsys > 1;
+  if (c.fp64) {
+    void (*kern)(int, const int *, int, int, const int *, const double4 *, double, double4 *);
+    if (soft) kern = forest ? k_direct<2, true, true, double> : k_direct<2, false, true, double>;
+    else kern = forest ? k_direct<2, true, false, double> : k_direct<2, false, false, double>;
+    hipLaunchKernelGGL(kern, grid, dim3(kBlock), 0, c.stream, ns, d_idx, c.n, c.nsys, c.d_off,
+                       as4<double>(c.pos_ph), eps2, part);
   } else {
-    if (c.nsys > 1) kern = check ? k_direct<1, true, false> : k_direct<2, true, false>;
-    else kern = check ? k_direct<1, false, false> : k_direct<2, false, false>;
+    void (*kern)(int, const int *, int, int, const int *, const float4 *, double, double4 *);
+    if (soft) {
+      if (forest) kern = check ? k_direct<1, true, true, float> : k_direct<2, true, true, float>;
+      else kern = check ? k_direct<1, false, true, float> : k_direct<2, false, true, float>;
+    } else {
+      if (forest) kern = check ? k_direct<1, true, false, float> : k_direct<2, true, false, float>;
+      else kern = check ? k_direct<1, false, false, float> : k_direct<2, false, false, float>;
+    }
+    hipLaunchKernelGGL(kern, grid, dim3(kBlock), 0, c.stream, ns, d_idx, c.n, c.nsys, c.d_off,
+                       c.pos_ph, eps2, part);
   }
-  hipLaunchKernelGGL(kern, grid, dim3(kBlock), 0, c.stream, ns, d_idx, c.n, c.nsys, c.d_off,
-                     c.pos_ph, eps2, part);
   const int tpb = kBlock / kWave;   // targets per reduce block
   hipLaunchKernelGGL(k_direct_reduce, dim3((ns + tpb - 1) / tpb), dim3(kBlock), 0, c.stream, ns,
                      slices, part, reinterpret_cast<double4 *>(d_out));

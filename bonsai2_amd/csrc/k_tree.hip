@@ -38,93 +38,114 @@ constexpr int kBlock = 256;
 
 // ---- root box -----------------------------------------------------------
 
-__device__ __forceinline__ void block_minmax(float3 &mn, float3 &mx, float3 *sh) {
-  mn.x = wave_min(mn.x); mn.y = wave_min(mn.y); mn.z = wave_min(mn.z);
-  mx.x = wave_max(mx.x); mx.y = wave_max(mx.y); mx.z = wave_max(mx.z);
+template <typename R>
+struct Ext {
+  R nx, ny, nz, xx, xy, xz;   // min and max per axis
+};
+
+template <typename R>
+__device__ __forceinline__ void ext_add(Ext<R> &e, R x, R y, R z) {
+  e.nx = rmin(e.nx, x); e.ny = rmin(e.ny, y); e.nz = rmin(e.nz, z);
+  e.xx = rmax(e.xx, x); e.xy = rmax(e.xy, y); e.xz = rmax(e.xz, z);
+}
+template <typename R>
+__device__ __forceinline__ Ext<R> ext_empty() {
+  return Ext<R>{R(1e10), R(1e10), R(1e10), R(-1e10), R(-1e10), R(-1e10)};
+}
+
+// block-wide min/max; the block's result in thread 0
+template <typename R>
+__device__ __forceinline__ void block_minmax(Ext<R> &e, Ext<R> *sh) {
+  e.nx = wave_min_r(e.nx); e.ny = wave_min_r(e.ny); e.nz = wave_min_r(e.nz);
+  e.xx = wave_max_r(e.xx); e.xy = wave_max_r(e.xy); e.xz = wave_max_r(e.xz);
   const int w = threadIdx.x >> 6;
-  if (lane_id() == 0) { sh[2 * w] = mn; sh[2 * w + 1] = mx; }
+  if (lane_id() == 0) sh[w] = e;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
-      mn.x = fminf(mn.x, sh[2 * k].x); mn.y = fminf(mn.y, sh[2 * k].y);
-      mn.z = fminf(mn.z, sh[2 * k].z);
-      mx.x = fmaxf(mx.x, sh[2 * k + 1].x); mx.y = fmaxf(mx.y, sh[2 * k + 1].y);
-      mx.z = fmaxf(mx.z, sh[2 * k + 1].z);
+      ext_add(e, sh[k].nx, sh[k].ny, sh[k].nz);
+      ext_add(e, sh[k].xx, sh[k].xy, sh[k].xz);
     }
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bbox_partial(const float4 *__restrict__ pos,
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_bbox_partial(const V4<R> *__restrict__ pos,
                                                          const int *__restrict__ off, int nb,
-                                                         float4 *__restrict__ part) {
-  __shared__ float3 sh[8];
+                                                         V4<R> *__restrict__ part) {
+  __shared__ Ext<R> sh[kBlock / kWave];
   const int r = blockIdx.y;
   const int b = off[r], e = off[r + 1];
-  float3 mn = make_float3(1e10f, 1e10f, 1e10f), mx = make_float3(-1e10f, -1e10f, -1e10f);
+  Ext<R> x = ext_empty<R>();
   for (int i = b + blockIdx.x * kBlock + threadIdx.x; i < e; i += nb * kBlock) {
-    const float4 p = pos[i];
-    mn.x = fminf(mn.x, p.x); mn.y = fminf(mn.y, p.y); mn.z = fminf(mn.z, p.z);
-    mx.x = fmaxf(mx.x, p.x); mx.y = fmaxf(mx.y, p.y); mx.z = fmaxf(mx.z, p.z);
+    const V4<R> p = pos[i];
+    ext_add(x, p.x, p.y, p.z);
   }
-  block_minmax(mn, mx, sh);
+  block_minmax(x, sh);
   if (threadIdx.x == 0) {
-    part[2 * (r * nb + blockIdx.x)] = make_float4(mn.x, mn.y, mn.z, 0.f);
-    part[2 * (r * nb + blockIdx.x) + 1] = make_float4(mx.x, mx.y, mx.z, 0.f);
+    part[2 * (r * nb + blockIdx.x)] = make_v4<R>(x.nx, x.ny, x.nz, R(0));
+    part[2 * (r * nb + blockIdx.x) + 1] = make_v4<R>(x.xx, x.xy, x.xz, R(0));
   }
 }
 
-// Box rules of code/buildTree.cu:145-161, same float operations.
-__device__ float4 box_from_extent(float3 mn, float3 mx) {
+// Box rules of code/buildTree.cu:145-161, same operations in real_t.  (The
+// reference's double instantiation would round the extremes to float in its
+// float2 block reduction, code/buildTree.cu:102-105; they stay double here.)
+template <typename R>
+__device__ V4<R> box_from_extent(const Ext<R> &e) {
 #pragma clang fp contract(off)
-  const float cx = (mx.x + mn.x) * 0.5f, cy = (mx.y + mn.y) * 0.5f, cz = (mx.z + mn.z) * 0.5f;
-  const float hx = (mx.x - mn.x) * 0.5f, hy = (mx.y - mn.y) * 0.5f, hz = (mx.z - mn.z) * 0.5f;
-  const float h = fmaxf(hz, fmaxf(hy, hx));
-  float hs = 1.0f;
-  if (h > 0.0f) {
-    while (hs > h) hs *= 0.5f;
-    while (hs < h) hs *= 2.0f;
+  const R cx = (e.xx + e.nx) * R(0.5), cy = (e.xy + e.ny) * R(0.5), cz = (e.xz + e.nz) * R(0.5);
+  const R hx = (e.xx - e.nx) * R(0.5), hy = (e.xy - e.ny) * R(0.5), hz = (e.xz - e.nz) * R(0.5);
+  const R h = rmax(hz, rmax(hy, hx));
+  R hs = R(1);
+  if (h > R(0)) {
+    while (hs > h) hs *= R(0.5);
+    while (hs < h) hs *= R(2);
   }
-  const float hq = hs / 1048576.0f;
+  const R hq = hs / R(1048576);
   const long long qx = (long long)(cx / hq), qy = (long long)(cy / hq), qz = (long long)(cz / hq);
-  return make_float4(hq * (float)qx, hq * (float)qy, hq * (float)qz, hs);
+  return make_v4<R>(hq * (R)qx, hq * (R)qy, hq * (R)qz, hs);
 }
 
-__global__ __launch_bounds__(kBlock) void k_bbox_final(const float4 *__restrict__ part, int nb,
-                                                       float4 *__restrict__ box) {
-  __shared__ float3 sh[8];
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_bbox_final(const V4<R> *__restrict__ part, int nb,
+                                                       V4<R> *__restrict__ box) {
+  __shared__ Ext<R> sh[kBlock / kWave];
   const int r = blockIdx.x;
-  float3 mn = make_float3(1e10f, 1e10f, 1e10f), mx = make_float3(-1e10f, -1e10f, -1e10f);
+  Ext<R> x = ext_empty<R>();
   for (int k = threadIdx.x; k < nb; k += kBlock) {
-    const float4 a = part[2 * (r * nb + k)], b = part[2 * (r * nb + k) + 1];
-    mn.x = fminf(mn.x, a.x); mn.y = fminf(mn.y, a.y); mn.z = fminf(mn.z, a.z);
-    mx.x = fmaxf(mx.x, b.x); mx.y = fmaxf(mx.y, b.y); mx.z = fmaxf(mx.z, b.z);
+    const V4<R> a = part[2 * (r * nb + k)], b = part[2 * (r * nb + k) + 1];
+    ext_add(x, a.x, a.y, a.z);
+    ext_add(x, b.x, b.y, b.z);
   }
-  block_minmax(mn, mx, sh);
-  if (threadIdx.x == 0) box[r] = box_from_extent(mn, mx);
+  block_minmax(x, sh);
+  if (threadIdx.x == 0) box[r] = box_from_extent(x);
 }
 
 // ---- geometric keys -----------------------------------------------------
 
 // One ChildBox step (code/Particle4.h:138-148): the centre moves by hsize/2.
-__device__ __forceinline__ void child_step(float &cx, float &cy, float &cz, float &h, int o) {
+template <typename R>
+__device__ __forceinline__ void child_step(R &cx, R &cy, R &cz, R &h, int o) {
 #pragma clang fp contract(off)
-  const float s = 0.5f * h;
+  const R s = R(0.5) * h;
   cx = (o & 1) ? cx + s : cx - s;
   cy = (o & 2) ? cy + s : cy - s;
   cz = (o & 4) ? cz + s : cz - s;
   h = s;
 }
 
+template <typename R>
 __global__ __launch_bounds__(kBlock) void k_geo_keys(int n, int nsys, const int *__restrict__ off,
-                                                     const float4 *__restrict__ box,
-                                                     const float4 *__restrict__ pos,
+                                                     const V4<R> *__restrict__ box,
+                                                     const V4<R> *__restrict__ pos,
                                                      uint64_t *__restrict__ keys,
                                                      int *__restrict__ vals) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const int r = nsys > 1 ? system_of(i, nsys, off) : 0;
-    const float4 b = box[r];
-    const float4 p = pos[i];
-    float cx = b.x, cy = b.y, cz = b.z, h = b.w;
+    const V4<R> b = box[r];
+    const V4<R> p = pos[i];
+    R cx = b.x, cy = b.y, cz = b.z, h = b.w;
     uint64_t k = 0;
 #pragma unroll
     for (int l = 0; l < kMaxLevel; l++) {
@@ -138,12 +159,13 @@ __global__ __launch_bounds__(kBlock) void k_geo_keys(int n, int nsys, const int 
   }
 }
 
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gather3(int n, const int *__restrict__ perm,
-                                                    const float4 *__restrict__ a,
-                                                    const float4 *__restrict__ b,
+                                                    const T *__restrict__ a,
+                                                    const T *__restrict__ b,
                                                     const int *__restrict__ c,
-                                                    float4 *__restrict__ a2,
-                                                    float4 *__restrict__ b2,
+                                                    T *__restrict__ a2,
+                                                    T *__restrict__ b2,
                                                     int *__restrict__ c2) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const int j = perm[i];
@@ -387,15 +409,17 @@ __global__ void k_levels_finish(DevCounters *dc, int2 *level_beg) {
 
 // ---- bottom-up multipoles -------------------------------------------------
 
+// T: the real type of particles and cell data (float or double).
+template <typename T>
 struct MpArgs {
   int L;
-  float inv_theta;
-  const float4 *pos;      // tree order
+  T inv_theta;
+  const V4<T> *pos;       // tree order
   const uint4 *cells;
-  float4 *scr;            // 6 per cell: (ref,M) (com-ref,-) (Qxx,Qyy,Qzz,Qxy) (Qxz,Qyz,-,-) bmin bmax
-  float4 *size, *mono, *q0;
-  float2 *q1, *ls;
-  float4 *wrec;           // 3 per cell, the walk's P2C record
+  V4<T> *scr;             // 6 per cell: (ref,M) (com-ref,-) (Qxx,Qyy,Qzz,Qxy) (Qxz,Qyz,-,-) bmin bmax
+  V4<T> *size, *mono, *q0;
+  V2<T> *q1, *ls;
+  V4<T> *wrec;            // 3 per cell, the walk's P2C record
   const DevCounters *dc;
 };
 
@@ -404,65 +428,67 @@ struct Moments {
   R M, sx, sy, sz, xx, yy, zz, xy, xz, yz;
 };
 
-__device__ void write_cell(const MpArgs &a, int c, float rx, float ry, float rz, double M,
-                           double dx, double dy, double dz, double qxx, double qyy, double qzz,
-                           double qxy, double qxz, double qyz, float3 mn, float3 mx) {
+template <typename T>
+__device__ void write_cell(const MpArgs<T> &a, int c, T rx, T ry, T rz, double M, double dx,
+                           double dy, double dz, double qxx, double qyy, double qzz, double qxy,
+                           double qxz, double qyz, const Ext<T> &e) {
 #pragma clang fp contract(off)
-  const float comx = (float)((double)rx + dx), comy = (float)((double)ry + dy),
-              comz = (float)((double)rz + dz);
-  const float bcx = (mx.x + mn.x) * 0.5f, bcy = (mx.y + mn.y) * 0.5f, bcz = (mx.z + mn.z) * 0.5f;
-  const float hx = (mx.x - mn.x) * 0.5f, hy = (mx.y - mn.y) * 0.5f, hz = (mx.z - mn.z) * 0.5f;
-  const float ex = bcx - comx, ey = bcy - comy, ez = bcz - comz;
-  const float s = sqrtf(ex * ex + ey * ey + ez * ez);
-  const float l = fmaxf(2.0f * fmaxf(hx, fmaxf(hy, hz)), 1.0e-6f);
-  const float op = l * a.inv_theta + s;
-  a.size[c] = make_float4(comx, comy, comz, op * op);
-  a.mono[c] = make_float4(comx, comy, comz, (float)M);
-  a.q0[c] = make_float4((float)qxx, (float)qyy, (float)qzz, (float)qxy);
-  a.q1[c] = make_float2((float)qxz, (float)qyz);
-  a.ls[c] = make_float2(l, s);
+  const T comx = (T)((double)rx + dx), comy = (T)((double)ry + dy), comz = (T)((double)rz + dz);
+  const T bcx = (e.xx + e.nx) * T(0.5), bcy = (e.xy + e.ny) * T(0.5), bcz = (e.xz + e.nz) * T(0.5);
+  const T hx = (e.xx - e.nx) * T(0.5), hy = (e.xy - e.ny) * T(0.5), hz = (e.xz - e.nz) * T(0.5);
+  const T ex = bcx - comx, ey = bcy - comy, ez = bcz - comz;
+  T s;
+  if constexpr (sizeof(T) == 4) s = sqrtf(ex * ex + ey * ey + ez * ez);
+  else s = sqrt(ex * ex + ey * ey + ez * ez);
+  const T l = rmax(T(2) * rmax(hx, rmax(hy, hz)), T(1.0e-6));
+  const T op = l * a.inv_theta + s;
+  a.size[c] = make_v4<T>(comx, comy, comz, op * op);
+  a.mono[c] = make_v4<T>(comx, comy, comz, (T)M);
+  a.q0[c] = make_v4<T>((T)qxx, (T)qyy, (T)qzz, (T)qxy);
+  a.q1[c] = make_v2<T>((T)qxz, (T)qyz);
+  a.ls[c] = make_v2<T>(l, s);
   // walk record: quadrupole pre-multiplied by 3, trace terms precomputed
-  const float fqxx = (float)qxx, fqyy = (float)qyy, fqzz = (float)qzz;
-  const float tr = fqxx + fqyy + fqzz;
-  a.wrec[3 * (size_t)c] = make_float4(comx, comy, comz, (float)M);
-  a.wrec[3 * (size_t)c + 1] = make_float4(3.0f * fqxx, 3.0f * fqyy, 3.0f * fqzz, 3.0f * (float)qxy);
+  const T fqxx = (T)qxx, fqyy = (T)qyy, fqzz = (T)qzz;
+  const T tr = fqxx + fqyy + fqzz;
+  a.wrec[3 * (size_t)c] = make_v4<T>(comx, comy, comz, (T)M);
+  a.wrec[3 * (size_t)c + 1] = make_v4<T>(T(3) * fqxx, T(3) * fqyy, T(3) * fqzz, T(3) * (T)qxy);
   a.wrec[3 * (size_t)c + 2] =
-      make_float4(3.0f * (float)qxz, 3.0f * (float)qyz, 0.5f * tr, 1.5f * tr);
-  float4 *s6 = a.scr + 6 * (size_t)c;
-  s6[0] = make_float4(rx, ry, rz, (float)M);
-  s6[1] = make_float4((float)dx, (float)dy, (float)dz, 0.f);
-  s6[2] = make_float4((float)qxx, (float)qyy, (float)qzz, (float)qxy);
-  s6[3] = make_float4((float)qxz, (float)qyz, 0.f, 0.f);
-  s6[4] = make_float4(mn.x, mn.y, mn.z, 0.f);
-  s6[5] = make_float4(mx.x, mx.y, mx.z, 0.f);
+      make_v4<T>(T(3) * (T)qxz, T(3) * (T)qyz, T(0.5) * tr, T(1.5) * tr);
+  V4<T> *s6 = a.scr + 6 * (size_t)c;
+  s6[0] = make_v4<T>(rx, ry, rz, (T)M);
+  s6[1] = make_v4<T>((T)dx, (T)dy, (T)dz, T(0));
+  s6[2] = make_v4<T>((T)qxx, (T)qyy, (T)qzz, (T)qxy);
+  s6[3] = make_v4<T>((T)qxz, (T)qyz, T(0), T(0));
+  s6[4] = make_v4<T>(e.nx, e.ny, e.nz, T(0));
+  s6[5] = make_v4<T>(e.xx, e.xy, e.xz, T(0));
 }
 
-template <typename R>
-__device__ void finish_moments(const MpArgs &a, int c, float rx, float ry, float rz,
-                               const Moments<R> &m, float3 mn, float3 mx) {
+template <typename T, typename R>
+__device__ void finish_moments(const MpArgs<T> &a, int c, T rx, T ry, T rz, const Moments<R> &m,
+                               const Ext<T> &e) {
   if (!(m.M > R(0))) {
-    write_cell(a, c, rx, ry, rz, 0.0, 0, 0, 0, 0, 0, 0, 0, 0, 0, mn, mx);
+    write_cell<T>(a, c, rx, ry, rz, 0.0, 0, 0, 0, 0, 0, 0, 0, 0, 0, e);
     return;
   }
   const R im = R(1) / m.M;
   const R dx = m.sx * im, dy = m.sy * im, dz = m.sz * im;
-  write_cell(a, c, rx, ry, rz, (double)m.M, (double)dx, (double)dy, (double)dz,
-             (double)(m.xx * im - dx * dx), (double)(m.yy * im - dy * dy),
-             (double)(m.zz * im - dz * dz), (double)(m.xy * im - dx * dy),
-             (double)(m.xz * im - dx * dz), (double)(m.yz * im - dy * dz), mn, mx);
+  write_cell<T>(a, c, rx, ry, rz, (double)m.M, (double)dx, (double)dy, (double)dz,
+                (double)(m.xx * im - dx * dx), (double)(m.yy * im - dy * dy),
+                (double)(m.zz * im - dz * dz), (double)(m.xy * im - dx * dy),
+                (double)(m.xz * im - dx * dz), (double)(m.yz * im - dy * dz), e);
 }
 
-template <typename R>
-__device__ void node_moments(const MpArgs &a, int c, const uint4 cd) {
+template <typename T, typename R>
+__device__ void node_moments(const MpArgs<T> &a, int c, const uint4 cd) {
   Moments<R> m{};
-  float3 mn = make_float3(1e10f, 1e10f, 1e10f), mx = make_float3(-1e10f, -1e10f, -1e10f);
+  Ext<T> e = ext_empty<T>();
   const int f = cell_first(cd), nc = cell_nchild(cd);
-  const float4 r0 = a.scr[6 * (size_t)f];  // this node's reference = first child's
+  const V4<T> r0 = a.scr[6 * (size_t)f];  // this node's reference = first child's
   for (int k = 0; k < nc; k++) {
-    const float4 *s6 = a.scr + 6 * (size_t)(f + k);
-    const float4 g = s6[0], d = s6[1], qa = s6[2], qb = s6[3], bmn = s6[4], bmx = s6[5];
+    const V4<T> *s6 = a.scr + 6 * (size_t)(f + k);
+    const V4<T> g = s6[0], d = s6[1], qa = s6[2], qb = s6[3], bmn = s6[4], bmx = s6[5];
     // child com relative to this node's reference: both references lie in the
-    // node's bbox, so their difference is at most l and (near-)exact in float
+    // node's bbox, so their difference is at most l and (near-)exact in T
     const R ox = R(g.x - r0.x) + R(d.x), oy = R(g.y - r0.y) + R(d.y),
             oz = R(g.z - r0.z) + R(d.z);
     const R Mc = R(g.w);
@@ -471,13 +497,16 @@ __device__ void node_moments(const MpArgs &a, int c, const uint4 cd) {
     m.xx += Mc * (R(qa.x) + ox * ox); m.yy += Mc * (R(qa.y) + oy * oy);
     m.zz += Mc * (R(qa.z) + oz * oz); m.xy += Mc * (R(qa.w) + ox * oy);
     m.xz += Mc * (R(qb.x) + ox * oz); m.yz += Mc * (R(qb.y) + oy * oz);
-    mn.x = fminf(mn.x, bmn.x); mn.y = fminf(mn.y, bmn.y); mn.z = fminf(mn.z, bmn.z);
-    mx.x = fmaxf(mx.x, bmx.x); mx.y = fmaxf(mx.y, bmx.y); mx.z = fmaxf(mx.z, bmx.z);
+    ext_add(e, bmn.x, bmn.y, bmn.z);
+    ext_add(e, bmx.x, bmx.y, bmx.z);
   }
-  finish_moments<R>(a, c, r0.x, r0.y, r0.z, m, mn, mx);
+  finish_moments<T, R>(a, c, r0.x, r0.y, r0.z, m, e);
 }
 
-__global__ __launch_bounds__(kBlock) void k_multipole_level(MpArgs a) {
+// Treecode<float>: leaves sum in FP32, nodes combine in FP32 below level
+// kFp64Levels and in FP64 at and above it.  Treecode<double>: FP64 throughout.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_multipole_level(MpArgs<T> a) {
   const int cnt = a.dc->lvl_count[a.L];
   const int c0 = a.dc->lvl_start[a.L];
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < cnt; t += gridDim.x * kBlock) {
@@ -485,13 +514,12 @@ __global__ __launch_bounds__(kBlock) void k_multipole_level(MpArgs a) {
     const uint4 cd = a.cells[c];
     const int pb = (int)cd.z, pe = (int)cd.w;
     if (cell_leaf(cd)) {
-      Moments<float> m{};
-      float3 mn = make_float3(1e10f, 1e10f, 1e10f), mx = make_float3(-1e10f, -1e10f, -1e10f);
-      const float4 r0 = a.pos[pb];  // reference point: the first particle
-      auto add = [&](const float4 p) {
-        mn.x = fminf(mn.x, p.x); mn.y = fminf(mn.y, p.y); mn.z = fminf(mn.z, p.z);
-        mx.x = fmaxf(mx.x, p.x); mx.y = fmaxf(mx.y, p.y); mx.z = fmaxf(mx.z, p.z);
-        const float dx = p.x - r0.x, dy = p.y - r0.y, dz = p.z - r0.z, w = p.w;
+      Moments<T> m{};
+      Ext<T> e = ext_empty<T>();
+      const V4<T> r0 = a.pos[pb];  // reference point: the first particle
+      auto add = [&](const V4<T> p) {
+        ext_add(e, p.x, p.y, p.z);
+        const T dx = p.x - r0.x, dy = p.y - r0.y, dz = p.z - r0.z, w = p.w;
         m.M += w;
         m.sx += w * dx; m.sy += w * dy; m.sz += w * dz;
         m.xx += w * dx * dx; m.yy += w * dy * dy; m.zz += w * dz * dz;
@@ -500,18 +528,18 @@ __global__ __launch_bounds__(kBlock) void k_multipole_level(MpArgs a) {
       // particles in order, four loads issued together
       int i = pb;
       for (; i + 4 <= pe; i += 4) {
-        const float4 p0 = a.pos[i], p1 = a.pos[i + 1], p2 = a.pos[i + 2], p3 = a.pos[i + 3];
+        const V4<T> p0 = a.pos[i], p1 = a.pos[i + 1], p2 = a.pos[i + 2], p3 = a.pos[i + 3];
         add(p0);
         add(p1);
         add(p2);
         add(p3);
       }
       for (; i < pe; i++) add(a.pos[i]);
-      finish_moments<float>(a, c, r0.x, r0.y, r0.z, m, mn, mx);
-    } else if (a.L <= kFp64Levels) {
-      node_moments<double>(a, c, cd);
+      finish_moments<T, T>(a, c, r0.x, r0.y, r0.z, m, e);
+    } else if (sizeof(T) == 8 || a.L <= kFp64Levels) {
+      node_moments<T, double>(a, c, cd);
     } else {
-      node_moments<float>(a, c, cd);
+      node_moments<T, float>(a, c, cd);
     }
   }
 }
@@ -520,20 +548,38 @@ __global__ __launch_bounds__(kBlock) void k_multipole_level(MpArgs a) {
 
 void launch_bbox(Ctx &c) {
   const int nb = c.bbox_blocks;
-  hipLaunchKernelGGL(k_bbox_partial, dim3(nb, c.nsys), dim3(kBlock), 0, c.stream, c.pos, c.d_off,
-                     nb, c.bbox_part);
-  hipLaunchKernelGGL(k_bbox_final, dim3(c.nsys), dim3(kBlock), 0, c.stream, c.bbox_part, nb,
-                     c.box);
+  if (c.fp64) {
+    hipLaunchKernelGGL(k_bbox_partial<double>, dim3(nb, c.nsys), dim3(kBlock), 0, c.stream,
+                       as4<double>(c.pos), c.d_off, nb, as4<double>(c.bbox_part));
+    hipLaunchKernelGGL(k_bbox_final<double>, dim3(c.nsys), dim3(kBlock), 0, c.stream,
+                       as4<double>(c.bbox_part), nb, as4<double>(c.box));
+  } else {
+    hipLaunchKernelGGL(k_bbox_partial<float>, dim3(nb, c.nsys), dim3(kBlock), 0, c.stream, c.pos,
+                       c.d_off, nb, c.bbox_part);
+    hipLaunchKernelGGL(k_bbox_final<float>, dim3(c.nsys), dim3(kBlock), 0, c.stream, c.bbox_part,
+                       nb, c.box);
+  }
 }
 
 void launch_geo_keys(Ctx &c) {
-  hipLaunchKernelGGL(k_geo_keys, dim3(grid_for(c.n, kBlock, 8192)), dim3(kBlock), 0, c.stream,
-                     c.n, c.nsys, c.d_off, c.box, c.pos, c.geo_keys, c.valA);
+  const dim3 g(grid_for(c.n, kBlock, 8192));
+  if (c.fp64)
+    hipLaunchKernelGGL(k_geo_keys<double>, g, dim3(kBlock), 0, c.stream, c.n, c.nsys, c.d_off,
+                       as4<double>(c.box), as4<double>(c.pos), c.geo_keys, c.valA);
+  else
+    hipLaunchKernelGGL(k_geo_keys<float>, g, dim3(kBlock), 0, c.stream, c.n, c.nsys, c.d_off,
+                       c.box, c.pos, c.geo_keys, c.valA);
 }
 
 void launch_gather_tree(Ctx &c) {
-  hipLaunchKernelGGL(k_gather3, dim3(grid_for(c.n, kBlock, 8192)), dim3(kBlock), 0, c.stream,
-                     c.n, c.valA, c.pos, c.vel, c.id, c.pos2, c.vel2, c.id2);
+  const dim3 g(grid_for(c.n, kBlock, 8192));
+  if (c.fp64)
+    hipLaunchKernelGGL(k_gather3<double4>, g, dim3(kBlock), 0, c.stream, c.n, c.valA,
+                       as4<double>(c.pos), as4<double>(c.vel), c.id, as4<double>(c.pos2),
+                       as4<double>(c.vel2), c.id2);
+  else
+    hipLaunchKernelGGL(k_gather3<float4>, g, dim3(kBlock), 0, c.stream, c.n, c.valA, c.pos,
+                       c.vel, c.id, c.pos2, c.vel2, c.id2);
   std::swap(c.pos, c.pos2);
   std::swap(c.vel, c.vel2);
   std::swap(c.id, c.id2);
@@ -562,24 +608,34 @@ void launch_levels(Ctx &c) {
   hipLaunchKernelGGL(k_levels_finish, dim3(1), dim3(64), 0, c.stream, c.dc, c.level_beg);
 }
 
-void launch_multipoles(Ctx &c) {
-  MpArgs a;
-  a.inv_theta = (float)(1.0 / (double)c.theta);
-  a.pos = c.pos;
+namespace {
+
+template <typename T>
+void multipoles_t(Ctx &c) {
+  MpArgs<T> a;
+  a.inv_theta = (T)(1.0 / (double)c.theta);
+  a.pos = as4<T>(c.pos);
   a.cells = c.cells;
-  a.scr = c.cscr;
-  a.size = c.csize;
-  a.mono = c.cmono;
-  a.q0 = c.cq0;
-  a.q1 = c.cq1;
-  a.ls = c.cls;
-  a.wrec = c.cwalk;
+  a.scr = as4<T>(c.cscr);
+  a.size = as4<T>(c.csize);
+  a.mono = as4<T>(c.cmono);
+  a.q0 = as4<T>(c.cq0);
+  a.q1 = as2<T>(c.cq1);
+  a.ls = as2<T>(c.cls);
+  a.wrec = as4<T>(c.cwalk);
   a.dc = c.dc;
   const int grid = grid_for(c.cap_cells, kBlock, 2048);
   for (int L = kMaxLevel; L >= 1; L--) {
     a.L = L;
-    hipLaunchKernelGGL(k_multipole_level, dim3(grid), dim3(kBlock), 0, c.stream, a);
+    hipLaunchKernelGGL(k_multipole_level<T>, dim3(grid), dim3(kBlock), 0, c.stream, a);
   }
+}
+
+}  // namespace
+
+void launch_multipoles(Ctx &c) {
+  if (c.fp64) multipoles_t<double>(c);
+  else multipoles_t<float>(c);
 }
 
 }  // namespace bt

@@ -46,11 +46,17 @@ constexpr int kMaxWalkSteps = 1 << 20;
 #ifndef BT_UNROLL_S
 #define BT_UNROLL_S 8
 #endif
+#ifndef BT_UNROLL_LDS2
+#define BT_UNROLL_LDS2 4
+#endif
 #ifndef BT_WALK_WAVES_PER_EU
 #define BT_WALK_WAVES_PER_EU 1
 #endif
 #ifndef BT_LEARN_WAVES_PER_EU
 #define BT_LEARN_WAVES_PER_EU 4
+#endif
+#ifndef BT_LEARN2_WAVES_PER_EU
+#define BT_LEARN2_WAVES_PER_EU 3   // two i-particles per lane: 4 would spill VGPRs
 #endif
 #ifndef BT_XCD_QUEUE
 #define BT_XCD_QUEUE 1
@@ -82,26 +88,29 @@ __device__ __forceinline__ int next_group(DevCounters *dc, int ngroups, int home
   return ngroups;
 }
 
+// R: the real type of particles, cell data and arithmetic (float for
+// Treecode<float>, double for Treecode<double>).
+template <typename R>
 struct WalkArgs {
   int n, nsys;
-  float eps2;
+  R eps2;
   const int2 *groups;
   const int *group_sys;
   const int2 *root_cells;
-  const float4 *pos_ph;
-  const float4 *pos_tree;
+  const V4<R> *pos_ph;
+  const V4<R> *pos_tree;
   const uint4 *cells;
-  const float4 *csize;
-  const float4 *cmono;
-  const float4 *cq0;
-  const float2 *cq1;
-  const float4 *cwalk;   // 3 per cell: (com, M) (3Qxx,3Qyy,3Qzz,3Qxy) (3Qxz,3Qyz,trQ/2,1.5trQ)
-  // learned opening angle
+  const V4<R> *csize;
+  const V4<R> *cmono;
+  const V4<R> *cq0;
+  const V2<R> *cq1;
+  const V4<R> *cwalk;    // 3 per cell: (com, M) (3Qxx,3Qyy,3Qzz,3Qxy) (3Qxz,3Qyz,trQ/2,1.5trQ)
+  // learned opening angle (Treecode<float> only)
   const float2 *cls;     // (l, s) per cell
   const float2 *sys_scale;
   const MlpDev *mlp;
   float shift, it_min, it_max;
-  float4 *acc;
+  V4<R> *acc;
   int2 *gcounts;
   double *gpot;
   int *stack;
@@ -111,47 +120,62 @@ struct WalkArgs {
   DevCounters *dc;
 };
 
+// The i-particles one lane carries.  IPL = 1: particle `lane` of the group.
+// IPL = 2: particles 2q and 2q + 1 (q = lane & 31), carried by both halves of
+// the wave; each batch of j-entries is split between the half-waves (lanes
+// 0-31 sweep its first half, lanes 32-63 its second), so every broadcast LDS
+// read serves two interactions, and the halves' sums are added once per group.
+template <typename R, int IPL>
+struct IParts {
+  R x[IPL], y[IPL], z[IPL];
+  Acc4T<R> s[IPL], c[IPL];   // Kahan sums and compensations
+};
+
+// LDS slots of one batch: 64 entries; with IPL = 2 the second half-wave's
+// entries start one slot later (its broadcast address differs from the first
+// half's by 132 dwords, not a multiple of the bank count) and one zero-mass
+// slot follows an odd count.
+constexpr int kJSlots = kWave + 2;
+
+template <typename R>
 struct WaveShared {
-  float4 jA[kWave];   // P2C: (com - gc, M)         P2P: (x - gc, m)
-  float4 jB[kWave];   // P2C: (3Qxx, 3Qyy, 3Qzz, 3Qxy)
-  float4 jC[kWave];   // P2C: (3Qxz, 3Qyz, trQ/2, 1.5 trQ)
+  V4<R> jA[kJSlots];    // P2C: (com - gc, M)         P2P: (x - gc, m)
+  V4<R> jB[kJSlots];    // P2C: (3Qxx, 3Qyy, 3Qzz, 3Qxy)
+  V4<R> jC[kJSlots];    // P2C: (3Qxz, 3Qyz, trQ/2, 1.5 trQ)
   int alist[2 * kWave];
   int dlist[2 * kWave];
   int lpre[kWave];
   int lbeg[kWave];
 };
 
-// Improved Barnes-Hut opening test, code/computeForces.cu:30-54 (same float ops).
-__device__ __forceinline__ bool open_cell(const float4 cs, float gcx, float gcy, float gcz,
-                                          float gsx, float gsy, float gsz) {
-#pragma clang fp contract(off)
-  float dx = fabsf(gcx - cs.x) - gsx;
-  float dy = fabsf(gcy - cs.y) - gsy;
-  float dz = fabsf(gcz - cs.z) - gsz;
-  dx += fabsf(dx); dx *= 0.5f;
-  dy += fabsf(dy); dy *= 0.5f;
-  dz += fabsf(dz); dz *= 0.5f;
-  const float ds2 = dx * dx + dy * dy + dz * dz;
-  return ds2 < fabsf(cs.w);
-}
+__device__ __forceinline__ float rabs(float v) { return __builtin_fabsf(v); }
+__device__ __forceinline__ double rabs(double v) { return __builtin_fabs(v); }
 
-// ds^2 of the opening test above.
-__device__ __forceinline__ float box_ds2(const float4 cs, float gcx, float gcy, float gcz,
-                                         float gsx, float gsy, float gsz) {
+// ds^2 of the improved Barnes-Hut opening test, code/computeForces.cu:30-54
+// (same operations in real_t).
+template <typename R>
+__device__ __forceinline__ R box_ds2(const V4<R> cs, R gcx, R gcy, R gcz, R gsx, R gsy, R gsz) {
 #pragma clang fp contract(off)
-  float dx = fabsf(gcx - cs.x) - gsx;
-  float dy = fabsf(gcy - cs.y) - gsy;
-  float dz = fabsf(gcz - cs.z) - gsz;
-  dx += fabsf(dx); dx *= 0.5f;
-  dy += fabsf(dy); dy *= 0.5f;
-  dz += fabsf(dz); dz *= 0.5f;
+  R dx = rabs(gcx - cs.x) - gsx;
+  R dy = rabs(gcy - cs.y) - gsy;
+  R dz = rabs(gcz - cs.z) - gsz;
+  dx += rabs(dx); dx *= R(0.5);
+  dy += rabs(dy); dy *= R(0.5);
+  dz += rabs(dz); dz *= R(0.5);
   return dx * dx + dy * dy + dz * dz;
 }
 
-template <bool KAHAN>
-__device__ __forceinline__ void fold(Acc4 &s, Acc4 &c, const Acc4 &b) {
+// The test itself: open iff ds^2 < cellOp^2.
+template <typename R>
+__device__ __forceinline__ bool open_cell(const V4<R> cs, R gcx, R gcy, R gcz, R gsx, R gsy,
+                                          R gsz) {
+  return box_ds2<R>(cs, gcx, gcy, gcz, gsx, gsy, gsz) < rabs(cs.w);
+}
+
+template <bool KAHAN, typename R>
+__device__ __forceinline__ void fold(Acc4T<R> &s, Acc4T<R> &c, const Acc4T<R> &b) {
   if (KAHAN) {
-    float y, t;
+    R y, t;
     y = b.x - c.x; t = s.x + y; c.x = (t - s.x) - y; s.x = t;
     y = b.y - c.y; t = s.y + y; c.y = (t - s.y) - y; s.y = t;
     y = b.z - c.z; t = s.z + y; c.z = (t - s.z) - y; s.z = t;
@@ -161,75 +185,113 @@ __device__ __forceinline__ void fold(Acc4 &s, Acc4 &c, const Acc4 &b) {
   }
 }
 
-template <bool KAHAN>
-__device__ __forceinline__ void eval_p2c(const WalkArgs &a, WaveShared &S, int cnt, float gcx,
-                                         float gcy, float gcz, float xi, float yi, float zi,
-                                         Acc4 &s, Acc4 &c) {
+// Entries per half-wave of a batch of cnt, and the LDS slot of entry k.
+template <int IPL>
+__device__ __forceinline__ int half_count(int cnt) { return IPL == 2 ? (cnt + 1) >> 1 : cnt; }
+template <int IPL>
+__device__ __forceinline__ int jslot(int k, int hc) { return IPL == 2 ? k + (k >= hc ? 1 : 0) : k; }
+
+// A zero-mass entry far from the group: every term it adds is exactly zero.
+template <typename R>
+__device__ __forceinline__ void stage_pad(WaveShared<R> &S, int slot) {
+  S.jA[slot] = make_v4<R>(R(1.0e18), R(0), R(0), R(0));
+  S.jB[slot] = make_v4<R>(R(0), R(0), R(0), R(0));
+  S.jC[slot] = make_v4<R>(R(0), R(0), R(0), R(0));
+}
+
+// Sweeps a staged batch: P2C (QUAD) or particle-like (P2P and monopole-only
+// P2C) interactions of the lane's i-particles with its half of the entries.
+template <bool KAHAN, bool QUAD, typename R, int IPL>
+__device__ __forceinline__ void sweep(const WaveShared<R> &S, int cnt, R eps2,
+                                      IParts<R, IPL> &ip) {
   const int lane = lane_id();
-  if (lane < cnt) {
-    const int ci = S.alist[lane];
-    const float4 r0 = a.cwalk[3 * ci];
-    S.jA[lane] = make_float4(r0.x - gcx, r0.y - gcy, r0.z - gcz, r0.w);
-    S.jB[lane] = a.cwalk[3 * ci + 1];
-    S.jC[lane] = a.cwalk[3 * ci + 2];
-  }
-  wave_lds_sync();
-  Acc4 b = {0.f, 0.f, 0.f, 0.f};
-  if (cnt == kWave) {
+  Acc4T<R> b[IPL];
+#pragma unroll
+  for (int k = 0; k < IPL; k++) b[k] = Acc4T<R>{R(0), R(0), R(0), R(0)};
+  auto one = [&](int slot) {
+    const V4<R> A = S.jA[slot];
+    if (QUAD) {
+      const V4<R> B = S.jB[slot], C = S.jC[slot];
+#pragma unroll
+      for (int k = 0; k < IPL; k++) p2c_rec(b[k], ip.x[k], ip.y[k], ip.z[k], A, B, C, eps2);
+    } else {
+#pragma unroll
+      for (int k = 0; k < IPL; k++) p2p(b[k], ip.x[k], ip.y[k], ip.z[k], A, eps2);
+    }
+  };
+  if (IPL == 2) {
+    const int hc = half_count<2>(cnt);
+    const int base = lane >= 32 ? hc + 1 : 0;
+    if (cnt == kWave) {
+#pragma unroll BT_UNROLL_LDS2
+      for (int j = 0; j < kWave / 2; j++) one(base + j);
+    } else {
+#pragma unroll 2
+      for (int j = 0; j < hc; j++) one(base + j);
+    }
+  } else if (cnt == kWave) {
 #pragma unroll BT_UNROLL_LDS
-    for (int j = 0; j < kWave; j++) p2c_rec(b, xi, yi, zi, S.jA[j], S.jB[j], S.jC[j], a.eps2);
+    for (int j = 0; j < kWave; j++) one(j);
   } else {
 #pragma unroll 2
-    for (int j = 0; j < cnt; j++) p2c_rec(b, xi, yi, zi, S.jA[j], S.jB[j], S.jC[j], a.eps2);
+    for (int j = 0; j < cnt; j++) one(j);
   }
-  fold<KAHAN>(s, c, b);
+#pragma unroll
+  for (int k = 0; k < IPL; k++) fold<KAHAN, R>(ip.s[k], ip.c[k], b[k]);
+}
+
+template <bool KAHAN, typename R, int IPL>
+__device__ __forceinline__ void eval_p2c(const WalkArgs<R> &a, WaveShared<R> &S, int cnt, R gcx,
+                                         R gcy, R gcz, IParts<R, IPL> &ip) {
+  const int lane = lane_id();
+  const int hc = half_count<IPL>(cnt);
+  if (lane < cnt) {
+    const int ci = S.alist[lane];
+    const V4<R> r0 = a.cwalk[3 * ci];
+    const int d = jslot<IPL>(lane, hc);
+    S.jA[d] = make_v4<R>(r0.x - gcx, r0.y - gcy, r0.z - gcz, r0.w);
+    S.jB[d] = a.cwalk[3 * ci + 1];
+    S.jC[d] = a.cwalk[3 * ci + 2];
+  } else if (IPL == 2 && lane == cnt) {
+    stage_pad(S, cnt + 1);
+  }
+  wave_lds_sync();
+  sweep<KAHAN, true, R, IPL>(S, cnt, a.eps2, ip);
   wave_lds_sync();
 }
 
-template <bool KAHAN>
-__device__ __forceinline__ void eval_p2p(const WalkArgs &a, WaveShared &S, int cnt, float gcx,
-                                         float gcy, float gcz, float xi, float yi, float zi,
-                                         Acc4 &s, Acc4 &c) {
+template <bool KAHAN, typename R, int IPL>
+__device__ __forceinline__ void eval_p2p(const WalkArgs<R> &a, WaveShared<R> &S, int cnt, R gcx,
+                                         R gcy, R gcz, IParts<R, IPL> &ip) {
   const int lane = lane_id();
+  const int hc = half_count<IPL>(cnt);
   if (lane < cnt) {
-    const float4 p = a.pos_tree[S.dlist[lane]];
-    S.jA[lane] = make_float4(p.x - gcx, p.y - gcy, p.z - gcz, p.w);
+    const V4<R> p = a.pos_tree[S.dlist[lane]];
+    S.jA[jslot<IPL>(lane, hc)] = make_v4<R>(p.x - gcx, p.y - gcy, p.z - gcz, p.w);
+  } else if (IPL == 2 && lane == cnt) {
+    stage_pad(S, cnt + 1);
   }
   wave_lds_sync();
-  Acc4 b = {0.f, 0.f, 0.f, 0.f};
-  if (cnt == kWave) {
-#pragma unroll BT_UNROLL_LDS
-    for (int j = 0; j < kWave; j++) p2p(b, xi, yi, zi, S.jA[j], a.eps2);
-  } else {
-#pragma unroll 2
-    for (int j = 0; j < cnt; j++) p2p(b, xi, yi, zi, S.jA[j], a.eps2);
-  }
-  fold<KAHAN>(s, c, b);
+  sweep<KAHAN, false, R, IPL>(S, cnt, a.eps2, ip);
   wave_lds_sync();
 }
 
 // Monopole-only P2C (the reference built without QUADRUPOLE,
 // code/computeForces.cu:198-225): the cell's mass at its centre of mass,
 // evaluated like a particle (20 flops).
-template <bool KAHAN>
-__device__ __forceinline__ void eval_p2c_mono(const WalkArgs &a, WaveShared &S, int cnt,
-                                              float gcx, float gcy, float gcz, float xi,
-                                              float yi, float zi, Acc4 &s, Acc4 &c) {
+template <bool KAHAN, typename R, int IPL>
+__device__ __forceinline__ void eval_p2c_mono(const WalkArgs<R> &a, WaveShared<R> &S, int cnt,
+                                              R gcx, R gcy, R gcz, IParts<R, IPL> &ip) {
   const int lane = lane_id();
+  const int hc = half_count<IPL>(cnt);
   if (lane < cnt) {
-    const float4 r0 = a.cwalk[3 * S.alist[lane]];
-    S.jA[lane] = make_float4(r0.x - gcx, r0.y - gcy, r0.z - gcz, r0.w);
+    const V4<R> r0 = a.cwalk[3 * S.alist[lane]];
+    S.jA[jslot<IPL>(lane, hc)] = make_v4<R>(r0.x - gcx, r0.y - gcy, r0.z - gcz, r0.w);
+  } else if (IPL == 2 && lane == cnt) {
+    stage_pad(S, cnt + 1);
   }
   wave_lds_sync();
-  Acc4 b = {0.f, 0.f, 0.f, 0.f};
-  if (cnt == kWave) {
-#pragma unroll 8
-    for (int j = 0; j < kWave; j++) p2p(b, xi, yi, zi, S.jA[j], a.eps2);
-  } else {
-#pragma unroll 2
-    for (int j = 0; j < cnt; j++) p2p(b, xi, yi, zi, S.jA[j], a.eps2);
-  }
-  fold<KAHAN>(s, c, b);
+  sweep<KAHAN, false, R, IPL>(S, cnt, a.eps2, ip);
   wave_lds_sync();
 }
 
@@ -244,9 +306,10 @@ __device__ __forceinline__ float4 ld_uniform(const cf4v *p, int i) {
   return make_float4(v[0], v[1], v[2], v[3]);
 }
 
+// Scalar-load j-data (one i-particle per lane, absolute coordinates).
 template <bool KAHAN>
-__device__ __forceinline__ void eval_p2c_s(const WalkArgs &a, WaveShared &S, int cnt, float xi,
-                                           float yi, float zi, Acc4 &s, Acc4 &c) {
+__device__ __forceinline__ void eval_p2c_s(const WalkArgs<float> &a, WaveShared<float> &S,
+                                           int cnt, IParts<float, 1> &ip) {
   const cf4v *rec = (const cf4v *)a.cwalk;
   const int my = S.alist[lane_id()];
   Acc4 b = {0.f, 0.f, 0.f, 0.f};
@@ -254,68 +317,70 @@ __device__ __forceinline__ void eval_p2c_s(const WalkArgs &a, WaveShared &S, int
 #pragma unroll BT_UNROLL_S
     for (int j = 0; j < kWave; j++) {
       const int r = 3 * __builtin_amdgcn_readlane(my, j);
-      p2c_rec(b, xi, yi, zi, ld_uniform(rec, r), ld_uniform(rec, r + 1), ld_uniform(rec, r + 2),
-              a.eps2);
+      p2c_rec(b, ip.x[0], ip.y[0], ip.z[0], ld_uniform(rec, r), ld_uniform(rec, r + 1),
+              ld_uniform(rec, r + 2), a.eps2);
     }
   } else {
 #pragma unroll 2
     for (int j = 0; j < cnt; j++) {
       const int r = 3 * __builtin_amdgcn_readlane(my, j);
-      p2c_rec(b, xi, yi, zi, ld_uniform(rec, r), ld_uniform(rec, r + 1), ld_uniform(rec, r + 2),
-              a.eps2);
+      p2c_rec(b, ip.x[0], ip.y[0], ip.z[0], ld_uniform(rec, r), ld_uniform(rec, r + 1),
+              ld_uniform(rec, r + 2), a.eps2);
     }
   }
-  fold<KAHAN>(s, c, b);
+  fold<KAHAN>(ip.s[0], ip.c[0], b);
 }
 
 template <bool KAHAN>
-__device__ __forceinline__ void eval_p2p_s(const WalkArgs &a, WaveShared &S, int cnt, float xi,
-                                           float yi, float zi, Acc4 &s, Acc4 &c) {
+__device__ __forceinline__ void eval_p2p_s(const WalkArgs<float> &a, WaveShared<float> &S,
+                                           int cnt, IParts<float, 1> &ip) {
   const cf4v *pj = (const cf4v *)a.pos_tree;
   const int my = S.dlist[lane_id()];
   Acc4 b = {0.f, 0.f, 0.f, 0.f};
   if (cnt == kWave) {
 #pragma unroll BT_UNROLL_S
-    for (int j = 0; j < kWave; j++) p2p(b, xi, yi, zi, ld_uniform(pj, __builtin_amdgcn_readlane(my, j)), a.eps2);
+    for (int j = 0; j < kWave; j++)
+      p2p(b, ip.x[0], ip.y[0], ip.z[0], ld_uniform(pj, __builtin_amdgcn_readlane(my, j)), a.eps2);
   } else {
 #pragma unroll 2
-    for (int j = 0; j < cnt; j++) p2p(b, xi, yi, zi, ld_uniform(pj, __builtin_amdgcn_readlane(my, j)), a.eps2);
+    for (int j = 0; j < cnt; j++)
+      p2p(b, ip.x[0], ip.y[0], ip.z[0], ld_uniform(pj, __builtin_amdgcn_readlane(my, j)), a.eps2);
   }
-  fold<KAHAN>(s, c, b);
+  fold<KAHAN>(ip.s[0], ip.c[0], b);
 }
 
 // One batch of cnt P2C (list in S.alist) or P2P (S.dlist) interactions.  The
 // LDS path takes i relative to the group centre, the scalar path absolute i.
-template <bool KAHAN, bool SJ, bool MONO>
-__device__ __forceinline__ void batch_p2c(const WalkArgs &a, WaveShared &S, int cnt, float gcx,
-                                          float gcy, float gcz, float xi, float yi, float zi,
-                                          Acc4 &s, Acc4 &c) {
-  if (MONO) eval_p2c_mono<KAHAN>(a, S, cnt, gcx, gcy, gcz, xi, yi, zi, s, c);
-  else if (SJ) eval_p2c_s<KAHAN>(a, S, cnt, xi, yi, zi, s, c);
-  else eval_p2c<KAHAN>(a, S, cnt, gcx, gcy, gcz, xi, yi, zi, s, c);
+template <bool KAHAN, bool SJ, bool MONO, typename R, int IPL>
+__device__ __forceinline__ void batch_p2c(const WalkArgs<R> &a, WaveShared<R> &S, int cnt, R gcx,
+                                          R gcy, R gcz, IParts<R, IPL> &ip) {
+  if constexpr (MONO) eval_p2c_mono<KAHAN, R, IPL>(a, S, cnt, gcx, gcy, gcz, ip);
+  else if constexpr (SJ) eval_p2c_s<KAHAN>(a, S, cnt, ip);
+  else eval_p2c<KAHAN, R, IPL>(a, S, cnt, gcx, gcy, gcz, ip);
 }
 
-template <bool KAHAN, bool SJ>
-__device__ __forceinline__ void batch_p2p(const WalkArgs &a, WaveShared &S, int cnt, float gcx,
-                                          float gcy, float gcz, float xi, float yi, float zi,
-                                          Acc4 &s, Acc4 &c) {
-  if (SJ) eval_p2p_s<KAHAN>(a, S, cnt, xi, yi, zi, s, c);
-  else eval_p2p<KAHAN>(a, S, cnt, gcx, gcy, gcz, xi, yi, zi, s, c);
+template <bool KAHAN, bool SJ, typename R, int IPL>
+__device__ __forceinline__ void batch_p2p(const WalkArgs<R> &a, WaveShared<R> &S, int cnt, R gcx,
+                                          R gcy, R gcz, IParts<R, IPL> &ip) {
+  if constexpr (SJ) eval_p2p_s<KAHAN>(a, S, cnt, ip);
+  else eval_p2p<KAHAN, R, IPL>(a, S, cnt, gcx, gcy, gcz, ip);
 }
 
 // LEARN: each popped cell gets its own opening angle from the network
 // (bt_learn.h), evaluated by the wave on the matrix cores for up to 64 cells.
 // MONO: monopole-only P2C (the reference's build without QUADRUPOLE).
-template <bool KAHAN, bool SJ, bool LEARN, bool MONO = false>
+template <typename R, bool KAHAN, bool SJ, bool LEARN, bool MONO, int IPL>
 __global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(LEARN ? BT_LEARN_WAVES_PER_EU : BT_WALK_WAVES_PER_EU)))
-void k_walk(WalkArgs a) {
-  __shared__ WaveShared sh[kWavesPerBlock];
+__attribute__((amdgpu_waves_per_eu(LEARN ? (IPL == 2 ? BT_LEARN2_WAVES_PER_EU : BT_LEARN_WAVES_PER_EU)
+                                         : BT_WALK_WAVES_PER_EU)))
+void k_walk(WalkArgs<R> a) {
+  static_assert(!(LEARN || SJ) || sizeof(R) == 4, "learned angle and scalar loads: FP32 only");
+  __shared__ WaveShared<R> sh[kWavesPerBlock];
   __shared__ float s_mlp[LEARN ? 64 : 1];
   __shared__ int s_stack[LEARN ? kWavesPerBlock * kLdsStack : 1];
   const int lane = lane_id();
   const int wv = threadIdx.x >> 6;
-  WaveShared &S = sh[wv];
+  WaveShared<R> &S = sh[wv];
   // LSTK: the top of the stack (up to kLdsStack entries) lives in LDS, the
   // older entries below it in global memory (stk[0, gtop)); otherwise the
   // whole stack is in global memory (stk[0, top))
@@ -324,7 +389,7 @@ void k_walk(WalkArgs a) {
   int *lstk = s_stack + (LSTK ? wv * kLdsStack : 0);
   half8_t ma1, ma2_0, ma2_1;
   float mb3 = 0.f;
-  if (LEARN) {
+  if constexpr (LEARN) {
     if (threadIdx.x < 32) s_mlp[threadIdx.x] = a.mlp->c2p[threadIdx.x >> 4][threadIdx.x & 15];
     else if (threadIdx.x < 64)
       s_mlp[threadIdx.x] = a.mlp->w3p[(threadIdx.x - 32) >> 4][threadIdx.x & 15];
@@ -353,25 +418,41 @@ void k_walk(WalkArgs a) {
     const int2 grp = a.groups[g];
     const int pbeg = grp.x, np = max(grp.y, 1);
     const int sys = a.nsys > 1 ? a.group_sys[g] : 0;
-    const bool valid = lane < np;
-    const float4 pi = a.pos_ph[pbeg + min(lane, np - 1)];
+    // the lane's i-particles (clamped to the last one, :534)
+    V4<R> pi[IPL];
+#pragma unroll
+    for (int k = 0; k < IPL; k++)
+      pi[k] = a.pos_ph[pbeg + min(IPL == 1 ? lane : 2 * (lane & 31) + k, np - 1)];
 
     // group box (code/computeForces.cu:539-555)
-    const float mnx = wave_min(pi.x), mny = wave_min(pi.y), mnz = wave_min(pi.z);
-    const float mxx = wave_max(pi.x), mxy = wave_max(pi.y), mxz = wave_max(pi.z);
-    float gcx, gcy, gcz, gsx, gsy, gsz;
+    R lmnx = pi[0].x, lmny = pi[0].y, lmnz = pi[0].z;
+    R lmxx = pi[0].x, lmxy = pi[0].y, lmxz = pi[0].z;
+#pragma unroll
+    for (int k = 1; k < IPL; k++) {
+      lmnx = rmin(lmnx, pi[k].x); lmny = rmin(lmny, pi[k].y); lmnz = rmin(lmnz, pi[k].z);
+      lmxx = rmax(lmxx, pi[k].x); lmxy = rmax(lmxy, pi[k].y); lmxz = rmax(lmxz, pi[k].z);
+    }
+    const R mnx = wave_min_r(lmnx), mny = wave_min_r(lmny), mnz = wave_min_r(lmnz);
+    const R mxx = wave_max_r(lmxx), mxy = wave_max_r(lmxy), mxz = wave_max_r(lmxz);
+    R gcx, gcy, gcz, gsx, gsy, gsz;
     {
 #pragma clang fp contract(off)
-      gcx = 0.5f * (mxx + mnx); gcy = 0.5f * (mxy + mny); gcz = 0.5f * (mxz + mnz);
-      gsx = 0.5f * (mxx - mnx); gsy = 0.5f * (mxy - mny); gsz = 0.5f * (mxz - mnz);
+      gcx = R(0.5) * (mxx + mnx); gcy = R(0.5) * (mxy + mny); gcz = R(0.5) * (mxz + mnz);
+      gsx = R(0.5) * (mxx - mnx); gsy = R(0.5) * (mxy - mny); gsz = R(0.5) * (mxz - mnz);
     }
     GroupGeom geom;
-    if (LEARN) geom = make_group_geom(gcx, gcy, gcz, gsx, gsy, gsz, np, a.sys_scale[sys]);
-    // i-particle: relative to the group centre (LDS path) or absolute (scalar path)
-    const float xi = SJ ? pi.x : pi.x - gcx, yi = SJ ? pi.y : pi.y - gcy,
-                zi = SJ ? pi.z : pi.z - gcz;
+    if constexpr (LEARN) geom = make_group_geom(gcx, gcy, gcz, gsx, gsy, gsz, np, a.sys_scale[sys]);
+    // i-particles: relative to the group centre (LDS path) or absolute (scalar path)
+    IParts<R, IPL> ip;
+#pragma unroll
+    for (int k = 0; k < IPL; k++) {
+      ip.x[k] = SJ ? pi[k].x : pi[k].x - gcx;
+      ip.y[k] = SJ ? pi[k].y : pi[k].y - gcy;
+      ip.z[k] = SJ ? pi[k].z : pi[k].z - gcz;
+      ip.s[k] = Acc4T<R>{R(0), R(0), R(0), R(0)};
+      ip.c[k] = Acc4T<R>{R(0), R(0), R(0), R(0)};
+    }
 
-    Acc4 s = {0.f, 0.f, 0.f, 0.f}, cmp = {0.f, 0.f, 0.f, 0.f};
     const int2 rc = a.root_cells[sys];
     if (lane < rc.y) {
       if (LSTK) lstk[lane] = rc.x + lane;
@@ -411,14 +492,14 @@ void k_walk(WalkArgs a) {
       const bool use = lane < take;
       int ci = 0;
       uint4 cd = make_uint4(0, 0, 0, 0);
-      float4 cs = make_float4(0.f, 0.f, 0.f, 0.f);
+      V4<R> cs = make_v4<R>(R(0), R(0), R(0), R(0));
       if (use) {
         ci = LSTK ? lstk[top + lane] : stk[top + lane];
         cd = a.cells[ci];
         cs = a.csize[ci];
       }
       bool split = false, leaf = false;
-      if (LEARN) {
+      if constexpr (LEARN) {
         float2 lsv = make_float2(0.f, 0.f);
         if (use) lsv = a.cls[ci];
         // A cell opened even at theta_hi or accepted even at theta_lo is
@@ -464,9 +545,11 @@ void k_walk(WalkArgs a) {
           split = ds2 < op || ((int)cd.w - (int)cd.z < 3);
           leaf = cell_leaf(cd);
         }
-      } else if (use) {
-        split = open_cell(cs, gcx, gcy, gcz, gsx, gsy, gsz) || ((int)cd.w - (int)cd.z < 3);
-        leaf = cell_leaf(cd);
+      } else {
+        if (use) {
+          split = open_cell<R>(cs, gcx, gcy, gcz, gsx, gsy, gsz) || ((int)cd.w - (int)cd.z < 3);
+          leaf = cell_leaf(cd);
+        }
       }
       const bool push = use && split && !leaf;
       const bool approx = use && !split;
@@ -542,7 +625,7 @@ void k_walk(WalkArgs a) {
           nd += min(kWave, ltot - k0);
           wave_lds_sync();
           if (nd >= kWave) {
-            batch_p2p<KAHAN, SJ>(a, S, kWave, gcx, gcy, gcz, xi, yi, zi, s, cmp);
+            batch_p2p<KAHAN, SJ, R, IPL>(a, S, kWave, gcx, gcy, gcz, ip);
             n_dir += kWave;
             nd -= kWave;
             if (lane < nd) S.dlist[lane] = S.dlist[kWave + lane];
@@ -552,7 +635,7 @@ void k_walk(WalkArgs a) {
       }
       wave_lds_sync();
       if (na >= kWave) {
-        batch_p2c<KAHAN, SJ, MONO>(a, S, kWave, gcx, gcy, gcz, xi, yi, zi, s, cmp);
+        batch_p2c<KAHAN, SJ, MONO, R, IPL>(a, S, kWave, gcx, gcy, gcz, ip);
         n_app += kWave;
         na -= kWave;
         if (lane < na) S.alist[lane] = S.alist[kWave + lane];
@@ -569,16 +652,45 @@ void k_walk(WalkArgs a) {
       nd = 0;
     }
     if (na > 0) {
-      batch_p2c<KAHAN, SJ, MONO>(a, S, na, gcx, gcy, gcz, xi, yi, zi, s, cmp);
+      batch_p2c<KAHAN, SJ, MONO, R, IPL>(a, S, na, gcx, gcy, gcz, ip);
       n_app += na;
     }
     if (nd > 0) {
-      batch_p2p<KAHAN, SJ>(a, S, nd, gcx, gcy, gcz, xi, yi, zi, s, cmp);
+      batch_p2p<KAHAN, SJ, R, IPL>(a, S, nd, gcx, gcy, gcz, ip);
       n_dir += nd;
     }
-    // lanes past the end of the group write to the kWave spare slots after acc[n)
-    a.acc[valid ? pbeg + lane : a.n + lane] = make_float4(s.x, s.y, s.z, s.w);
-    const double gp = wave_sum(valid ? (double)pi.w * (double)s.w : 0.0);
+    // this lane stores particle `idx`; lanes past the end of the group write
+    // to the kWave spare slots after acc[n)
+    Acc4T<R> res;
+    R mi;
+    int idx;
+    if (IPL == 1) {
+      res = ip.s[0];
+      mi = pi[0].w;
+      idx = lane;
+    } else {
+      // each half-wave summed half of every batch: add the other half's sums
+      // (compensated values s - c) to get the particles' totals
+      const int h = lane >> 5;
+      Acc4T<R> t[IPL];
+#pragma unroll
+      for (int k = 0; k < IPL; k++) {
+        Acc4T<R> v = ip.s[k];
+        if (KAHAN) {
+          v.x -= ip.c[k].x; v.y -= ip.c[k].y; v.z -= ip.c[k].z; v.w -= ip.c[k].w;
+        }
+        t[k].x = v.x + __shfl_xor(v.x, 32, kWave);
+        t[k].y = v.y + __shfl_xor(v.y, 32, kWave);
+        t[k].z = v.z + __shfl_xor(v.z, 32, kWave);
+        t[k].w = v.w + __shfl_xor(v.w, 32, kWave);
+      }
+      res = h ? t[IPL - 1] : t[0];
+      mi = h ? pi[IPL - 1].w : pi[0].w;
+      idx = 2 * (lane & 31) + h;
+    }
+    const bool valid = idx < np;
+    a.acc[valid ? pbeg + idx : a.n + lane] = make_v4<R>(res.x, res.y, res.z, res.w);
+    const double gp = wave_sum(valid ? (double)mi * (double)res.w : 0.0);
     a.gcounts[g] = overflow ? make_int2(-1, -1) : make_int2(n_app, n_dir);
     a.gpot[g] = gp;
     atomicMax(&a.dc->stack_max, lane == 0 ? deepest : 0);
@@ -671,29 +783,32 @@ __global__ void k_walk_reset(DevCounters *dc, const int *run_if) {
 
 }  // namespace
 
-void launch_walk(Ctx &c, const int *run_if) {
-  WalkArgs a;
+namespace {
+
+template <typename R>
+WalkArgs<R> walk_args(Ctx &c, const int *run_if) {
+  WalkArgs<R> a;
   a.n = c.n;
   a.nsys = c.nsys;
-  a.eps2 = c.eps * c.eps;
+  a.eps2 = (R)c.eps * (R)c.eps;
   a.groups = c.groups;
   a.group_sys = c.group_sys;
   a.root_cells = c.root_cells;
-  a.pos_ph = c.pos_ph;
-  a.pos_tree = c.pos;
+  a.pos_ph = as4<R>(c.pos_ph);
+  a.pos_tree = as4<R>(c.pos);
   a.cells = c.cells;
-  a.csize = c.csize;
-  a.cmono = c.cmono;
-  a.cq0 = c.cq0;
-  a.cq1 = c.cq1;
-  a.cwalk = c.cwalk;
+  a.csize = as4<R>(c.csize);
+  a.cmono = as4<R>(c.cmono);
+  a.cq0 = as4<R>(c.cq0);
+  a.cq1 = as2<R>(c.cq1);
+  a.cwalk = as4<R>(c.cwalk);
   a.cls = c.cls;
   a.sys_scale = c.sys_scale;
   a.mlp = c.mlp_dev;
   a.shift = c.learn_shift;
   a.it_min = 1.0f / c.learn_theta_hi;
   a.it_max = 1.0f / c.learn_theta_lo;
-  a.acc = c.acc;
+  a.acc = as4<R>(c.acc);
   a.gcounts = c.gcounts;
   a.gpot = c.gpot;
   a.stack = c.stack;
@@ -701,19 +816,42 @@ void launch_walk(Ctx &c, const int *run_if) {
   a.lstk_cap = c.lds_stack;
   a.run_if = run_if;
   a.dc = c.dc;
+  return a;
+}
+
+}  // namespace
+
+void launch_walk(Ctx &c, const int *run_if) {
   hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, c.stream, c.dc, run_if);
-  void (*kern)(WalkArgs);
-  if (!c.quadrupole)
-    kern = c.kahan ? k_walk<true, false, false, true> : k_walk<false, false, false, true>;
-  else if (c.opening == 1 && c.have_mlp && c.walk_variant == 1)
-    kern = c.kahan ? k_walk<true, true, true> : k_walk<false, true, true>;
-  else if (c.opening == 1 && c.have_mlp)
-    kern = c.kahan ? k_walk<true, false, true> : k_walk<false, false, true>;
-  else if (c.walk_variant == 1)
-    kern = c.kahan ? k_walk<true, true, false> : k_walk<false, true, false>;
-  else
-    kern = c.kahan ? k_walk<true, false, false> : k_walk<false, false, false>;
-  hipLaunchKernelGGL(kern, dim3(c.walk_blocks), dim3(kBlock), 0, c.stream, a);
+  if (c.fp64) {
+    // Treecode<double>: one i-particle per lane, LDS j-data, FP64 sums
+    const WalkArgs<double> a = walk_args<double>(c, run_if);
+    void (*kern)(WalkArgs<double>) = c.quadrupole ? k_walk<double, false, false, false, false, 1>
+                                                  : k_walk<double, false, false, false, true, 1>;
+    hipLaunchKernelGGL(kern, dim3(c.walk_blocks), dim3(kBlock), 0, c.stream, a);
+  } else {
+    // variant 0: two i-particles per lane, batches split between the
+    // half-waves (LDS j-data); 1: scalar-load j-data, one i-particle per lane;
+    // 2: LDS j-data, one i-particle per lane
+    const WalkArgs<float> a = walk_args<float>(c, run_if);
+    void (*kern)(WalkArgs<float>);
+    const bool learn = c.opening == 1 && c.have_mlp;
+    const int v = c.walk_variant;
+#define BT_PICK(K, LEARN, MONO)                                                     \
+  (v == 1 ? k_walk<float, K, true, LEARN, MONO, 1>                                  \
+          : v == 2 ? k_walk<float, K, false, LEARN, MONO, 1> : k_walk<float, K, false, LEARN, MONO, 2>)
+    if (!c.quadrupole)
+      kern = c.kahan ? (v == 2 ? k_walk<float, true, false, false, true, 1>
+                               : k_walk<float, true, false, false, true, 2>)
+                     : (v == 2 ? k_walk<float, false, false, false, true, 1>
+                               : k_walk<float, false, false, false, true, 2>);
+    else if (learn)
+      kern = c.kahan ? BT_PICK(true, true, false) : BT_PICK(false, true, false);
+    else
+      kern = c.kahan ? BT_PICK(true, false, false) : BT_PICK(false, false, false);
+#undef BT_PICK
+    hipLaunchKernelGGL(kern, dim3(c.walk_blocks), dim3(kBlock), 0, c.stream, a);
+  }
   WStat *part = reinterpret_cast<WStat *>(c.d_wstats + 4);
   hipLaunchKernelGGL(k_walk_stats_part, dim3(kWalkStatBlocks), dim3(kBlock), 0, c.stream, c.groups,
                      c.gcounts, c.gpot, c.dc, part, run_if);

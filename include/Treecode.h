@@ -1,8 +1,9 @@
 // Treecode.h -- drop-in C++ host API of the MI355X treecode.
 //
 // Rebuilds the reference class template Treecode<real_t>
-// (code/Treecode.h:123-223, instantiated for float only,
-// code/TreecodeInstances.h:3) on top of the C ABI in bonsai2_amd.h, so a
+// (code/Treecode.h:123-223) on top of the C ABI in bonsai2_amd.h, for float
+// (the reference's only instantiation, code/TreecodeInstances.h:3) and for
+// double (Particle4<double>, code/Particle4.h:106-125, computed in FP64), so a
 // caller written against the reference keeps its call sequence
 // (code/treecode.cu:61-165) unchanged:
 //
@@ -138,15 +139,57 @@ class device_view {
 
 }  // namespace bonsai2
 
+namespace bonsai2 {
+
+// vec<N, T>::type as the reference spells it (code/Particle4.h:8-19)
+template <int N, typename T> struct vec;
+template <> struct vec<4, float> { typedef float4 type; };
+template <> struct vec<4, double> { typedef double4 type; };
+template <> struct vec<2, float> { typedef float2 type; };
+template <> struct vec<2, double> { typedef double2 type; };
+
+// The C ABI per real type: Treecode<float> and Treecode<double>.
+namespace detail {
+inline int create(bt_ctx **c, float eps, float theta, int ncrit) {
+  return bt_create(c, eps, theta, ncrit);
+}
+inline int create(bt_ctx **c, double eps, double theta, int ncrit) {
+  return bt_create_f64(c, eps, theta, ncrit);
+}
+inline int set_params(bt_ctx *c, float eps, float theta) { return bt_set_params(c, eps, theta); }
+inline int set_params(bt_ctx *c, double eps, double theta) {
+  return bt_set_params_f64(c, eps, theta);
+}
+inline int set_particles(bt_ctx *c, const float *p, const float *v) {
+  return bt_set_particles(c, p, v);
+}
+inline int set_particles(bt_ctx *c, const double *p, const double *v) {
+  return bt_set_particles_f64(c, p, v);
+}
+inline int get_particles(bt_ctx *c, float *p, float *v, int order) {
+  return bt_get_particles(c, p, v, order);
+}
+inline int get_particles(bt_ctx *c, double *p, double *v, int order) {
+  return bt_get_particles_f64(c, p, v, order);
+}
+inline int get_acc(bt_ctx *c, float *a, int order) { return bt_get_acc(c, a, order); }
+inline int get_acc(bt_ctx *c, double *a, int order) { return bt_get_acc_f64(c, a, order); }
+inline int move(bt_ctx *c, float dt) { return bt_move_particles(c, dt); }
+inline int move(bt_ctx *c, double dt) { return bt_move_particles_f64(c, dt); }
+}  // namespace detail
+
+}  // namespace bonsai2
+
+// Treecode<float> and Treecode<double> (code/Treecode.h:123-223).
 template <typename real_t>
-struct Treecode;
+struct Treecode {
+  static_assert(sizeof(real_t) == 4 || sizeof(real_t) == 8, "Treecode<float> or Treecode<double>");
+  typedef bonsai2::Particle4<real_t> Particle;
+  typedef typename bonsai2::vec<4, real_t>::type real4_t;
+  typedef typename bonsai2::vec<2, real_t>::type real2_t;
 
-template <>
-struct Treecode<float> {
-  typedef bonsai2::Particle4<float> Particle;
-
-  float theta, eps2;
-  float dt = 1.0f / 64;              // step of the no-argument moveParticles()
+  real_t theta, eps2;
+  real_t dt = real_t(1) / 64;        // step of the no-argument moveParticles()
   double grav_potential = 0.0;       // code/Treecode.h:162
   double ekin = 0.0, epot = 0.0;     // set by computeEnergies()
 
@@ -158,7 +201,8 @@ struct Treecode<float> {
     if (rc != 0) throw bonsai2::Error(rc, std::string(what) + ": " + bt_last_error(ctx_));
   }
   void sync_params() {
-    check(bt_set_params(ctx_, eps2 > 0.f ? std::sqrt(eps2) : 0.f, theta), "set_params");
+    check(bonsai2::detail::set_params(ctx_, eps2 > real_t(0) ? std::sqrt(eps2) : real_t(0), theta),
+          "set_params");
   }
 
  public:
@@ -169,13 +213,13 @@ struct Treecode<float> {
   bonsai2::device_view<Particle> d_ptclVel{&ctx_, BT_BUF_PTCL_VEL};
   bonsai2::device_view<Particle> d_ptclPos_tmp{&ctx_, BT_BUF_PTCL_POS_TMP};
   bonsai2::device_view<Particle> d_ptclAcc{&ctx_, BT_BUF_PTCL_ACC};
-  bonsai2::device_view<float4> d_domain{&ctx_, BT_BUF_DOMAIN};
+  bonsai2::device_view<real4_t> d_domain{&ctx_, BT_BUF_DOMAIN};
   bonsai2::device_view<int2> d_level_begIdx{&ctx_, BT_BUF_LEVEL_BEGIDX};
   bonsai2::device_view<uint4> d_cellDataList{&ctx_, BT_BUF_CELL_DATA};
-  bonsai2::device_view<float4> d_cellSize{&ctx_, BT_BUF_CELL_SIZE};
-  bonsai2::device_view<float4> d_cellMonopole{&ctx_, BT_BUF_CELL_MONOPOLE};
-  bonsai2::device_view<float4> d_cellQuad0{&ctx_, BT_BUF_CELL_QUAD0};
-  bonsai2::device_view<float2> d_cellQuad1{&ctx_, BT_BUF_CELL_QUAD1};
+  bonsai2::device_view<real4_t> d_cellSize{&ctx_, BT_BUF_CELL_SIZE};
+  bonsai2::device_view<real4_t> d_cellMonopole{&ctx_, BT_BUF_CELL_MONOPOLE};
+  bonsai2::device_view<real4_t> d_cellQuad0{&ctx_, BT_BUF_CELL_QUAD0};
+  bonsai2::device_view<real2_t> d_cellQuad1{&ctx_, BT_BUF_CELL_QUAD1};
   bonsai2::device_view<int2> d_groupList{&ctx_, BT_BUF_GROUP_LIST};
 
   int get_nPtcl() const { return nPtcl; }
@@ -183,9 +227,10 @@ struct Treecode<float> {
   int get_nLeaf() const { return nLeaf; }
 
   // code/Treecode.h:164-173
-  Treecode(const float _eps = 0.01f, const float _theta = 0.75f, const int _ncrit = 64)
+  Treecode(const real_t _eps = real_t(0.01), const real_t _theta = real_t(0.75),
+           const int _ncrit = 64)
       : theta(_theta), eps2(_eps * _eps), nCrit(_ncrit) {
-    const int rc = bt_create(&ctx_, _eps, _theta, _ncrit);
+    const int rc = bonsai2::detail::create(&ctx_, _eps, _theta, _ncrit);
     if (rc != 0) throw bonsai2::Error(rc, "bt_create failed (no HIP device?)");
   }
   ~Treecode() { bt_destroy(ctx_); }
@@ -203,19 +248,22 @@ struct Treecode<float> {
   // code/Treecode.h:207-214 (also keeps the initial positions in ptcl0)
   void ptcl_h2d() {
     sync_params();
-    check(bt_set_particles(ctx_, &h_ptclPos[0].data[0], &h_ptclVel[0].data[0]), "ptcl_h2d");
+    check(bonsai2::detail::set_particles(ctx_, &h_ptclPos[0].data[0], &h_ptclVel[0].data[0]),
+          "ptcl_h2d");
     ptcl0.assign(h_ptclPos.ptr, h_ptclPos.ptr + nPtcl);
   }
 
   // code/Treecode.h:200-204: d_ptclPos / d_ptclVel as they are (tree order
   // after buildTree, input order before the first build)
   void ptcl_d2h() {
-    check(bt_get_particles(ctx_, &h_ptclPos[0].data[0], &h_ptclVel[0].data[0], BT_ORDER_TREE),
+    check(bonsai2::detail::get_particles(ctx_, &h_ptclPos[0].data[0], &h_ptclVel[0].data[0],
+                                         BT_ORDER_TREE),
           "ptcl_d2h");
   }
   // the same particles in the order ptcl_h2d gave them
   void ptcl_d2h_input() {
-    check(bt_get_particles(ctx_, &h_ptclPos[0].data[0], &h_ptclVel[0].data[0], BT_ORDER_INPUT),
+    check(bonsai2::detail::get_particles(ctx_, &h_ptclPos[0].data[0], &h_ptclVel[0].data[0],
+                                         BT_ORDER_INPUT),
           "ptcl_d2h_input");
   }
 
@@ -242,7 +290,7 @@ struct Treecode<float> {
   }
   // first half of a kick-drift-kick step with dt; the next force evaluation
   // applies the second half kick
-  void moveParticles() { check(bt_move_particles(ctx_, dt), "moveParticles"); }
+  void moveParticles() { check(bonsai2::detail::move(ctx_, dt), "moveParticles"); }
   void computeEnergies() {
     check(bt_compute_energies(ctx_, &ekin, &epot), "computeEnergies");
     check(bt_get_potential(ctx_, &grav_potential), "grav_potential");
@@ -256,7 +304,7 @@ struct Treecode<float> {
   // (ax, ay, az, phi) per particle (d_ptclAcc), in input order by default
   std::vector<Particle> get_acc(int order = BT_ORDER_INPUT) {
     std::vector<Particle> a(nPtcl);
-    check(bt_get_acc(ctx_, &a[0].data[0], order), "get_acc");
+    check(bonsai2::detail::get_acc(ctx_, &a[0].data[0], order), "get_acc");
     return a;
   }
 };

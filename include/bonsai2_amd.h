@@ -71,7 +71,8 @@ const char *bt_last_error(const bt_ctx *ctx);
 
 /* ptcl_h2d(), code/Treecode.h:207-214: copies n particles (input order). */
 int bt_set_particles(bt_ctx *ctx, const float *pos4, const float *vel4);
-/* Same, from device pointers (no host staging). */
+/* Same, from device pointers (no host staging): float4 arrays, or double4
+ * for a Treecode<double> context (bt_create_f64). */
 int bt_set_particles_device(bt_ctx *ctx, const void *d_pos4, const void *d_vel4);
 /* ptcl_d2h(), code/Treecode.h:200-204, in the requested order. */
 int bt_get_particles(bt_ctx *ctx, float *pos4, float *vel4, int order);
@@ -164,11 +165,14 @@ int bt_get_timings(bt_ctx *ctx, float ms[10]);
  *   a walk that needs more returns BT_ERR_STACK);
  * blocks_per_cu = persistent grid size (default 0 = automatic). */
 int bt_set_walk_options(bt_ctx *ctx, int kahan, int stack_ints, int blocks_per_cu);
-/* How the walk feeds interaction data to the lanes: 0 (default) = the wave
- * stages each batch of 64 cells / particles in LDS and sweeps it with
- * broadcast reads; 1 = each batch entry is read by scalar loads straight from
- * the immutable cell / particle arrays and used from SGPRs.  Both evaluate the
- * same interaction lists; forces agree to FP32 rounding. */
+/* How the walk feeds interaction data to the lanes: 0 (default) = each lane
+ * carries two of the group's i-particles, the wave stages each batch of 64
+ * cells / particles in LDS and each half-wave sweeps half of it with
+ * broadcast reads (every LDS read serves two interactions); 1 = one
+ * i-particle per lane, each batch entry read by scalar loads straight from
+ * the immutable cell / particle arrays and used from SGPRs; 2 = one
+ * i-particle per lane, the whole batch swept from LDS.  All evaluate the same
+ * interaction lists; forces agree to FP32 rounding. */
 int bt_set_walk_variant(bt_ctx *ctx, int variant);
 /* Learned-angle walk only (no reference counterpart): entries of each wave's
  * traversal stack kept in LDS, 512..1024 (default 1024); deeper entries spill
@@ -267,6 +271,33 @@ int bt_set_opening_check(bt_ctx *ctx, int nsamples, float p99_max, uint64_t seed
  * checks} */
 int bt_get_opening_check(bt_ctx *ctx, double out[6]);
 
+/* ---- Treecode<double> ----------------------------------------------------
+ * The reference declares Particle4<double> (code/Particle4.h:106-125) but
+ * instantiates Treecode for float only (code/TreecodeInstances.h:3).  A
+ * context made by bt_create_f64 is Treecode<double>: particles, root box,
+ * geometric and PH keys, moments, walk and integration in FP64 (double4
+ * particles (x, y, z, m), double2 / double4 cell data).  The root box keeps
+ * its extremes in FP64 (the reference's double build would pass them through
+ * its float2 block reduction, code/buildTree.cu:102-105).  Entry points
+ * without a real-typed argument (build, multipoles, groups, forces, energies,
+ * counts, groups, keys, statistics, bt_direct_fp64, bt_step_graph, bt_step,
+ * bt_move_particles, the device buffers) serve both kinds; the float data
+ * entry points (bt_set_particles, bt_get_particles, bt_get_acc, the moments of
+ * bt_get_tree, bt_get_cell_ls, bt_get_boxes) return BT_ERR_STATE on a double
+ * context, and the _f64 ones on a float context.  The walk evaluates one
+ * i-particle per lane (bt_set_walk_variant has no effect); the learned
+ * opening angle and its training are float only. */
+int bt_create_f64(bt_ctx **ctx, double eps, double theta, int ncrit);
+int bt_set_params_f64(bt_ctx *ctx, double eps, double theta);
+int bt_set_particles_f64(bt_ctx *ctx, const double *pos4, const double *vel4);
+int bt_get_particles_f64(bt_ctx *ctx, double *pos4, double *vel4, int order);
+int bt_get_acc_f64(bt_ctx *ctx, double *acc4, int order);
+/* size4 = (com, cellOp^2), mono4 = (com, M), quad0_4, quad1_2 in FP64. */
+int bt_get_tree_f64(bt_ctx *ctx, double *size4, double *mono4, double *quad0_4, double *quad1_2);
+int bt_get_boxes_f64(bt_ctx *ctx, double *boxes4);
+int bt_move_particles_f64(bt_ctx *ctx, double dt);
+int bt_step_f64(bt_ctx *ctx, double dt, int nleaf, int level_split, int ncrit);
+
 /* ---- memory of the reference's public members ---------------------------- */
 
 /* Pinned host memory, the reference's host_mem<T> (code/cudamem.h:64-96,
@@ -277,7 +308,9 @@ int bt_host_free(void *ptr);
 /* The device arrays behind the reference's public cuda_mem members
  * (code/Treecode.h:142-160): current address and element count.  Particle
  * arrays change address at every buildTree (the reference swaps its pointers
- * too, code/buildTree.cu:929), so a pointer is valid until the next build. */
+ * too, code/buildTree.cu:929), so a pointer is valid until the next build.
+ * In a Treecode<double> context the float4 / float2 arrays hold double4 /
+ * double2. */
 enum {
   BT_BUF_PTCL_POS = 0,     /* d_ptclPos: float4 (x,y,z,m), tree order after buildTree */
   BT_BUF_PTCL_VEL = 1,     /* d_ptclVel: float4 (vx,vy,vz,m), same order */

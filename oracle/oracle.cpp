@@ -40,6 +40,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 #include <omp.h>
 
@@ -50,6 +51,12 @@ struct F2 { float x, y; };
 struct U4 { uint32_t x, y, z, w; };
 struct I2 { int32_t x, y; };
 struct D4 { double x, y, z, w; };
+struct D2 { double x, y; };
+
+// 4- and 2-vectors of the real type (Treecode<float> / Treecode<double>)
+template <typename R> struct V;
+template <> struct V<float> { typedef F4 v4; typedef F2 v2; };
+template <> struct V<double> { typedef D4 v4; typedef D2 v2; };
 
 constexpr int kMaxLevel = 21;   // depth of the geometric octant key (63 bits)
 
@@ -58,18 +65,22 @@ constexpr int kMaxLevel = 21;   // depth of the geometric octant key (63 bits)
 // iff centre <= pos on that axis (x -> 1, y -> 2, z -> 4).  The child centre
 // moves by +-hsize/2 per axis; all values are exact dyadic floats.
 // ---------------------------------------------------------------------------
-inline int octant_of(float cx, float cy, float cz, float px, float py, float pz) {
+template <typename R>
+inline int octant_of(R cx, R cy, R cz, R px, R py, R pz) {
   return (int)(cx <= px) | ((int)(cy <= py) << 1) | ((int)(cz <= pz) << 2);
 }
 
-struct BoxF { float cx, cy, cz, h; };
+template <typename R>
+struct BoxT { R cx, cy, cz, h; };
+typedef BoxT<float> BoxF;
 
-inline BoxF child_box(const BoxF &b, int oct) {
-  const float s = 0.5f * b.h;
-  BoxF c;
-  c.cx = b.cx + s * ((oct & 1) ? 1.0f : -1.0f);
-  c.cy = b.cy + s * ((oct & 2) ? 1.0f : -1.0f);
-  c.cz = b.cz + s * ((oct & 4) ? 1.0f : -1.0f);
+template <typename R>
+inline BoxT<R> child_box(const BoxT<R> &b, int oct) {
+  const R s = R(0.5) * b.h;
+  BoxT<R> c;
+  c.cx = b.cx + s * ((oct & 1) ? R(1) : R(-1));
+  c.cy = b.cy + s * ((oct & 2) ? R(1) : R(-1));
+  c.cz = b.cz + s * ((oct & 4) ? R(1) : R(-1));
   c.h = s;
   return c;
 }
@@ -102,32 +113,45 @@ int or_version() { return 3; }
 // A zero extent (the reference's loop would underflow to 0 and divide by
 // it) is given hsize = 1.
 // out box = {cx, cy, cz, hsize}; bmin/bmax = raw extremes.
+// Treecode<double> (R = double) keeps the extremes in double; the
+// reference's double instantiation would pass them through its float2 block
+// reduction (code/buildTree.cu:102-105), rounding them to float.
 // ---------------------------------------------------------------------------
-void or_bounding_box(int n, const F4 *pos, float *box, float *bmin, float *bmax) {
-  float mn[3] = {1e10f, 1e10f, 1e10f}, mx[3] = {-1e10f, -1e10f, -1e10f};
+}  // extern "C"
+template <typename R>
+static void bounding_box_t(int n, const typename V<R>::v4 *pos, R *box, R *bmin, R *bmax) {
+  R mn[3] = {R(1e10), R(1e10), R(1e10)}, mx[3] = {R(-1e10), R(-1e10), R(-1e10)};
   for (int i = 0; i < n; i++) {
     mn[0] = std::fmin(mn[0], pos[i].x); mx[0] = std::fmax(mx[0], pos[i].x);
     mn[1] = std::fmin(mn[1], pos[i].y); mx[1] = std::fmax(mx[1], pos[i].y);
     mn[2] = std::fmin(mn[2], pos[i].z); mx[2] = std::fmax(mx[2], pos[i].z);
   }
-  float c[3], hv[3];
+  R c[3], hv[3];
   for (int k = 0; k < 3; k++) {
-    c[k] = (mx[k] + mn[k]) * 0.5f;
-    hv[k] = (mx[k] - mn[k]) * 0.5f;
+    c[k] = (mx[k] + mn[k]) * R(0.5);
+    hv[k] = (mx[k] - mn[k]) * R(0.5);
   }
-  const float h = std::fmax(hv[2], std::fmax(hv[1], hv[0]));
-  float hsize = 1.0f;
-  if (h > 0.0f) {
-    while (hsize > h) hsize *= 0.5f;
-    while (hsize < h) hsize *= 2.0f;
+  const R h = std::fmax(hv[2], std::fmax(hv[1], hv[0]));
+  R hsize = R(1);
+  if (h > R(0)) {
+    while (hsize > h) hsize *= R(0.5);
+    while (hsize < h) hsize *= R(2);
   }
-  const float hquant = hsize / (float)(1 << 20);
+  const R hquant = hsize / (R)(1 << 20);
   for (int k = 0; k < 3; k++) {
     const long long q = (long long)(c[k] / hquant);
-    box[k] = hquant * (float)q;
+    box[k] = hquant * (R)q;
   }
   box[3] = hsize;
   for (int k = 0; k < 3; k++) { bmin[k] = mn[k]; bmax[k] = mx[k]; }
+}
+extern "C" {
+
+void or_bounding_box(int n, const F4 *pos, float *box, float *bmin, float *bmax) {
+  bounding_box_t<float>(n, pos, box, bmin, bmax);
+}
+void or_bounding_box_d(int n, const D4 *pos, double *box, double *bmin, double *bmax) {
+  bounding_box_t<double>(n, pos, box, bmin, bmax);
 }
 
 // ---------------------------------------------------------------------------
@@ -135,18 +159,28 @@ void or_bounding_box(int n, const F4 *pos, float *box, float *bmin, float *bmax)
 // the way down the ChildBox recursion of code/buildTree.cu:233-247, 21
 // levels, most significant digit first (bits 62..60 = level 1).
 // ---------------------------------------------------------------------------
-void or_geo_keys(int n, const F4 *pos, const float *box, uint64_t *keys) {
+}  // extern "C"
+template <typename R>
+static void geo_keys_t(int n, const typename V<R>::v4 *pos, const R *box, uint64_t *keys) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; i++) {
-    BoxF b{box[0], box[1], box[2], box[3]};
+    BoxT<R> b{box[0], box[1], box[2], box[3]};
     uint64_t k = 0;
     for (int l = 0; l < kMaxLevel; l++) {
-      const int o = octant_of(b.cx, b.cy, b.cz, pos[i].x, pos[i].y, pos[i].z);
+      const int o = octant_of<R>(b.cx, b.cy, b.cz, pos[i].x, pos[i].y, pos[i].z);
       k = (k << 3) | (uint64_t)o;
       b = child_box(b, o);
     }
     keys[i] = k;
   }
+}
+extern "C" {
+
+void or_geo_keys(int n, const F4 *pos, const float *box, uint64_t *keys) {
+  geo_keys_t<float>(n, pos, box, keys);
+}
+void or_geo_keys_d(int n, const D4 *pos, const double *box, uint64_t *keys) {
+  geo_keys_t<double>(n, pos, box, keys);
 }
 
 // ---------------------------------------------------------------------------
@@ -181,15 +215,17 @@ void or_geo_keys(int n, const F4 *pos, const float *box, uint64_t *keys) {
 // ---------------------------------------------------------------------------
 // ocb: octant scratch of n bytes; a call uses only [b, e), disjoint from
 // every concurrent call's range and released before its children run.
+}  // extern "C"
+template <typename R>
 static void stable_partition_rec(std::vector<int> &idx, std::vector<int> &tmp,
-                                 std::vector<unsigned char> &ocb, const F4 *pos, int b, int e,
-                                 BoxF box, int level) {
+                                 std::vector<unsigned char> &ocb, const typename V<R>::v4 *pos,
+                                 int b, int e, BoxT<R> box, int level) {
   if (e - b <= 1 || level >= kMaxLevel) return;
   int cnt[8] = {0};
   unsigned char *oc = ocb.data() + b;
   for (int i = b; i < e; i++) {
-    const F4 &p = pos[idx[i]];
-    const int o = octant_of(box.cx, box.cy, box.cz, p.x, p.y, p.z);
+    const typename V<R>::v4 &p = pos[idx[i]];
+    const int o = octant_of<R>(box.cx, box.cy, box.cz, p.x, p.y, p.z);
     oc[i - b] = (unsigned char)o;
     cnt[o]++;
   }
@@ -203,13 +239,13 @@ static void stable_partition_rec(std::vector<int> &idx, std::vector<int> &tmp,
   const bool par = (e - b) > 65536 && level < 3;
   for (int o = 0; o < 8; o++) {
     if (cnt[o] == 0) continue;
-    const BoxF cb = child_box(box, o);
+    const BoxT<R> cb = child_box(box, o);
     const int cb0 = off[o], ce0 = off[o + 1];
     if (par) {
 #pragma omp task shared(idx, tmp, ocb)
-      stable_partition_rec(idx, tmp, ocb, pos, cb0, ce0, cb, level + 1);
+      stable_partition_rec<R>(idx, tmp, ocb, pos, cb0, ce0, cb, level + 1);
     } else {
-      stable_partition_rec(idx, tmp, ocb, pos, cb0, ce0, cb, level + 1);
+      stable_partition_rec<R>(idx, tmp, ocb, pos, cb0, ce0, cb, level + 1);
     }
   }
   if (par) {
@@ -217,25 +253,26 @@ static void stable_partition_rec(std::vector<int> &idx, std::vector<int> &tmp,
   }
 }
 
-int or_build_tree(int n, const F4 *pos, int nroot, const int *offsets,
-                  const float *boxes, int nleaf, int ncell_max,
-                  int *order, U4 *cells, int *level_range, int *root_cells,
-                  int *stats) {
+template <typename R>
+static int build_tree_t(int n, const typename V<R>::v4 *pos, int nroot, const int *offsets,
+                        const R *boxes, int nleaf, int ncell_max,
+                        int *order, U4 *cells, int *level_range, int *root_cells,
+                        int *stats) {
   std::vector<int> idx(n), tmp(n);
   std::vector<unsigned char> ocb(n);
   for (int i = 0; i < n; i++) idx[i] = i;
 #pragma omp parallel
 #pragma omp single
   for (int r = 0; r < nroot; r++) {
-    const BoxF b{boxes[4 * r], boxes[4 * r + 1], boxes[4 * r + 2], boxes[4 * r + 3]};
-    stable_partition_rec(idx, tmp, ocb, pos, offsets[r], offsets[r + 1], b, 0);
+    const BoxT<R> b{boxes[4 * r], boxes[4 * r + 1], boxes[4 * r + 2], boxes[4 * r + 3]};
+    stable_partition_rec<R>(idx, tmp, ocb, pos, offsets[r], offsets[r + 1], b, 0);
   }
   for (int i = 0; i < n; i++) order[i] = idx[i];
 
-  struct Par { int b, e; BoxF box; int cell; int level; };
+  struct Par { int b, e; BoxT<R> box; int cell; int level; };
   std::vector<Par> parents, next;
   for (int r = 0; r < nroot; r++) {
-    const BoxF b{boxes[4 * r], boxes[4 * r + 1], boxes[4 * r + 2], boxes[4 * r + 3]};
+    const BoxT<R> b{boxes[4 * r], boxes[4 * r + 1], boxes[4 * r + 2], boxes[4 * r + 3]};
     parents.push_back({offsets[r], offsets[r + 1], b, -1 - r, 0});
   }
   for (int l = 0; l < 64; l++) { level_range[l] = 0; }
@@ -248,8 +285,8 @@ int or_build_tree(int n, const F4 *pos, int nroot, const int *offsets,
       int cnt[8] = {0};
       int prev = -1;
       for (int i = p.b; i < p.e; i++) {
-        const F4 &q = pos[order[i]];
-        const int o = octant_of(p.box.cx, p.box.cy, p.box.cz, q.x, q.y, q.z);
+        const typename V<R>::v4 &q = pos[order[i]];
+        const int o = octant_of<R>(p.box.cx, p.box.cy, p.box.cz, q.x, q.y, q.z);
         if (o < prev) return -2;  // partition order broken (cannot happen)
         prev = o;
         cnt[o]++;
@@ -301,6 +338,20 @@ int or_build_tree(int n, const F4 *pos, int nroot, const int *offsets,
   stats[0] = ncells; stats[1] = nnodes; stats[2] = nleaves; stats[3] = nlevels;
   stats[4] = capped; stats[5] = (int)nb_leaf;
   return 0;
+}
+extern "C" {
+
+int or_build_tree(int n, const F4 *pos, int nroot, const int *offsets, const float *boxes,
+                  int nleaf, int ncell_max, int *order, U4 *cells, int *level_range,
+                  int *root_cells, int *stats) {
+  return build_tree_t<float>(n, pos, nroot, offsets, boxes, nleaf, ncell_max, order, cells,
+                             level_range, root_cells, stats);
+}
+int or_build_tree_d(int n, const D4 *pos, int nroot, const int *offsets, const double *boxes,
+                    int nleaf, int ncell_max, int *order, U4 *cells, int *level_range,
+                    int *root_cells, int *stats) {
+  return build_tree_t<double>(n, pos, nroot, offsets, boxes, nleaf, ncell_max, order, cells,
+                              level_range, root_cells, stats);
 }
 
 // ---------------------------------------------------------------------------
@@ -399,6 +450,47 @@ void or_multipoles(int ncells, const U4 *cells, const F4 *pos, float inv_theta, 
   }
 }
 
+// Treecode<double>: the same cell quantities from FP64 particles, all in
+// FP64 (two-pass moments as mode 0), outputs double.
+void or_multipoles_d(int ncells, const U4 *cells, const D4 *pos, double inv_theta, D4 *size,
+                     D4 *mono, D4 *quad0, D2 *quad1, D2 *ls_out) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int c = 0; c < ncells; c++) {
+    const int b = (int)cells[c].z, e = (int)cells[c].w;
+    double mn[3] = {1e10, 1e10, 1e10}, mx[3] = {-1e10, -1e10, -1e10};
+    double M = 0, cx = 0, cy = 0, cz = 0;
+    for (int i = b; i < e; i++) {
+      mn[0] = std::fmin(mn[0], pos[i].x); mx[0] = std::fmax(mx[0], pos[i].x);
+      mn[1] = std::fmin(mn[1], pos[i].y); mx[1] = std::fmax(mx[1], pos[i].y);
+      mn[2] = std::fmin(mn[2], pos[i].z); mx[2] = std::fmax(mx[2], pos[i].z);
+      const double m = pos[i].w;
+      M += m; cx += m * pos[i].x; cy += m * pos[i].y; cz += m * pos[i].z;
+    }
+    const double im = 1.0 / M;
+    cx *= im; cy *= im; cz *= im;
+    double qxx = 0, qyy = 0, qzz = 0, qxy = 0, qxz = 0, qyz = 0;
+    for (int i = b; i < e; i++) {
+      const double m = pos[i].w;
+      const double dx = pos[i].x - cx, dy = pos[i].y - cy, dz = pos[i].z - cz;
+      qxx += m * dx * dx; qyy += m * dy * dy; qzz += m * dz * dz;
+      qxy += m * dx * dy; qxz += m * dx * dz; qyz += m * dy * dz;
+    }
+    qxx *= im; qyy *= im; qzz *= im; qxy *= im; qxz *= im; qyz *= im;
+    const double ex = (mx[0] + mn[0]) * 0.5 - cx, ey = (mx[1] + mn[1]) * 0.5 - cy,
+                 ez = (mx[2] + mn[2]) * 0.5 - cz;
+    const double s = std::sqrt(ex * ex + ey * ey + ez * ez);
+    const double l = std::max(2.0 * std::max((mx[0] - mn[0]) * 0.5,
+                                             std::max((mx[1] - mn[1]) * 0.5, (mx[2] - mn[2]) * 0.5)),
+                              1.0e-6);
+    const double op = l * inv_theta + s;
+    size[c] = D4{cx, cy, cz, op * op};
+    mono[c] = D4{cx, cy, cz, M};
+    quad0[c] = D4{qxx, qyy, qzz, qxy};
+    quad1[c] = D2{qxz, qyz};
+    if (ls_out) ls_out[c] = D2{l, s};
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Peano-Hilbert key: code/makeGroups.cu:19-106 (get_key<21>) and the
 // quantisation of computeKeys :108-135.  Coordinates are quantised in
@@ -433,6 +525,19 @@ void or_ph_keys(int n, const F4 *pos, const float *box, uint64_t *keys) {
   const float inv = 0.5f / box[3];
   const float bx = box[0] - box[3], by = box[1] - box[3], bz = box[2] - box[3];
   const float scale = (float)(1 << 21);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    const int xc = (int)((pos[i].x - bx) * inv * scale);
+    const int yc = (int)((pos[i].y - by) * inv * scale);
+    const int zc = (int)((pos[i].z - bz) * inv * scale);
+    keys[i] = hilbert_key(xc, yc, zc, 21);
+  }
+}
+
+void or_ph_keys_d(int n, const D4 *pos, const double *box, uint64_t *keys) {
+  const double inv = 0.5 / box[3];
+  const double bx = box[0] - box[3], by = box[1] - box[3], bz = box[2] - box[3];
+  const double scale = (double)(1 << 21);
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; i++) {
     const int xc = (int)((pos[i].x - bx) * inv * scale);
@@ -653,65 +758,70 @@ inline float learned_inv_theta(const LearnCfg &lc, const float x[8]) {
   return std::fmin(std::fmax(it, 1.0f / lc.theta_hi), 1.0f / lc.theta_lo);
 }
 
-template <typename A>
-static void p2p(A acc[4], float px, float py, float pz, const F4 &j, float eps2) {
-  const float dx = j.x - px, dy = j.y - py, dz = j.z - pz;
-  const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-  const float rinv = 1.0f / std::sqrt(r2);
-  const float rinv2 = rinv * rinv;
-  const float mrinv = j.w * rinv;
-  const float mrinv3 = mrinv * rinv2;
+template <typename A, typename R = float>
+static void p2p(A acc[4], R px, R py, R pz, const typename V<R>::v4 &j, R eps2) {
+  const R dx = j.x - px, dy = j.y - py, dz = j.z - pz;
+  const R r2 = dx * dx + dy * dy + dz * dz + eps2;
+  const R rinv = R(1) / std::sqrt(r2);
+  const R rinv2 = rinv * rinv;
+  const R mrinv = j.w * rinv;
+  const R mrinv3 = mrinv * rinv2;
   acc[3] -= mrinv;
   acc[0] += mrinv3 * dx;
   acc[1] += mrinv3 * dy;
   acc[2] += mrinv3 * dz;
 }
 
-template <typename A>
-static void p2c(A acc[4], float px, float py, float pz, const F4 &m, const F4 &q0,
-                const F2 &q1, float eps2) {
-  const float dx = px - m.x, dy = py - m.y, dz = pz - m.z;
-  const float r2 = dx * dx + dy * dy + dz * dz + eps2;
-  const float rinv = 1.0f / std::sqrt(r2);
-  const float rinv2 = rinv * rinv;
-  const float mrinv = m.w * rinv;
-  const float mrinv3 = rinv2 * mrinv;
-  const float mrinv5 = rinv2 * mrinv3;
-  const float mrinv7 = rinv2 * mrinv5;
-  const float D0 = mrinv, D1 = -mrinv3, D2 = mrinv5 * 3.0f, D3 = mrinv7 * (-15.0f);
-  const float q11 = q0.x, q22 = q0.y, q33 = q0.z, q12 = q0.w, q13 = q1.x, q23 = q1.y;
-  const float q = q11 + q22 + q33;
-  const float qRx = q11 * dx + q12 * dy + q13 * dz;
-  const float qRy = q12 * dx + q22 * dy + q23 * dz;
-  const float qRz = q13 * dx + q23 * dy + q33 * dz;
-  const float qRR = qRx * dx + qRy * dy + qRz * dz;
-  acc[3] -= D0 + 0.5f * (D1 * q + D2 * qRR);
-  const float C = D1 + 0.5f * (D2 * q + D3 * qRR);
+template <typename A, typename R = float>
+static void p2c(A acc[4], R px, R py, R pz, const typename V<R>::v4 &m,
+                const typename V<R>::v4 &q0, const typename V<R>::v2 &q1, R eps2) {
+  const R dx = px - m.x, dy = py - m.y, dz = pz - m.z;
+  const R r2 = dx * dx + dy * dy + dz * dz + eps2;
+  const R rinv = R(1) / std::sqrt(r2);
+  const R rinv2 = rinv * rinv;
+  const R mrinv = m.w * rinv;
+  const R mrinv3 = rinv2 * mrinv;
+  const R mrinv5 = rinv2 * mrinv3;
+  const R mrinv7 = rinv2 * mrinv5;
+  const R D0 = mrinv, D1 = -mrinv3, D2 = mrinv5 * R(3), D3 = mrinv7 * R(-15);
+  const R q11 = q0.x, q22 = q0.y, q33 = q0.z, q12 = q0.w, q13 = q1.x, q23 = q1.y;
+  const R q = q11 + q22 + q33;
+  const R qRx = q11 * dx + q12 * dy + q13 * dz;
+  const R qRy = q12 * dx + q22 * dy + q23 * dz;
+  const R qRz = q13 * dx + q23 * dy + q33 * dz;
+  const R qRR = qRx * dx + qRy * dy + qRz * dz;
+  acc[3] -= D0 + R(0.5) * (D1 * q + D2 * qRR);
+  const R C = D1 + R(0.5) * (D2 * q + D3 * qRR);
   acc[0] += C * dx + D2 * qRx;
   acc[1] += C * dy + D2 * qRy;
   acc[2] += C * dz + D2 * qRz;
 }
 
-static inline bool open_test(const F4 &cs, float gcx, float gcy, float gcz, float gsx,
-                             float gsy, float gsz, const F2 *ls, float inv_theta) {
-  float dx = std::fabs(gcx - cs.x) - gsx;
-  float dy = std::fabs(gcy - cs.y) - gsy;
-  float dz = std::fabs(gcz - cs.z) - gsz;
-  dx += std::fabs(dx); dx *= 0.5f;
-  dy += std::fabs(dy); dy *= 0.5f;
-  dz += std::fabs(dz); dz *= 0.5f;
-  const float ds2 = dx * dx + dy * dy + dz * dz;
+template <typename R = float>
+static inline bool open_test(const typename V<R>::v4 &cs, R gcx, R gcy, R gcz, R gsx, R gsy,
+                             R gsz, const F2 *ls, float inv_theta) {
+  R dx = std::fabs(gcx - cs.x) - gsx;
+  R dy = std::fabs(gcy - cs.y) - gsy;
+  R dz = std::fabs(gcz - cs.z) - gsz;
+  dx += std::fabs(dx); dx *= R(0.5);
+  dy += std::fabs(dy); dy *= R(0.5);
+  dz += std::fabs(dz); dz *= R(0.5);
+  const R ds2 = dx * dx + dy * dy + dz * dz;
   if (!ls) return ds2 < std::fabs(cs.w);
-  const float op = ls->x * inv_theta + ls->y;
+  const R op = ls->x * inv_theta + ls->y;
   return ds2 < op * op;
 }
 
-template <typename A>
+// R: the real type of positions and cell data (float: Treecode<float>,
+// double: Treecode<double>, fixed angle only); A: the accumulator type.
+template <typename A, typename R = float>
 static void walk_impl(int ngroups, const I2 *groups, const int *group_cells,
-                      const int *level_range, const F4 *pos_ph, const F4 *pos_tree,
-                      const U4 *cells, const F4 *csize, const F4 *mono, const F4 *quad0,
-                      const F2 *quad1, float eps2, const F2 *cell_ls,
-                      const float *group_inv_theta, F4 *acc_out,
+                      const int *level_range, const typename V<R>::v4 *pos_ph,
+                      const typename V<R>::v4 *pos_tree, const U4 *cells,
+                      const typename V<R>::v4 *csize, const typename V<R>::v4 *mono,
+                      const typename V<R>::v4 *quad0, const typename V<R>::v2 *quad1, R eps2,
+                      const F2 *cell_ls, const float *group_inv_theta,
+                      typename V<R>::v4 *acc_out,
                       long long *stats_out, double *pot_out, int *counts_out,
                       const LearnCfg *lc = nullptr, bool monopole_only = false) {
   long long dsum = 0, asum = 0, dmax = 0, amax = 0;
@@ -722,17 +832,17 @@ static void walk_impl(int ngroups, const I2 *groups, const int *group_cells,
 #pragma omp for schedule(dynamic, 16)
     for (int g = 0; g < ngroups; g++) {
       const int pbeg = groups[g].x, np = groups[g].y;
-      float mn[3] = {pos_ph[pbeg].x, pos_ph[pbeg].y, pos_ph[pbeg].z};
-      float mx[3] = {mn[0], mn[1], mn[2]};
+      R mn[3] = {pos_ph[pbeg].x, pos_ph[pbeg].y, pos_ph[pbeg].z};
+      R mx[3] = {mn[0], mn[1], mn[2]};
       for (int i = pbeg; i < pbeg + np; i++) {
         mn[0] = std::min(mn[0], pos_ph[i].x); mx[0] = std::max(mx[0], pos_ph[i].x);
         mn[1] = std::min(mn[1], pos_ph[i].y); mx[1] = std::max(mx[1], pos_ph[i].y);
         mn[2] = std::min(mn[2], pos_ph[i].z); mx[2] = std::max(mx[2], pos_ph[i].z);
       }
-      const float gcx = 0.5f * (mx[0] + mn[0]), gcy = 0.5f * (mx[1] + mn[1]),
-                  gcz = 0.5f * (mx[2] + mn[2]);
-      const float gsx = 0.5f * (mx[0] - mn[0]), gsy = 0.5f * (mx[1] - mn[1]),
-                  gsz = 0.5f * (mx[2] - mn[2]);
+      const R gcx = R(0.5) * (mx[0] + mn[0]), gcy = R(0.5) * (mx[1] + mn[1]),
+              gcz = R(0.5) * (mx[2] + mn[2]);
+      const R gsx = R(0.5) * (mx[0] - mn[0]), gsy = R(0.5) * (mx[1] - mn[1]),
+              gsz = R(0.5) * (mx[2] - mn[2]);
       const float it = group_inv_theta ? group_inv_theta[g] : 0.0f;
       const F2 *lsp = group_inv_theta ? cell_ls : nullptr;
       int c0, nc;
@@ -745,7 +855,10 @@ static void walk_impl(int ngroups, const I2 *groups, const int *group_cells,
         for (int c : cur) {
           const U4 cd = cells[c];
           bool split;
-          if (lc) {
+          if constexpr (!std::is_same<R, float>::value) {
+            split = open_test<R>(csize[c], gcx, gcy, gcz, gsx, gsy, gsz, nullptr, 0.f) ||
+                    ((int)cd.w - (int)cd.z < 3);
+          } else if (lc) {
             const F4 &m = mono[c];
             const PairIn pin{gcx, gcy, gcz, gsx, gsy, gsz, np, m.x, m.y, m.z, m.w,
                              cell_ls[c].x, quad0[c].x, quad0[c].y, quad0[c].z, quad0[c].w,
@@ -789,14 +902,14 @@ static void walk_impl(int ngroups, const I2 *groups, const int *group_cells,
       if (counts_out) { counts_out[2 * g] = (int)na; counts_out[2 * g + 1] = (int)nd; }
       for (int i = pbeg; i < pbeg + np; i++) {
         A acc[4] = {0, 0, 0, 0};
-        const float px = pos_ph[i].x, py = pos_ph[i].y, pz = pos_ph[i].z;
+        const R px = pos_ph[i].x, py = pos_ph[i].y, pz = pos_ph[i].z;
         if (monopole_only)  // code/computeForces.cu:198-225: the cell as a point mass
-          for (int c : alist) p2p(acc, px, py, pz, mono[c], eps2);
+          for (int c : alist) p2p<A, R>(acc, px, py, pz, mono[c], eps2);
         else
-          for (int c : alist) p2c(acc, px, py, pz, mono[c], quad0[c], quad1[c], eps2);
-        for (int p : dlist) p2p(acc, px, py, pz, pos_tree[p], eps2);
-        acc_out[i] = F4{(float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]};
-        pot += 0.5 * (double)pos_ph[i].w * (double)(float)acc[3];
+          for (int c : alist) p2c<A, R>(acc, px, py, pz, mono[c], quad0[c], quad1[c], eps2);
+        for (int p : dlist) p2p<A, R>(acc, px, py, pz, pos_tree[p], eps2);
+        acc_out[i] = typename V<R>::v4{(R)acc[0], (R)acc[1], (R)acc[2], (R)acc[3]};
+        pot += 0.5 * (double)pos_ph[i].w * (double)(R)acc[3];
       }
       dsum += nd * np; asum += na * np;
       dmax = std::max(dmax, nd); amax = std::max(amax, na);
@@ -823,6 +936,17 @@ void or_walk(int ngroups, const I2 *groups, const int *group_cells, const int *l
                       stats_out, pot_out, counts_out, nullptr, monopole_only != 0);
 }
 
+// Treecode<double>: the same walk with FP64 positions, cell data and sums
+// (fixed opening angle).
+void or_walk_d(int ngroups, const I2 *groups, const int *level_range, const D4 *pos_ph,
+               const D4 *pos_tree, const U4 *cells, const D4 *csize, const D4 *mono,
+               const D4 *quad0, const D2 *quad1, double eps2, D4 *acc_out, long long *stats_out,
+               double *pot_out, int *counts_out, int monopole_only) {
+  walk_impl<double, double>(ngroups, groups, nullptr, level_range, pos_ph, pos_tree, cells,
+                            csize, mono, quad0, quad1, eps2, nullptr, nullptr, acc_out,
+                            stats_out, pot_out, counts_out, nullptr, monopole_only != 0);
+}
+
 // ---------------------------------------------------------------------------
 // FP64 direct sum (no reference counterpart; the error oracle).  For each
 // target t in idx[0..ns): a = sum_j m_j (x_j - x_t) / (r^2+eps2)^{3/2},
@@ -842,6 +966,23 @@ void or_direct_fp64(int n, const F4 *pos, int ns, const int *idx, double eps2, D
       const double mr = pos[j].w * ri;
       const double mr3 = mr * ri * ri;
       ax += mr3 * dx; ay += mr3 * dy; az += mr3 * dz; ph -= mr;
+    }
+    out[s] = D4{ax, ay, az, ph};
+  }
+}
+
+// The same for FP64 positions (Treecode<double>).
+void or_direct_fp64_d(int n, const D4 *pos, int ns, const int *idx, double eps2, D4 *out) {
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int s = 0; s < ns; s++) {
+    const D4 t = pos[idx[s]];
+    double ax = 0, ay = 0, az = 0, ph = 0;
+    for (int j = 0; j < n; j++) {
+      const double dx = pos[j].x - t.x, dy = pos[j].y - t.y, dz = pos[j].z - t.z;
+      const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+      const double ri = 1.0 / std::sqrt(r2);
+      const double mr3 = pos[j].w * ri * ri * ri;
+      ax += mr3 * dx; ay += mr3 * dy; az += mr3 * dz; ph -= pos[j].w * ri;
     }
     out[s] = D4{ax, ay, az, ph};
   }

@@ -44,6 +44,16 @@ def lib():
         L.or_make_groups.restype = ctypes.c_int
         L.or_walk.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, P, P, P, ctypes.c_float,
                               ctypes.c_int, P, P, P, P, P, P, ctypes.c_int]
+        L.or_bounding_box_d.argtypes = [ctypes.c_int, P, P, P, P]
+        L.or_geo_keys_d.argtypes = [ctypes.c_int, P, P, P]
+        L.or_build_tree_d.argtypes = [ctypes.c_int, P, ctypes.c_int, P, P, ctypes.c_int,
+                                      ctypes.c_int, P, P, P, P, P]
+        L.or_build_tree_d.restype = ctypes.c_int
+        L.or_multipoles_d.argtypes = [ctypes.c_int, P, P, ctypes.c_double, P, P, P, P, P]
+        L.or_ph_keys_d.argtypes = [ctypes.c_int, P, P, P]
+        L.or_walk_d.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, P, P, ctypes.c_double, P, P,
+                                P, P, ctypes.c_int]
+        L.or_direct_fp64_d.argtypes = [ctypes.c_int, P, ctypes.c_int, P, ctypes.c_double, P]
         L.or_walk_lists.argtypes = [ctypes.c_int, P, P, P, P, P, P, ctypes.c_float,
                                     ctypes.c_int, P, P, P, P]
         L.or_direct_fp64.argtypes = [ctypes.c_int, P, ctypes.c_int, P, ctypes.c_double, P]
@@ -87,8 +97,24 @@ def _f4(a):
     return a
 
 
+def _d4(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    assert a.ndim == 2 and a.shape[1] == 4
+    return a
+
+
+def _is64(pos):
+    return np.asarray(pos).dtype == np.float64
+
+
 def bounding_box(pos):
-    """code/buildTree.cu:80-166 -> (box[4] = cx,cy,cz,hsize, bmin[3], bmax[3])."""
+    """code/buildTree.cu:80-166 -> (box[4] = cx,cy,cz,hsize, bmin[3], bmax[3]).
+    float64 positions: the Treecode<double> box (extremes kept in FP64)."""
+    if _is64(pos):
+        pos = _d4(pos)
+        box, bmin, bmax = np.zeros(4), np.zeros(3), np.zeros(3)
+        lib().or_bounding_box_d(len(pos), _ptr(pos), _ptr(box), _ptr(bmin), _ptr(bmax))
+        return box, bmin, bmax
     pos = _f4(pos)
     box = np.zeros(4, np.float32)
     bmin = np.zeros(3, np.float32)
@@ -99,9 +125,13 @@ def bounding_box(pos):
 
 def geo_keys(pos, box):
     """Octant digits down the ChildBox recursion (code/buildTree.cu:233-247)."""
+    keys = np.zeros(len(pos), np.uint64)
+    if _is64(pos):
+        lib().or_geo_keys_d(len(pos), _ptr(_d4(pos)), _ptr(np.ascontiguousarray(box, np.float64)),
+                            _ptr(keys))
+        return keys
     pos = _f4(pos)
     box = np.ascontiguousarray(box, np.float32)
-    keys = np.zeros(len(pos), np.uint64)
     lib().or_geo_keys(len(pos), _ptr(pos), _ptr(box), _ptr(keys))
     return keys
 
@@ -119,15 +149,17 @@ class Tree:
 
 
 def build_tree(pos, nleaf, box=None, offsets=None, boxes=None):
-    pos = _f4(pos)
+    fp64 = _is64(pos)
+    pos = _d4(pos) if fp64 else _f4(pos)
+    rt = np.float64 if fp64 else np.float32
     n = len(pos)
     if offsets is None:
         offsets = np.array([0, n], np.int32)
         if box is None:
             box = bounding_box(pos)[0]
-        boxes = np.ascontiguousarray(box, np.float32).reshape(1, 4)
+        boxes = np.ascontiguousarray(box, rt).reshape(1, 4)
     offsets = np.ascontiguousarray(offsets, np.int32)
-    boxes = np.ascontiguousarray(boxes, np.float32)
+    boxes = np.ascontiguousarray(boxes, rt)
     nroot = len(offsets) - 1
     ncell_max = 3 * n + 64 * nroot + 1024   # the largest tree (levels add <= n cells each)
     order = np.zeros(n, np.int32)
@@ -135,9 +167,9 @@ def build_tree(pos, nleaf, box=None, offsets=None, boxes=None):
     level_range = np.zeros(64, np.int32)
     root_cells = np.zeros(2 * nroot, np.int32)
     stats = np.zeros(6, np.int32)
-    rc = lib().or_build_tree(n, _ptr(pos), nroot, _ptr(offsets), _ptr(boxes), int(nleaf),
-                             ncell_max, _ptr(order), _ptr(cells), _ptr(level_range),
-                             _ptr(root_cells), _ptr(stats))
+    fn = lib().or_build_tree_d if fp64 else lib().or_build_tree
+    rc = fn(n, _ptr(pos), nroot, _ptr(offsets), _ptr(boxes), int(nleaf), ncell_max, _ptr(order),
+            _ptr(cells), _ptr(level_range), _ptr(root_cells), _ptr(stats))
     if rc != 0:
         raise RuntimeError("or_build_tree failed: %d" % rc)
     t = Tree(order, cells, level_range.reshape(32, 2), root_cells.reshape(nroot, 2), stats)
@@ -161,12 +193,30 @@ def multipoles(cells, pos_tree, theta, mode=0):
     return size, mono, q0, q1, ls
 
 
+def multipoles_d(cells, pos_tree, theta):
+    """Treecode<double>: code/computeMultipoles.cu:81-190 in FP64 throughout ->
+    size, mono, quad0, quad1, ls (float64)."""
+    cells = np.ascontiguousarray(cells, np.uint32)
+    pos_tree = _d4(pos_tree)
+    nc = len(cells)
+    size, mono, q0 = np.zeros((nc, 4)), np.zeros((nc, 4)), np.zeros((nc, 4))
+    q1, ls = np.zeros((nc, 2)), np.zeros((nc, 2))
+    lib().or_multipoles_d(nc, _ptr(cells), _ptr(pos_tree), ctypes.c_double(1.0 / float(theta)),
+                          _ptr(size), _ptr(mono), _ptr(q0), _ptr(q1), _ptr(ls))
+    return size, mono, q0, q1, ls
+
+
 def hilbert_key(x, y, z, nbits=21):
     return int(lib().or_hilbert_key(int(x), int(y), int(z), int(nbits)))
 
 
 def ph_keys(pos, box):
     """code/makeGroups.cu:19-135."""
+    if _is64(pos):
+        keys = np.zeros(len(pos), np.uint64)
+        lib().or_ph_keys_d(len(pos), _ptr(_d4(pos)), _ptr(np.ascontiguousarray(box, np.float64)),
+                           _ptr(keys))
+        return keys
     pos = _f4(pos)
     box = np.ascontiguousarray(box, np.float32)
     keys = np.zeros(len(pos), np.uint64)
@@ -280,8 +330,55 @@ def walk_lists(g, groups, level_range, pos_ph, cells, size, ls=None, theta=None,
     return alist[:na[0]].copy(), leaves[:nl[0]].copy()
 
 
+def walk_d(groups, level_range, pos_ph, pos_tree, cells, size, mono, q0, q1, eps,
+           monopole=False):
+    """Treecode<double>: the walk of code/computeForces.cu:30-727 in FP64 ->
+    (acc[N,4] float64 PH order, stats[4], pot, per-group counts)."""
+    groups = np.ascontiguousarray(groups, np.int32)
+    ng = len(groups)
+    acc = np.zeros((len(pos_ph), 4))
+    stats = np.zeros(4, np.int64)
+    pot = np.zeros(1)
+    cnt = np.zeros((ng, 2), np.int32)
+    lib().or_walk_d(ng, _ptr(groups), _ptr(np.ascontiguousarray(level_range, np.int32).reshape(-1)),
+                    _ptr(_d4(pos_ph)), _ptr(_d4(pos_tree)),
+                    _ptr(np.ascontiguousarray(cells, np.uint32)), _ptr(_d4(size)), _ptr(_d4(mono)),
+                    _ptr(_d4(q0)), _ptr(np.ascontiguousarray(q1, np.float64)),
+                    ctypes.c_double(float(eps) ** 2), _ptr(acc), _ptr(stats), _ptr(pot), _ptr(cnt),
+                    int(bool(monopole)))
+    return acc, stats, float(pot[0]), cnt
+
+
+def reference_pipeline_d(pos, nleaf, ncrit, theta, eps, level_split=5):
+    """Treecode<double> through the reference call order on the oracle (FP64
+    positions, box, keys, moments and walk)."""
+    pos = _d4(pos)
+    box, _, _ = bounding_box(pos)
+    tree = build_tree(pos, nleaf, box)
+    pos_tree = pos[tree.order]
+    size, mono, q0, q1, ls = multipoles_d(tree.cells, pos_tree, theta)
+    keys = ph_keys(pos_tree, box)
+    perm = sort_keys(keys)
+    skeys = keys[perm]
+    pos_ph = pos_tree[perm]
+    groups = make_groups(skeys, level_split, ncrit)
+    acc, stats, pot, cnt = walk_d(groups, tree.level_range, pos_ph, pos_tree, tree.cells, size,
+                                  mono, q0, q1, eps)
+    return dict(box=box, tree=tree, pos_tree=pos_tree, size=size, mono=mono, q0=q0, q1=q1,
+                ls=ls, keys=keys, perm=perm, sorted_keys=skeys, pos_ph=pos_ph, groups=groups,
+                acc=acc, stats=stats, pot=pot, counts=cnt)
+
+
 def direct_fp64(pos, idx, eps, longdouble=False):
-    """FP64 direct sum over all sources for targets idx -> float64 (ns, 4)."""
+    """FP64 direct sum over all sources for targets idx -> float64 (ns, 4).
+    float64 positions: sources and targets in FP64 (Treecode<double>)."""
+    if _is64(pos):
+        pos = _d4(pos)
+        idx = np.ascontiguousarray(idx, np.int32)
+        out = np.zeros((len(idx), 4), np.float64)
+        lib().or_direct_fp64_d(len(pos), _ptr(pos), len(idx), _ptr(idx),
+                               ctypes.c_double(float(eps) ** 2), _ptr(out))
+        return out
     pos = _f4(pos)
     idx = np.ascontiguousarray(idx, np.int32)
     out = np.zeros((len(idx), 4), np.float64)

@@ -1,34 +1,25 @@
 // ref_caller.cpp -- a caller of the drop-in include/Treecode.h written to the
 // call sequence of the reference driver, code/treecode.cu:61-165: the same
-// Treecode<float> members in the same order, the reference's double4
-// statistics and TFLOP/s line.  The reference's Plummer class (code/plummer.h)
+// Treecode<real_t> members in the same order and the reference's double4
+// statistics.  The reference's Plummer class (code/plummer.h)
 // is not part of the library, so the particles come from bt_ic_plummer.
 //
-// Usage: ref_caller N NLEAF NGROUP THETA EPS OUT
-// OUT (binary): double[8] = interactions.xyzw, grav_potential after
-// computeForces, ekin, epot, grav_potential after computeEnergies; then
-// float4[N] d_ptclAcc (PH order, read through the d_ptclAcc view after
-// computeForces); then float4[N] h_ptclPos after moveParticles +
-// computeEnergies + ptcl_d2h (tree order of the last build).
+// Usage: ref_caller N NLEAF NGROUP THETA EPS OUT [f64]
+// With f64 the caller is Treecode<double> (particles converted from the
+// float sample).  OUT (binary): double[8] = interactions.xyzw,
+// grav_potential after computeForces, ekin, epot, grav_potential after
+// computeEnergies; then real4[N] d_ptclAcc (PH order, read through the
+// d_ptclAcc view after computeForces); then real4[N] h_ptclPos after
+// moveParticles + computeEnergies + ptcl_d2h (tree order of the last build).
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "Treecode.h"
 
-int main(int argc, char *argv[]) {
-  if (argc != 7) {
-    std::fprintf(stderr, "usage: %s N NLEAF NGROUP THETA EPS OUT\n", argv[0]);
-    return 2;
-  }
-  const int nPtcl = std::atoi(argv[1]);
-  const int nLeaf = std::atoi(argv[2]);
-  const int nGroup = std::atoi(argv[3]);
-  typedef float real_t;
-  const real_t theta = (real_t)std::atof(argv[4]);
-  const real_t eps = (real_t)std::atof(argv[5]);
-  const char *out = argv[6];
-
+template <typename real_t>
+int run(int nPtcl, int nLeaf, int nGroup, real_t theta, real_t eps, const char *out) {
   std::vector<float> ppos(4 * (size_t)nPtcl), pvel(4 * (size_t)nPtcl);
   if (bt_ic_plummer(nPtcl, 19810614ull, 22.8, ppos.data(), pvel.data()) != 0) return 3;
 
@@ -62,7 +53,7 @@ int main(int argc, char *argv[]) {
   const double pot_forces = tree.grav_potential;
   std::fprintf(stderr, " direct: <n>= %g  max= %g  approx: <n>=%g max= %g\n", interactions.x,
                interactions.y, interactions.z, interactions.w);
-  std::vector<Tree::Particle> acc(nPtcl);
+  std::vector<typename Tree::Particle> acc(nPtcl);
   tree.d_ptclAcc.d2h(acc.data());
   if (tree.d_ptclAcc.size() != (size_t)nPtcl || tree.d_cellDataList.size() == 0 ||
       tree.d_groupList.size() == 0 || tree.d_ptclAcc.d() == nullptr)
@@ -77,8 +68,23 @@ int main(int argc, char *argv[]) {
   const double head[8] = {interactions.x, interactions.y, interactions.z, interactions.w,
                           pot_forces, tree.ekin, tree.epot, tree.grav_potential};
   std::fwrite(head, sizeof(double), 8, f);
-  std::fwrite(acc.data(), sizeof(Tree::Particle), nPtcl, f);
-  std::fwrite(&tree.h_ptclPos[0], sizeof(Tree::Particle), nPtcl, f);
+  std::fwrite(acc.data(), sizeof(typename Tree::Particle), nPtcl, f);
+  std::fwrite(&tree.h_ptclPos[0], sizeof(typename Tree::Particle), nPtcl, f);
   std::fclose(f);
   return 0;
+}
+
+int main(int argc, char *argv[]) {
+  if (argc != 7 && argc != 8) {
+    std::fprintf(stderr, "usage: %s N NLEAF NGROUP THETA EPS OUT [f64]\n", argv[0]);
+    return 2;
+  }
+  const int nPtcl = std::atoi(argv[1]);
+  const int nLeaf = std::atoi(argv[2]);
+  const int nGroup = std::atoi(argv[3]);
+  const double theta = std::atof(argv[4]);
+  const double eps = std::atof(argv[5]);
+  if (argc == 8 && std::strcmp(argv[7], "f64") == 0)
+    return run<double>(nPtcl, nLeaf, nGroup, theta, eps, argv[6]);
+  return run<float>(nPtcl, nLeaf, nGroup, (float)theta, (float)eps, argv[6]);
 }

@@ -66,3 +66,36 @@ def test_reference_call_sequence(bt, tmp_path):
             s.close()
     finally:
         t.close()
+
+
+def test_reference_call_sequence_double(bt, tmp_path):
+    """The same caller instantiating Treecode<double>: its FP64 results equal
+    the Python mirror's FP64 context bit for bit."""
+    n, nleaf, ngroup, theta, eps = 1 << 15, 16, 64, 0.4, 0.01
+    out = str(tmp_path / "ref_caller64.bin")
+    print("ref_caller f64 n=%d" % n, flush=True)
+    r = subprocess.run([EXE, str(n), str(nleaf), str(ngroup), repr(theta), repr(eps), out, "f64"],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
+    raw = np.fromfile(out, np.uint8)
+    head = raw[:64].view(np.float64)
+    acc_c = raw[64:64 + 32 * n].view(np.float64).reshape(n, 4)
+    pos_c = raw[64 + 32 * n:].view(np.float64).reshape(n, 4)
+    pos, vel = bt.plummer(n, seed=19810614)
+    t = bt.Treecode(eps, theta, 64, fp64=True)
+    try:
+        t.alloc(n)
+        t.ptcl_h2d(pos.astype(np.float64), vel.astype(np.float64))
+        t.buildTree(nleaf)
+        t.computeMultipoles()
+        t.makeGroups(5, ngroup)
+        st = t.computeForces()
+        assert tuple(head[:4]) == st and head[4] == t.grav_potential
+        assert np.array_equal(acc_c, t.acc(bt.ORDER_PH))
+        t.moveParticles(1.0 / 64)
+        ek, ep = t.computeEnergies()
+        assert (head[5], head[6]) == (ek[0], ep[0])
+        assert np.array_equal(pos_c, t.ptcl_d2h()[0])
+    finally:
+        t.close()

@@ -464,3 +464,75 @@ def test_opening_rule_fp64_clamp_distance(oracle, bt):
                 found[mut] = True
     assert between > 0
     assert all(found.values()), found
+
+
+# ---- the Treecode<double> oracle -------------------------------------------
+
+def test_fp64_oracle_dyadic_inputs_build_the_float_tree(oracle, bt):
+    """On inputs whose coordinates are short dyadic rationals, every ChildBox
+    and quantisation step is exact in both precisions: the FP64 box, tree,
+    order and keys equal the FP32 ones."""
+    rng = np.random.default_rng(12)
+    p = np.zeros((6000, 4))
+    p[:, :3] = rng.integers(-2 ** 12, 2 ** 12, size=(6000, 3)) / 2.0 ** 12
+    p[:, 3] = 1.0 / 6000
+    r = oracle.reference_pipeline(p.astype(np.float32), 16, 64, 0.5, 0.01)
+    d = oracle.reference_pipeline_d(p, 16, 64, 0.5, 0.01)
+    assert np.array_equal(d["box"], r["box"].astype(np.float64))
+    assert np.array_equal(d["tree"].cells, r["tree"].cells)
+    assert np.array_equal(d["tree"].order, r["tree"].order)
+    assert np.array_equal(d["sorted_keys"], r["sorted_keys"])
+    assert np.array_equal(d["groups"], r["groups"])
+    a = d["acc"][:, :3]
+    assert np.sqrt(((r["acc"][:, :3] - a) ** 2).sum() / (a ** 2).sum()) < 1e-5
+
+
+def test_fp64_oracle_multipoles_closed_forms_and_parallel_axis(oracle, bt):
+    a = 0.25
+    p = np.array([[a, 0, 0, 1], [-a, 0, 0, 1]], np.float64)
+    cells = np.array([[1 << 27, 0xFFFFFFFF, 0, 2]], np.uint32)
+    size, mono, q0, q1, ls = oracle.multipoles_d(cells, p, 0.5)
+    assert mono[0, 3] == 2 and not mono[0, :3].any()
+    assert q0[0, 0] == a * a and not q0[0, 1:].any() and not q1.any()
+    assert ls[0, 0] == 2 * a and size[0, 3] == (2 * a / 0.5) ** 2
+    pos, _ = bt.plummer(5000, seed=4)
+    pos = pos.astype(np.float64)
+    pos[:, :3] += np.random.default_rng(1).uniform(-1e-9, 1e-9, (5000, 3))  # digits below FP32
+    t = oracle.build_tree(pos, 16)
+    pt = pos[t.order]
+    size, mono, q0, q1, ls = oracle.multipoles_d(t.cells, pt, 0.5)
+    Q = np.zeros((len(t.cells), 3, 3))
+    Q[:, 0, 0], Q[:, 1, 1], Q[:, 2, 2] = q0[:, 0], q0[:, 1], q0[:, 2]
+    Q[:, 0, 1] = Q[:, 1, 0] = q0[:, 3]
+    Q[:, 0, 2] = Q[:, 2, 0] = q1[:, 0]
+    Q[:, 1, 2] = Q[:, 2, 1] = q1[:, 1]
+    cells = t.cells.astype(np.int64)
+    for c in np.where(cells[:, 1] != 0xFFFFFFFF)[0][:200]:
+        f, n = cells[c, 1] & 0x1FFFFFFF, (cells[c, 1] >> 29) + 1
+        kids = np.arange(f, f + n)
+        m = mono[kids, 3]
+        x = mono[kids, :3]
+        M = m.sum()
+        com = (m[:, None] * x).sum(0) / M
+        dd = x - com
+        Qp = ((m[:, None, None] * (Q[kids] + dd[:, :, None] * dd[:, None, :])).sum(0)) / M
+        assert np.allclose(com, mono[c, :3], rtol=0, atol=1e-14 * max(1.0, np.abs(com).max()))
+        assert np.allclose(Qp, Q[c], rtol=0, atol=1e-12 * ls[c, 0] ** 2)
+
+
+def test_fp64_oracle_walk_theta_to_zero_is_direct_sum(oracle, bt):
+    pos, _ = bt.plummer(2000, seed=2)
+    pos = pos.astype(np.float64)
+    r = oracle.reference_pipeline_d(pos, 16, 64, 1e-4, 0.01)
+    assert r["stats"][2] == 0
+    d = oracle.direct_fp64(r["pos_ph"], np.arange(2000, dtype=np.int32), 0.01)
+    rel = np.linalg.norm(r["acc"][:, :3] - d[:, :3], axis=1) / np.linalg.norm(d[:, :3], axis=1)
+    assert rel.max() < 1e-12
+    assert np.allclose(r["acc"][:, 3], d[:, 3], rtol=1e-12, atol=0)
+    # the FP64 direct sum itself against numpy brute force
+    x = r["pos_ph"][:64]
+    dx = x[None, :, :3] - x[:, None, :3]
+    r2 = (dx ** 2).sum(-1) + 0.01 ** 2
+    acc = (x[None, :, 3, None] * dx / r2[..., None] ** 1.5).sum(1)
+    d64 = oracle.direct_fp64(x, np.arange(64, dtype=np.int32), 0.01)
+    assert np.allclose(d64[:, :3], acc, rtol=1e-13, atol=1e-13 * np.abs(acc).max())
