@@ -392,6 +392,10 @@ class Treecode:
         ek = np.zeros(self.nsys, np.float64)
         ep = np.zeros(self.nsys, np.float64)
         self._ck(self._L.bt_compute_energies(self._h, _p(ek), _p(ep)), "computeEnergies")
+        # the forces (and 1/2 sum m phi) may have been re-evaluated: as the C++ mirror
+        pot = np.zeros(1, np.float64)
+        self._ck(self._L.bt_get_potential(self._h, _p(pot)), "grav_potential")
+        self.grav_potential = float(pot[0])
         return ek, ep
 
     def step(self, dt, nLeaf=16, levelSplit=5, nCrit=64):

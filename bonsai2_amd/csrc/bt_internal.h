@@ -10,6 +10,7 @@
 
 namespace bt {
 
+constexpr int kSortErr = 8;   // DevCounters::err bit: a sort pass's look-back wait timed out
 constexpr int kLevels = 32;   // level table slots (reference int2[32], code/Treecode.h:171)
 #ifndef BT_LEVEL_BLOCKS
 #define BT_LEVEL_BLOCKS 2048
@@ -50,7 +51,7 @@ struct DevCounters {
   int nnodes, nleaves, nlevels, ncells;
   int ngroups;
   int flags;   // BT_FLAG_*
-  int err;     // 1 = cell capacity, 2 = group capacity, 4 = walk stack
+  int err;     // 1 = cell capacity, 2 = group capacity, 4 = walk stack, 8 = sort
   int queue;   // walk group queue head
   int stack_max;  // deepest walk stack of the last walk (entries)
   int xq[8];      // walk group queues, one per XCD range
@@ -107,6 +108,14 @@ struct Ctx {
   int *valA = nullptr, *valB = nullptr;
   int *sort_hist = nullptr;   // 256 x blocks
   int sort_hist_cap = 0;
+  // one-sweep passes: per-pass digit counts of every pass (from one histogram
+  // launch), tile status words (two buffers, pass parity) and tile counters
+  int *sort_hpart = nullptr;        // kSortHistBlocks x passes x 256 partial counts
+  int *sort_gcount = nullptr;       // passes x 256 global digit counts
+  unsigned int *sort_status = nullptr;  // 2 x tiles x 256
+  int *sort_ctr = nullptr;          // 2 tile counters
+  int sort_tiles_cap = 0;
+  bool sort_onesweep = true;
   uint64_t *geo_keys = nullptr;   // sorted geometric keys (tree order)
 
   // scans

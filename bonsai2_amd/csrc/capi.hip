@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -138,6 +139,13 @@ void record(Ctx &c, int k) {
   if (c.timing) (void)hipEventRecord(c.ev[k], c.stream);
 }
 
+// Device-to-host copy ordered after the work queued on the context stream
+// (the stream is non-blocking: a plain hipMemcpy would not wait for it).
+hipError_t d2h(Ctx &c, void *dst, const void *src, size_t bytes) {
+  const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream);
+  return e != hipSuccess ? e : hipStreamSynchronize(c.stream);
+}
+
 int read_counters(Ctx &c) {
   BT_CK(hipMemcpyAsync(&c.hc, c.dc, sizeof(DevCounters), hipMemcpyDeviceToHost, c.stream));
   BT_CK(hipStreamSynchronize(c.stream));
@@ -196,6 +204,7 @@ int phase_forces(Ctx &c) {
   return 0;
 }
 
+const char *const kSortErrMsg = "radix sort: a pass's look-back wait timed out";
 const char *const kGroupCapacityMsg =
     "group capacity exceeded (leaf groups: a leaf holds more than 64 particles)";
 
@@ -230,6 +239,7 @@ void launch_learned_check(Ctx &c) {
 
 int check_build_errors(Ctx &c) {
   if (read_counters(c)) return BT_ERR_HIP;
+  if (c.hc.err & kSortErr) return fail(c, BT_ERR_HIP, kSortErrMsg);
   if (c.hc.err & 1) return fail(c, BT_ERR_CAPACITY, "cell capacity exceeded");
   return 0;
 }
@@ -253,6 +263,7 @@ int regrow_cells(Ctx &c) {
 int build_checked(Ctx &c) {
   if (phase_build(c)) return BT_ERR_HIP;
   if (read_counters(c)) return BT_ERR_HIP;
+  if (c.hc.err & kSortErr) return fail(c, BT_ERR_HIP, kSortErrMsg);
   if (c.hc.err & 1) {
     const int rc = regrow_cells(c);
     if (rc) return rc;
@@ -289,6 +300,7 @@ int finish_step(Ctx &c, double dt) {
     if (rc) return rc;
     if (read_counters(c)) return BT_ERR_HIP;
   }
+  if (c.hc.err_sticky & kSortErr) return fail(c, BT_ERR_HIP, kSortErrMsg);
   if (c.hc.err_sticky & 1) return fail(c, BT_ERR_CAPACITY, "cell capacity exceeded");
   if (c.hc.err_sticky & 2) return fail(c, BT_ERR_CAPACITY, kGroupCapacityMsg);
   if (c.hc.err_sticky & 4) return fail(c, BT_ERR_STACK, "walk stack overflow");
@@ -341,6 +353,9 @@ int create_ctx(bt_ctx **out, double eps, double theta, int ncrit, bool fp64) {
     return BT_ERR_HIP;
   }
   c.own_stream = true;
+  // BT_SORT_ONESWEEP=0 selects the three-launch radix passes (histogram, scan,
+  // scatter) instead of the one-sweep passes: same result, for comparison
+  if (const char *e = std::getenv("BT_SORT_ONESWEEP")) c.sort_onesweep = std::atoi(e) != 0;
   for (int k = 0; k < 12; k++) (void)hipEventCreate(&c.ev[k]);
   if (dev_alloc(c, &c.dc, 1) || dev_alloc(c, &c.level_beg, kLevels) ||
       dev_alloc(c, &c.d_wstats, 4 + 4 * (size_t)kWalkStatBlocks) || dev_alloc(c, &c.d_wpot, 1) ||
@@ -392,6 +407,8 @@ int bt_destroy(bt_ctx *h) {
   dev_free(&c.d_wpot);
   dev_free(&c.lvl_blk);
   dev_free(&c.sort_hist);
+  dev_free(&c.sort_hpart); dev_free(&c.sort_gcount); dev_free(&c.sort_status);
+  dev_free(&c.sort_ctr);
   dev_free(&c.scan_blk);
   dev_free(&c.d_direct);
   dev_free(&c.mlp_params); dev_free(&c.mlp_m); dev_free(&c.mlp_v); dev_free(&c.mlp_dev);
@@ -621,6 +638,7 @@ int bt_make_groups(bt_ctx *h, int level_split, int ncrit) {
   record(c, 4);
   if (phase_groups(c)) return BT_ERR_HIP;
   if (read_counters(c)) return BT_ERR_HIP;
+  if (c.hc.err & kSortErr) return fail(c, BT_ERR_HIP, kSortErrMsg);
   if (c.hc.err & 2) return fail(c, BT_ERR_CAPACITY, kGroupCapacityMsg);
   c.have_groups = true;
   c.have_acc = false;
@@ -646,7 +664,7 @@ int bt_compute_forces(bt_ctx *h, int intcount, double stats[4]) {
   c.have_acc = true;
   if (intcount && stats) {
     unsigned long long w[4];
-    BT_CK(hipMemcpy(w, c.d_wstats, sizeof(w), hipMemcpyDeviceToHost));
+    BT_CK(d2h(c, w, c.d_wstats, sizeof(w)));
     stats[0] = (double)w[0] / c.n;
     stats[1] = (double)w[1];
     stats[2] = (double)w[2] / c.n;
@@ -838,7 +856,8 @@ int bt_get_walk_stats(bt_ctx *h, double stats[4], double totals2[2]) {
   Ctx &c = h->c;
   if (!c.have_acc) return fail(c, BT_ERR_STATE, "no forces");
   unsigned long long w[4];
-  BT_CK(hipMemcpy(w, c.d_wstats, sizeof(w), hipMemcpyDeviceToHost));
+  BT_CK(hipMemcpyAsync(w, c.d_wstats, sizeof(w), hipMemcpyDeviceToHost, c.stream));
+  BT_CK(hipStreamSynchronize(c.stream));
   stats[0] = (double)w[0] / c.n;
   stats[1] = (double)w[1];
   stats[2] = (double)w[2] / c.n;
@@ -854,7 +873,9 @@ int bt_get_potential(bt_ctx *h, double *pot) {
   if (!h || !pot) return BT_ERR_ARG;
   Ctx &c = h->c;
   if (!c.have_acc) return fail(c, BT_ERR_STATE, "no forces");
-  BT_CK(hipMemcpy(pot, c.d_wpot, sizeof(double), hipMemcpyDeviceToHost));
+  // on the context stream: a plain hipMemcpy would not wait for its work
+  BT_CK(hipMemcpyAsync(pot, c.d_wpot, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  BT_CK(hipStreamSynchronize(c.stream));
   return 0;
 }
 
@@ -881,15 +902,15 @@ int bt_get_tree(bt_ctx *h, uint32_t *cells4, int *level_range64, float *size4, f
   if ((size4 || mono4 || q0_4 || q1_2) && check_real<float>(c)) return BT_ERR_STATE;
   if (read_counters(c)) return BT_ERR_HIP;
   const size_t nc = (size_t)c.hc.ncells;
-  if (cells4) BT_CK(hipMemcpy(cells4, c.cells, sizeof(uint4) * nc, hipMemcpyDeviceToHost));
+  if (cells4) BT_CK(d2h(c, cells4, c.cells, sizeof(uint4) * nc));
   if (level_range64)
-    BT_CK(hipMemcpy(level_range64, c.level_beg, sizeof(int2) * kLevels, hipMemcpyDeviceToHost));
+    BT_CK(d2h(c, level_range64, c.level_beg, sizeof(int2) * kLevels));
   if ((size4 || mono4 || q0_4 || q1_2) && !c.have_mp)
     return fail(c, BT_ERR_STATE, "no multipoles");
-  if (size4) BT_CK(hipMemcpy(size4, c.csize, sizeof(float4) * nc, hipMemcpyDeviceToHost));
-  if (mono4) BT_CK(hipMemcpy(mono4, c.cmono, sizeof(float4) * nc, hipMemcpyDeviceToHost));
-  if (q0_4) BT_CK(hipMemcpy(q0_4, c.cq0, sizeof(float4) * nc, hipMemcpyDeviceToHost));
-  if (q1_2) BT_CK(hipMemcpy(q1_2, c.cq1, sizeof(float2) * nc, hipMemcpyDeviceToHost));
+  if (size4) BT_CK(d2h(c, size4, c.csize, sizeof(float4) * nc));
+  if (mono4) BT_CK(d2h(c, mono4, c.cmono, sizeof(float4) * nc));
+  if (q0_4) BT_CK(d2h(c, q0_4, c.cq0, sizeof(float4) * nc));
+  if (q1_2) BT_CK(d2h(c, q1_2, c.cq1, sizeof(float2) * nc));
   return 0;
 }
 
@@ -900,10 +921,10 @@ int bt_get_tree_f64(bt_ctx *h, double *size4, double *mono4, double *q0_4, doubl
   if (!c.have_mp) return fail(c, BT_ERR_STATE, "no multipoles");
   if (read_counters(c)) return BT_ERR_HIP;
   const size_t nc = (size_t)c.hc.ncells;
-  if (size4) BT_CK(hipMemcpy(size4, c.csize, sizeof(double4) * nc, hipMemcpyDeviceToHost));
-  if (mono4) BT_CK(hipMemcpy(mono4, c.cmono, sizeof(double4) * nc, hipMemcpyDeviceToHost));
-  if (q0_4) BT_CK(hipMemcpy(q0_4, c.cq0, sizeof(double4) * nc, hipMemcpyDeviceToHost));
-  if (q1_2) BT_CK(hipMemcpy(q1_2, c.cq1, sizeof(double2) * nc, hipMemcpyDeviceToHost));
+  if (size4) BT_CK(d2h(c, size4, c.csize, sizeof(double4) * nc));
+  if (mono4) BT_CK(d2h(c, mono4, c.cmono, sizeof(double4) * nc));
+  if (q0_4) BT_CK(d2h(c, q0_4, c.cq0, sizeof(double4) * nc));
+  if (q1_2) BT_CK(d2h(c, q1_2, c.cq1, sizeof(double2) * nc));
   return 0;
 }
 
@@ -913,7 +934,7 @@ int bt_get_cell_ls(bt_ctx *h, float *ls2) {
   if (check_real<float>(c)) return BT_ERR_STATE;
   if (!c.have_mp) return fail(c, BT_ERR_STATE, "no multipoles");
   if (read_counters(c)) return BT_ERR_HIP;
-  BT_CK(hipMemcpy(ls2, c.cls, sizeof(float2) * (size_t)c.hc.ncells, hipMemcpyDeviceToHost));
+  BT_CK(d2h(c, ls2, c.cls, sizeof(float2) * (size_t)c.hc.ncells));
   return 0;
 }
 
@@ -922,8 +943,7 @@ int bt_get_groups(bt_ctx *h, int *groups2) {
   Ctx &c = h->c;
   if (!c.have_groups) return fail(c, BT_ERR_STATE, "no groups");
   if (read_counters(c)) return BT_ERR_HIP;
-  BT_CK(hipMemcpy(groups2, c.groups, sizeof(int2) * (size_t)c.hc.ngroups,
-                  hipMemcpyDeviceToHost));
+  BT_CK(d2h(c, groups2, c.groups, sizeof(int2) * (size_t)c.hc.ngroups));
   return 0;
 }
 
@@ -932,8 +952,7 @@ int bt_get_group_counts(bt_ctx *h, int *counts2) {
   Ctx &c = h->c;
   if (!c.have_acc) return fail(c, BT_ERR_STATE, "no forces");
   if (read_counters(c)) return BT_ERR_HIP;
-  BT_CK(hipMemcpy(counts2, c.gcounts, sizeof(int2) * (size_t)c.hc.ngroups,
-                  hipMemcpyDeviceToHost));
+  BT_CK(d2h(c, counts2, c.gcounts, sizeof(int2) * (size_t)c.hc.ngroups));
   return 0;
 }
 
@@ -941,8 +960,8 @@ int bt_get_ph(bt_ctx *h, uint64_t *keys, int *perm) {
   if (!h) return BT_ERR_ARG;
   Ctx &c = h->c;
   if (!c.have_groups) return fail(c, BT_ERR_STATE, "no groups");
-  if (keys) BT_CK(hipMemcpy(keys, c.ph_keys, sizeof(uint64_t) * c.n, hipMemcpyDeviceToHost));
-  if (perm) BT_CK(hipMemcpy(perm, c.perm, sizeof(int) * c.n, hipMemcpyDeviceToHost));
+  if (keys) BT_CK(d2h(c, keys, c.ph_keys, sizeof(uint64_t) * c.n));
+  if (perm) BT_CK(d2h(c, perm, c.perm, sizeof(int) * c.n));
   return 0;
 }
 
@@ -951,10 +970,10 @@ int bt_get_ids(bt_ctx *h, int *ids, int order) {
   Ctx &c = h->c;
   if (order == BT_ORDER_TREE) {
     if (!c.have_tree) return fail(c, BT_ERR_STATE, "no tree");
-    BT_CK(hipMemcpy(ids, c.id, sizeof(int) * c.n, hipMemcpyDeviceToHost));
+    BT_CK(d2h(c, ids, c.id, sizeof(int) * c.n));
   } else if (order == BT_ORDER_PH) {
     if (!c.have_groups) return fail(c, BT_ERR_STATE, "no groups");
-    BT_CK(hipMemcpy(ids, c.id_ph, sizeof(int) * c.n, hipMemcpyDeviceToHost));
+    BT_CK(d2h(c, ids, c.id_ph, sizeof(int) * c.n));
   } else {
     for (int i = 0; i < c.n; i++) ids[i] = i;
   }
@@ -966,7 +985,7 @@ int bt_get_boxes(bt_ctx *h, float *boxes4) {
   Ctx &c = h->c;
   if (check_real<float>(c)) return BT_ERR_STATE;
   if (!c.have_tree) return fail(c, BT_ERR_STATE, "no tree");
-  BT_CK(hipMemcpy(boxes4, c.box, sizeof(float4) * c.nsys, hipMemcpyDeviceToHost));
+  BT_CK(d2h(c, boxes4, c.box, sizeof(float4) * c.nsys));
   return 0;
 }
 
@@ -975,7 +994,7 @@ int bt_get_boxes_f64(bt_ctx *h, double *boxes4) {
   Ctx &c = h->c;
   if (check_real<double>(c)) return BT_ERR_STATE;
   if (!c.have_tree) return fail(c, BT_ERR_STATE, "no tree");
-  BT_CK(hipMemcpy(boxes4, c.box, sizeof(double4) * c.nsys, hipMemcpyDeviceToHost));
+  BT_CK(d2h(c, boxes4, c.box, sizeof(double4) * c.nsys));
   return 0;
 }
 
@@ -1241,7 +1260,7 @@ int bt_learn_collect(bt_ctx *h, int nsel, const int *gsel, float theta_lo, float
   if (read_counters(c)) return BT_ERR_HIP;
   const int ng = c.hc.ngroups;
   std::vector<int2> grp(ng);
-  BT_CK(hipMemcpy(grp.data(), c.groups, sizeof(int2) * ng, hipMemcpyDeviceToHost));
+  BT_CK(d2h(c, grp.data(), c.groups, sizeof(int2) * ng));
   std::vector<int> pidx;
   for (int k = 0; k < nsel; k++) {
     if (gsel[k] < 0 || gsel[k] >= ng) return fail(c, BT_ERR_ARG, "group index out of range");

@@ -200,6 +200,238 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
 }
 
 
+// ---- one-sweep passes ------------------------------------------------------
+// The digit counts of every pass come from one histogram launch over the
+// input (the digits of a key do not change while it moves between passes).
+// Each pass is then one launch: a block takes the next tile from a counter,
+// ranks its pairs as above, publishes its per-digit counts and finds the
+// counts of all earlier tiles by looking back through their status words
+// (decoupled look-back), then writes its pairs to their sorted positions.
+// Per pass: 12 B read + 12 B written per pair.
+//
+// A status word is (flag << 30) | count, flag 1 = this tile's count only,
+// 2 = the inclusive count of tiles 0..t.  A block waits only for tiles that
+// were handed out before its own, whose blocks are already running and never
+// wait for later tiles, so the wait ends; it is bounded anyway and reports
+// kSortErr instead of hanging.  Two status buffers and two tile counters
+// alternate between passes: a pass clears the other buffer's words of its
+// tile and resets the other counter for the next pass.
+
+constexpr int kMaxSortPasses = 11;
+constexpr int kSortHistBlocks = 1024;
+constexpr unsigned kStAgg = 1u << 30, kStIncl = 2u << 30, kStCount = (1u << 30) - 1u;
+constexpr int kLookbackSpins = 1 << 22;
+
+struct PassSet {
+  int np;
+  int nsys;
+  const int *off;
+  int shift[kMaxSortPasses];
+  int bysys[kMaxSortPasses];
+};
+
+// Adds 1 for every valid lane at hist[d]: lanes with equal digits find each
+// other with 8 ballots and the lowest of them adds the group's size (no LDS
+// atomics on one address from many lanes of a wave).
+__device__ __forceinline__ void wave_count_digit(int *hist, int d, bool valid) {
+  uint64_t peers = ballot64(valid);
+#pragma unroll
+  for (int b = 0; b < 8; b++) {
+    const bool bit = (d >> b) & 1;
+    const uint64_t m = ballot64(bit);
+    peers &= bit ? m : ~m;
+  }
+  if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&hist[d], __popcll(peers));
+}
+
+// grid kSortHistBlocks: block b counts the digits of every pass over a
+// contiguous chunk; part[b][p][256].
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist_all(const uint64_t *__restrict__ keys,
+                                                                 const int *__restrict__ vals,
+                                                                 int n, PassSet ps,
+                                                                 int *__restrict__ part,
+                                                                 DevCounters *dc) {
+  __shared__ int hist[kMaxSortPasses * 256];
+  // the sort error bit reports this sort only (err_sticky keeps it for the step)
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAnd(&dc->err, ~kSortErr);
+  for (int k = threadIdx.x; k < ps.np * 256; k += kSortThreads) hist[k] = 0;
+  __syncthreads();
+  const int chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  for (int j0 = i0; j0 < i1; j0 += kSortThreads) {
+    const int i = j0 + threadIdx.x;
+    const bool valid = i < i1;
+    const uint64_t key = valid ? keys[i] : 0ull;
+    const int val = valid ? vals[i] : 0;
+    for (int p = 0; p < ps.np; p++) {
+      const DigitSrc src{ps.shift[p], ps.nsys, ps.off};
+      const int d = ps.bysys[p] ? digit_of<true>(key, val, src) : digit_of<false>(key, val, src);
+      wave_count_digit(hist + p * 256, d, valid);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < ps.np * 256; k += kSortThreads)
+    part[(size_t)blockIdx.x * ps.np * 256 + k] = hist[k];
+}
+
+// grid (passes, 8): block (p, g) sums digits [32g, 32g + 32) of pass p over
+// the histogram blocks, 8 segments of blocks per digit in parallel.
+__global__ __launch_bounds__(256) void k_radix_hist_sum(const int *__restrict__ part, int np,
+                                                        int nparts, int *__restrict__ gcount) {
+  __shared__ int sh[8][32];
+  const int p = blockIdx.x;
+  const int d = blockIdx.y * 32 + (threadIdx.x & 31);
+  const int seg = threadIdx.x >> 5;
+  const int per = (nparts + 7) / 8;
+  const int b0 = seg * per, b1 = min(nparts, b0 + per);
+  int s = 0;
+#pragma unroll 8
+  for (int b = b0; b < b1; b++) s += part[(size_t)b * np * 256 + p * 256 + d];
+  sh[seg][threadIdx.x & 31] = s;
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    int t = 0;
+    for (int k = 0; k < 8; k++) t += sh[k][threadIdx.x];
+    gcount[p * 256 + d] = t;
+  }
+}
+
+template <bool BYSYS>
+__global__ __launch_bounds__(kSortThreads) void k_radix_onesweep(
+    const uint64_t *__restrict__ keys, const int *__restrict__ vals, int n, DigitSrc src,
+    const int *__restrict__ gcount, unsigned int *status, unsigned int *status_next, int *ctr,
+    int *ctr_next, int ntiles, DevCounters *dc, uint64_t *__restrict__ keys_out,
+    int *__restrict__ vals_out) {
+  __shared__ uint64_t skeys[kSortTile];   // the tile in sorted order
+  __shared__ int svals[kSortTile];
+  __shared__ int tstart[256];     // first tile-local position of each digit
+  __shared__ int gbase[256];      // global position of this tile's first pair of each digit
+  __shared__ int wcnt[4][256];    // per-wave digit counts -> per-wave digit bases
+  __shared__ int sh_wave[4];
+  __shared__ int s_tile;
+  const int tid = threadIdx.x;
+  const int w = tid >> 6;
+  const int lane = lane_id();
+  if (tid == 0) s_tile = atomicAdd(ctr, 1);
+#pragma unroll
+  for (int k = 0; k < 4; k++) wcnt[k][tid] = 0;
+  __syncthreads();
+  const int tile = s_tile;
+  if (tile >= ntiles) {   // a counter not reset: never index past the status words
+    if (tid == 0) {
+      atomicOr(&dc->err, kSortErr);
+      atomicOr(&dc->err_sticky, kSortErr);
+    }
+    return;
+  }
+  if (tile == 0 && tid == 0) *ctr_next = 0;
+  status_next[(size_t)tile * 256 + tid] = 0u;
+  // Each wave ranks its own contiguous quarter of the tile (16 rounds of 64
+  // pairs) against its private digit counters; pairs stay in registers.
+  const int wbase = tile * kSortTile + w * (kSortRounds * kWave);
+  uint64_t kk[kSortRounds];
+  int vv[kSortRounds], dd[kSortRounds], rr[kSortRounds];
+#pragma unroll
+  for (int r = 0; r < kSortRounds; r++) {
+    const int i = wbase + r * kWave + lane;
+    kk[r] = i < n ? keys[i] : 0ull;
+    vv[r] = i < n ? vals[i] : 0;
+  }
+#pragma unroll
+  for (int r = 0; r < kSortRounds; r++) {
+    const int i = wbase + r * kWave + lane;
+    const bool valid = i < n;
+    const int d = valid ? digit_of<BYSYS>(kk[r], vv[r], src) : 0;
+    uint64_t peers = ballot64(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const bool bit = (d >> b) & 1;
+      const uint64_t m = ballot64(bit);
+      peers &= bit ? m : ~m;
+    }
+    const int rank = __popcll(peers & lanemask_lt());
+    const int before = wcnt[w][d];
+    wave_lds_sync();
+    if (valid && rank == 0) wcnt[w][d] = before + __popcll(peers);
+    wave_lds_sync();
+    dd[r] = valid ? d : -1;
+    rr[r] = before + rank;
+  }
+  __syncthreads();
+  // digit tid: this tile's count, published before anything else is done
+  const int cnt = wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+  unsigned int *mine = status + (size_t)tile * 256 + tid;
+  __hip_atomic_store(mine, (tile == 0 ? kStIncl : kStAgg) | (unsigned)cnt, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  // tile-local start of each digit, and the global start of each digit
+  int tot;
+  const int incl = block_incl_scan<OpSum>(cnt, sh_wave, &tot);
+  tstart[tid] = incl - cnt;
+  const int g = gcount[tid];
+  const int gincl = block_incl_scan<OpSum>(g, sh_wave, &tot);
+  // look back over the earlier tiles until one holds an inclusive count
+  int excl = 0;
+  if (tile > 0) {
+    int j = tile - 1;
+    int spins = 0;
+    for (;;) {
+      const unsigned v = __hip_atomic_load(status + (size_t)j * 256 + tid, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+      if ((v & ~kStCount) == 0u) {
+        if (++spins > kLookbackSpins) {
+          atomicOr(&dc->err, kSortErr);
+          atomicOr(&dc->err_sticky, kSortErr);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      excl += (int)(v & kStCount);
+      if ((v & ~kStCount) == kStIncl || j == 0) break;
+      j--;
+    }
+    __hip_atomic_store(mine, kStIncl | (unsigned)(excl + cnt), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  gbase[tid] = gincl - g + excl;
+  {
+    // digit tid: the waves' pairs follow each other in tile order
+    int acc = tstart[tid];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int c = wcnt[k][tid];
+      wcnt[k][tid] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortRounds; r++) {
+    if (dd[r] >= 0) {
+      const int pos = wcnt[w][dd[r]] + rr[r];
+      skeys[pos] = kk[r];
+      svals[pos] = vv[r];
+    }
+  }
+  __syncthreads();
+  const int tile_n = min(kSortTile, n - tile * kSortTile);
+  for (int j = tid; j < tile_n; j += kSortThreads) {
+    const uint64_t key = skeys[j];
+    const int val = svals[j];
+    const int d = digit_of<BYSYS>(key, val, src);
+    const int dst = gbase[d] + (j - tstart[d]);
+    keys_out[dst] = key;
+    vals_out[dst] = val;
+  }
+}
+
+__global__ void k_sort_clear(unsigned int *status, int nwords, int *ctr) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += gridDim.x * blockDim.x)
+    status[i] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ctr = 0;
+}
+
+
 template <class Op>
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int *__restrict__ in, int n,
                                                               int *__restrict__ blk) {
@@ -316,6 +548,12 @@ void scan_impl(Ctx &c, const int *in, int *out, int n, int *total_dev) {
                      in, n, c.scan_blk, out);
 }
 
+template <typename T>
+void dev_free_raw(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
 }  // namespace
 
 int grid_for(int n, int block, int cap) {
@@ -368,6 +606,56 @@ void radix_sort_pairs(Ctx &c, uint64_t *keys, int *vals, uint64_t *keys_tmp, int
   }
   uint64_t *ks = keys, *kd = keys_tmp;
   int *vs = vals, *vd = vals_tmp;
+  if (c.sort_onesweep && (int)passes.size() <= kMaxSortPasses) {
+    const int np = (int)passes.size();
+    if (nblocks > c.sort_tiles_cap || !c.sort_hpart) {
+      dev_free_raw(c.sort_hpart);
+      dev_free_raw(c.sort_gcount);
+      dev_free_raw(c.sort_status);
+      dev_free_raw(c.sort_ctr);
+      c.sort_tiles_cap = 0;
+      if (hipMalloc(&c.sort_hpart, sizeof(int) * kSortHistBlocks * kMaxSortPasses * 256) ||
+          hipMalloc(&c.sort_gcount, sizeof(int) * kMaxSortPasses * 256) ||
+          hipMalloc(&c.sort_status, sizeof(unsigned int) * 2 * 256 * (size_t)nblocks) ||
+          hipMalloc(&c.sort_ctr, sizeof(int) * 2)) {
+        c.err = "sort scratch allocation failed";
+        return;
+      }
+      c.sort_tiles_cap = nblocks;
+      hipLaunchKernelGGL(k_sort_clear, dim3(1024), dim3(256), 0, c.stream, c.sort_status,
+                         2 * 256 * nblocks, c.sort_ctr);
+      hipLaunchKernelGGL(k_sort_clear, dim3(1), dim3(64), 0, c.stream, c.sort_status, 0,
+                         c.sort_ctr + 1);
+    }
+    PassSet ps;
+    ps.np = np;
+    ps.nsys = c.nsys;
+    ps.off = c.d_off;
+    for (int k = 0; k < np; k++) {
+      ps.shift[k] = passes[k].shift;
+      ps.bysys[k] = passes[k].by_sys;
+    }
+    hipLaunchKernelGGL(k_radix_hist_all, dim3(kSortHistBlocks), dim3(kSortThreads), 0, c.stream,
+                       keys, vals, n, ps, c.sort_hpart, c.dc);
+    hipLaunchKernelGGL(k_radix_hist_sum, dim3(np, 8), dim3(256), 0, c.stream, c.sort_hpart, np,
+                       kSortHistBlocks, c.sort_gcount);
+    // pass k uses status buffer k & 1; every sort starts on a clean buffer 0
+    const size_t words = (size_t)256 * c.sort_tiles_cap;
+    for (int k = 0; k < np; k++) {
+      const int cur = k & 1, nxt = cur ^ 1;
+      DigitSrc src{passes[k].shift, c.nsys, c.d_off};
+      auto kern = passes[k].by_sys ? k_radix_onesweep<true> : k_radix_onesweep<false>;
+      hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kSortThreads), 0, c.stream, ks, vs, n, src,
+                         c.sort_gcount + 256 * k, c.sort_status + cur * words,
+                         c.sort_status + nxt * words, c.sort_ctr + cur, c.sort_ctr + nxt,
+                         nblocks, c.dc, kd, vd);
+      std::swap(ks, kd);
+      std::swap(vs, vd);
+    }
+    if (np & 1)   // the last pass used buffer 0: clear it for the next sort
+      hipLaunchKernelGGL(k_sort_clear, dim3(grid_for(nblocks, 1, 1024)), dim3(256), 0, c.stream,
+                         c.sort_status, 256 * nblocks, c.sort_ctr);
+  } else {
   for (const Pass &p : passes) {
     DigitSrc src{p.shift, c.nsys, c.d_off};
     auto hist = p.by_sys ? k_radix_hist<true> : k_radix_hist<false>;
@@ -379,6 +667,7 @@ void radix_sort_pairs(Ctx &c, uint64_t *keys, int *vals, uint64_t *keys_tmp, int
                        nblocks, c.sort_hist, kd, vd);
     std::swap(ks, kd);
     std::swap(vs, vd);
+  }
   }
   if (ks != keys) {
     (void)hipMemcpyAsync(keys, ks, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, c.stream);

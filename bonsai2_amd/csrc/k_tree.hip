@@ -52,6 +52,13 @@ template <typename R>
 __device__ __forceinline__ Ext<R> ext_empty() {
   return Ext<R>{R(1e10), R(1e10), R(1e10), R(-1e10), R(-1e10), R(-1e10)};
 }
+// Union of two extents, either of which may be empty (min > max): minima with
+// minima, maxima with maxima.
+template <typename R>
+__device__ __forceinline__ void ext_merge(Ext<R> &e, R nx, R ny, R nz, R xx, R xy, R xz) {
+  e.nx = rmin(e.nx, nx); e.ny = rmin(e.ny, ny); e.nz = rmin(e.nz, nz);
+  e.xx = rmax(e.xx, xx); e.xy = rmax(e.xy, xy); e.xz = rmax(e.xz, xz);
+}
 
 // block-wide min/max; the block's result in thread 0
 template <typename R>
@@ -62,10 +69,8 @@ __device__ __forceinline__ void block_minmax(Ext<R> &e, Ext<R> *sh) {
   if (lane_id() == 0) sh[w] = e;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
-      ext_add(e, sh[k].nx, sh[k].ny, sh[k].nz);
-      ext_add(e, sh[k].xx, sh[k].xy, sh[k].xz);
-    }
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++)
+      ext_merge(e, sh[k].nx, sh[k].ny, sh[k].nz, sh[k].xx, sh[k].xy, sh[k].xz);
   }
 }
 
@@ -115,8 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(const V4<R> *__restrict__
   Ext<R> x = ext_empty<R>();
   for (int k = threadIdx.x; k < nb; k += kBlock) {
     const V4<R> a = part[2 * (r * nb + k)], b = part[2 * (r * nb + k) + 1];
-    ext_add(x, a.x, a.y, a.z);
-    ext_add(x, b.x, b.y, b.z);
+    ext_merge(x, a.x, a.y, a.z, b.x, b.y, b.z);   // a block may have had no particles
   }
   block_minmax(x, sh);
   if (threadIdx.x == 0) box[r] = box_from_extent(x);
@@ -497,8 +501,7 @@ __device__ void node_moments(const MpArgs<T> &a, int c, const uint4 cd) {
     m.xx += Mc * (R(qa.x) + ox * ox); m.yy += Mc * (R(qa.y) + oy * oy);
     m.zz += Mc * (R(qa.z) + oz * oz); m.xy += Mc * (R(qa.w) + ox * oy);
     m.xz += Mc * (R(qb.x) + ox * oz); m.yz += Mc * (R(qb.y) + oy * oz);
-    ext_add(e, bmn.x, bmn.y, bmn.z);
-    ext_add(e, bmx.x, bmx.y, bmx.z);
+    ext_merge(e, bmn.x, bmn.y, bmn.z, bmx.x, bmx.y, bmx.z);
   }
   finish_moments<T, R>(a, c, r0.x, r0.y, r0.z, m, e);
 }

@@ -402,7 +402,7 @@ void k_walk(WalkArgs<R> a) {
   if (a.run_if && *a.run_if == 0) return;   // a fallback walk that is not needed
   const int ngroups = a.dc->ngroups;
   // a failed build (cell or group capacity) leaves placeholder links: do not walk it
-  if (a.dc->err & 3) return;
+  if (a.dc->err & (3 | kSortErr)) return;
 
   // The persistent loop has no lane-dependent branches at its ends: two
   // "if (lane == 0)" blocks around the back edge (queue fetch, stats store) are
