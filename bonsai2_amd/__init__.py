@@ -58,6 +58,7 @@ _SIGS = {
     "bt_get_ids": [P, P, I],
     "bt_get_boxes": [P, P],
     "bt_direct_fp64": [P, I, P, P],
+    "bt_direct_fp64_noalloc": [P, I, P, P],
     "bt_get_timings": [P, P],
     "bt_set_walk_options": [P, I, I, I],
     "bt_set_walk_variant": [P, I],
@@ -494,6 +495,12 @@ class Treecode:
         out = np.empty((len(idx), 4), np.float64)
         self._ck(self._L.bt_direct_fp64(self._h, len(idx), _p(idx), _p(out)), "direct_fp64")
         return out
+
+    def direct_fp64_noalloc(self, ns, d_ph_idx, d_out4):
+        """bt_direct_fp64_noalloc: device pointers in and out (ints) on the
+        context stream, no allocation and no synchronisation; returns the
+        status code (0 = ok) without raising."""
+        return int(self._L.bt_direct_fp64_noalloc(self._h, int(ns), P(d_ph_idx), P(d_out4)))
 
     def timings(self):
         t = np.zeros(10, np.float32)

@@ -265,6 +265,8 @@ void launch_clear_sticky(Ctx &c);
 void launch_energies(Ctx &c);
 int direct_slices(int ns);   // d_out of launch_direct: ns * (slices + 1) double4
 void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out, bool check = false);
+void launch_direct_parts(Ctx &c, int ns, const int *d_idx, double *d_out, double *d_part,
+                         int slices, bool check = false);
 
 // host-side helpers
 int grid_for(int n, int block, int cap);

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "bt_internal.h"
@@ -331,6 +332,34 @@ extern "C" {
 
 namespace {
 
+// ---- the device scratch of bt_direct_fp64_noalloc ------------------------
+// The partial sums of that entry point live in this fixed array, one copy per
+// GPU, held until the process ends.  Host code never passes the array's name
+// to a kernel (on the host it is only a stand-in for the symbol): its device
+// address is looked up once per device with hipGetSymbolAddress, at the first
+// context created on that device, and kept in scratch_dev[].
+constexpr size_t kScratchBytes = size_t(16) << 20;
+constexpr size_t kScratchD4 = kScratchBytes / sizeof(double4);
+constexpr int kNoAllocMaxTargets = 4096;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double4 scratch[kScratchD4];
+std::mutex scratch_mu;
+double4 *scratch_dev[kMaxDevices];
+
+// With device `dev` current: its copy of the array, looked up on first use.
+int scratch_lookup(int dev, double4 **out) {
+  *out = nullptr;
+  if (dev < 0 || dev >= kMaxDevices) return BT_ERR_ARG;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return BT_ERR_HIP;
+    scratch_dev[dev] = static_cast<double4 *>(p);
+  }
+  *out = scratch_dev[dev];
+  return 0;
+}
+
 int create_ctx(bt_ctx **out, double eps, double theta, int ncrit, bool fp64) {
   if (!out) return BT_ERR_ARG;
   *out = nullptr;
@@ -363,6 +392,11 @@ int create_ctx(bt_ctx **out, double eps, double theta, int ncrit, bool fp64) {
       dev_alloc(c, &c.mlp_m, kMlpParams) || dev_alloc(c, &c.mlp_v, kMlpParams) ||
       dev_alloc(c, &c.mlp_dev, 1)) {
     delete h;
+    return BT_ERR_HIP;
+  }
+  double4 *scr = nullptr;
+  if (scratch_lookup(c.dev, &scr)) {
+    bt_destroy(h);
     return BT_ERR_HIP;
   }
   std::memset(&c.hc, 0, sizeof(c.hc));
@@ -1016,6 +1050,22 @@ int bt_direct_fp64(bt_ctx *h, int ns, const int *ph_idx, double *out4) {
   BT_CK(hipMemcpyAsync(out4, c.d_direct, sizeof(double) * 4 * ns, hipMemcpyDeviceToHost,
                        c.stream));
   BT_CK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int bt_direct_fp64_noalloc(bt_ctx *h, int ns, const int *d_ph_idx, double *d_out4) {
+  if (!h) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  if (!d_ph_idx || !d_out4) return fail(c, BT_ERR_ARG, "null device pointer");
+  if (ns <= 0 || ns > kNoAllocMaxTargets)
+    return fail(c, BT_ERR_ARG, "bt_direct_fp64_noalloc: ns must be in [1, 4096]");
+  if (!c.have_groups) return fail(c, BT_ERR_STATE, "makeGroups first (targets are PH indices)");
+  double4 *scr = nullptr;
+  if (scratch_lookup(c.dev, &scr)) return fail(c, BT_ERR_HIP, "scratch symbol lookup failed");
+  // as many source slices as bt_direct_fp64 uses, up to what the scratch holds
+  const int slices = std::min(direct_slices(ns), (int)(kScratchD4 / (size_t)ns));
+  launch_direct_parts(c, ns, d_ph_idx, d_out4, reinterpret_cast<double *>(scr), slices);
+  BT_LAUNCH_CK();
   return 0;
 }
 

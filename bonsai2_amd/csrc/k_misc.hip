@@ -147,7 +147,10 @@ __global__ __launch_bounds__(kBlock) void k_direct(int ns, const int *__restrict
   __shared__ double4 tile[kBlock];
   const int t = blockIdx.x * kBlock + threadIdx.x;
   const bool active = t < ns;
-  const int k = active ? idx[t] : 0;
+  // a target index outside [0, n) reads particle 0 and yields a NaN row
+  const int k_in = active ? idx[t] : 0;
+  const bool ok = k_in >= 0 && k_in < n;
+  const int k = ok ? k_in : 0;
   const int r = FOREST ? system_of(k, nsys, off) : 0;
   const int lo = FOREST ? off[r] : 0, hi = FOREST ? off[r + 1] : n;
   const V4<R> pt = pos[k];
@@ -175,7 +178,11 @@ __global__ __launch_bounds__(kBlock) void k_direct(int ns, const int *__restrict
     }
     __syncthreads();
   }
-  if (active) part[(size_t)blockIdx.y * ns + t] = make_double4(ax, ay, az, ph);
+  if (active) {
+    const double nan = __builtin_nan("");
+    part[(size_t)blockIdx.y * ns + t] =
+        ok ? make_double4(ax, ay, az, ph) : make_double4(nan, nan, nan, nan);
+  }
 }
 
 // One wave per target: lane l sums slices l, l + 64, ... in order, then the
@@ -246,11 +253,18 @@ int direct_slices(int ns) {
 
 // d_out must hold ns * (direct_slices(ns) + 1) double4: the result, then the partials.
 void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out, bool check) {
-  double4 *part = reinterpret_cast<double4 *>(d_out) + ns;
+  launch_direct_parts(c, ns, d_idx, d_out, d_out + (size_t)4 * ns, direct_slices(ns), check);
+}
+
+// d_out: ns double4; d_part: ns * slices double4, every entry written before
+// it is read (nothing needs clearing).  The sums of slices are added in slice
+// order, so equal slices give equal results.
+void launch_direct_parts(Ctx &c, int ns, const int *d_idx, double *d_out, double *d_part,
+                         int slices, bool check) {
+  double4 *part = reinterpret_cast<double4 *>(d_part);
   // the softening the step uses: float(eps) squared in FP64 (Treecode<float>),
   // the FP64 eps squared (Treecode<double>)
   const double eps2 = c.eps * c.eps;
-  const int slices = direct_slices(ns);
   // the per-step check of a learned walk needs relative errors of ~1e-6:
   // one Newton step (~3e-14) is ample; the measurement path keeps two
   const dim3 grid((ns + kDirectTargets - 1) / kDirectTargets, slices);

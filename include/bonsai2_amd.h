@@ -156,6 +156,16 @@ int bt_get_boxes(bt_ctx *ctx, float *boxes4);
  * indices; sources = all particles of the target's system.  out = ns x
  * (ax, ay, az, phi) doubles, self term included in phi. */
 int bt_direct_fp64(bt_ctx *ctx, int ns, const int *ph_idx, double *out4);
+/* bt_direct_fp64 for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Targets d_ph_idx (ns ints) and results
+ * d_out4 (ns x 4 doubles) are device memory the caller owns; the call is
+ * enqueued on the context stream and does not synchronise.  The partial sums
+ * live in a fixed 16 MiB device array per GPU (held until the process ends),
+ * so ns <= 4096 (else BT_ERR_ARG) and calls on one GPU that overlap in time,
+ * from any context or stream, race on that array.  An index outside [0, n)
+ * gives a NaN row.  No hipMalloc, hipFree or other allocating call. */
+int bt_direct_fp64_noalloc(bt_ctx *ctx, int ns, const int *d_ph_idx, double *d_out4);
 /* Milliseconds of the phases of the last step, measured with HIP events on
  * the context stream: {bbox+keys, sort, tree levels, multipoles, ph keys,
  * ph sort, groups, walk, integrate, energies}. */
