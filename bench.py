@@ -367,8 +367,8 @@ def main():
     ap.add_argument("--group-mode", type=int, default=0, choices=(0, 1, 2),
                     help="0 PH-key groups (makeGroups), 1 leaves, 2 tree cells <= ncrit")
     ap.add_argument("--walk-variant", type=int, default=0, choices=(0, 1, 2),
-                    help="0: two i-particles per lane, LDS j-data; 1: scalar-load j-data; "
-                         "2: one i-particle per lane, LDS j-data")
+                    help="0: one i-particle per lane, LDS j-data; 1: scalar-load j-data; "
+                         "2: two i-particles per lane, half a batch per half-wave")
     ap.add_argument("--opening", default="learned", choices=("learned", "fixed"))
     ap.add_argument("--mlp", default=None, help="trained network (default bonsai2_amd/theta_mlp.json)")
     ap.add_argument("--check-samples", type=int, default=256)

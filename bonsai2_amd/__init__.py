@@ -209,7 +209,8 @@ class Treecode:
                                              int(blocks_per_cu)), "set_walk_options")
 
     def set_walk_variant(self, variant):
-        """0 = LDS-staged j-data (default), 1 = scalar-load j-data."""
+        """0 = LDS-staged j-data, one i-particle per lane (default); 1 = scalar-load
+        j-data; 2 = two i-particles per lane, each half-wave sweeping half a batch."""
         self._ck(self._L.bt_set_walk_variant(self._h, int(variant)), "set_walk_variant")
 
     def set_walk_lds_stack(self, entries):

@@ -12,10 +12,8 @@ namespace bt {
 
 constexpr int kSortErr = 8;   // DevCounters::err bit: a sort pass's look-back wait timed out
 constexpr int kLevels = 32;   // level table slots (reference int2[32], code/Treecode.h:171)
-#ifndef BT_LEVEL_BLOCKS
-#define BT_LEVEL_BLOCKS 2048
-#endif
-constexpr int kLevelBlocks = BT_LEVEL_BLOCKS;  // fixed grid of the per-level tree kernels
+constexpr int kLevelBuildBlocks = 1024;  // persistent grid of the per-level tree kernel
+constexpr int kLevelTile = 128;          // parents per tile of the per-level tree kernel
 constexpr int kFp64Levels = 4;     // cells at levels <= this combine moments in FP64
 
 struct MlpDev;   // matrix-core operands of the opening-angle network (bt_learn.h)
@@ -68,6 +66,10 @@ struct DevCounters {
   int err_sticky;
   int stack_hw;
   int steps_done;
+  // level build: per-level tile counters and the build count that tags the
+  // tiles' status words (so that words of earlier builds never need clearing)
+  int lvl_ctr[kLevels + 2];
+  unsigned int build_epoch;
 };
 
 // The values captured graphs bake into their kernel arguments: a graph is
@@ -134,8 +136,8 @@ struct Ctx {
   float2 *cq1 = nullptr, *cls = nullptr;
   float4 *cscr = nullptr;      // 6 float4 per cell (bottom-up moments)
   float4 *cwalk = nullptr;     // 3 float4 per cell: walk record (com,M) (3Q) (3Qxz,3Qyz,trQ/2,1.5trQ)
-  int *child_bnd = nullptr;    // 8 ints per parent of the current level
-  int *lvl_blk = nullptr;      // block sums / offsets of the level scan
+  unsigned long long *lvl_status = nullptr;  // per-tile status words of the level build
+  int lvl_status_cap = 0;
   int2 *level_beg = nullptr;   // int2[32] (reference d_level_begIdx)
   int2 *root_cells = nullptr;  // per system (first level-1 cell, count)
 
@@ -163,7 +165,7 @@ struct Ctx {
   int walk_blocks = 0;         // persistent grid (blocks of 256)
   int blocks_per_cu = 0;
   int kahan = 1;
-  int walk_variant = 0;        // 0: j-data staged in LDS, 1: j-data through scalar loads
+  int walk_variant = 0;        // 0: LDS j-data, 1: scalar loads, 2: two i-particles per lane
   int quadrupole = 1;          // 0: monopole-only P2C (the reference without QUADRUPOLE)
   int num_cus = 256;
   unsigned long long *d_wstats = nullptr;  // direct_sum, direct_max, approx_sum, approx_max

@@ -82,7 +82,7 @@ void free_particle_arrays(Ctx &c) {
 
 void free_cell_arrays(Ctx &c) {
   dev_free(&c.cells); dev_free(&c.csize); dev_free(&c.cmono); dev_free(&c.cq0);
-  dev_free(&c.cq1); dev_free(&c.cls); dev_free(&c.cscr); dev_free(&c.child_bnd);
+  dev_free(&c.cq1); dev_free(&c.cls); dev_free(&c.cscr); dev_free(&c.lvl_status);
   dev_free(&c.cwalk);
 }
 
@@ -102,8 +102,13 @@ int alloc_cells(Ctx &c, int cap) {
   rc |= dev_alloc(c, &c.cls, n * w);
   rc |= dev_alloc(c, &c.cscr, n * 6 * w);
   rc |= dev_alloc(c, &c.cwalk, n * 3 * w);
-  rc |= dev_alloc(c, &c.child_bnd, (size_t)cap * 8 + 8 * (size_t)c.nsys);
-  return rc ? BT_ERR_HIP : 0;
+  // a level's parents are at most all cells (or the nsys roots)
+  c.lvl_status_cap = (cap + c.nsys + kLevelTile - 1) / kLevelTile + 1;
+  rc |= dev_alloc(c, &c.lvl_status, (size_t)c.lvl_status_cap);
+  if (rc) return BT_ERR_HIP;
+  // status words start clear (flag 0 = not published)
+  fill_int(c, reinterpret_cast<int *>(c.lvl_status), 0, 2 * c.lvl_status_cap);
+  return 0;
 }
 
 void release_graph(Ctx &c) {
@@ -388,7 +393,7 @@ int create_ctx(bt_ctx **out, double eps, double theta, int ncrit, bool fp64) {
   for (int k = 0; k < 12; k++) (void)hipEventCreate(&c.ev[k]);
   if (dev_alloc(c, &c.dc, 1) || dev_alloc(c, &c.level_beg, kLevels) ||
       dev_alloc(c, &c.d_wstats, 4 + 4 * (size_t)kWalkStatBlocks) || dev_alloc(c, &c.d_wpot, 1) ||
-      dev_alloc(c, &c.lvl_blk, kLevelBlocks) || dev_alloc(c, &c.mlp_params, kMlpParams) ||
+      dev_alloc(c, &c.mlp_params, kMlpParams) ||
       dev_alloc(c, &c.mlp_m, kMlpParams) || dev_alloc(c, &c.mlp_v, kMlpParams) ||
       dev_alloc(c, &c.mlp_dev, 1)) {
     delete h;
@@ -439,7 +444,6 @@ int bt_destroy(bt_ctx *h) {
   dev_free(&c.level_beg);
   dev_free(&c.d_wstats);
   dev_free(&c.d_wpot);
-  dev_free(&c.lvl_blk);
   dev_free(&c.sort_hist);
   dev_free(&c.sort_hpart); dev_free(&c.sort_gcount); dev_free(&c.sort_status);
   dev_free(&c.sort_ctr);

@@ -218,6 +218,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
 // tile and resets the other counter for the next pass.
 
 constexpr int kMaxSortPasses = 11;
+// One-sweep tiles: 256 threads x 8 rounds = 2048 pairs (30 KiB of LDS, so
+// five blocks share a CU while others wait in their look-back).
+constexpr int kOsRounds = 8;
+constexpr int kOsTile = kSortThreads * kOsRounds;
 constexpr int kSortHistBlocks = 1024;
 constexpr unsigned kStAgg = 1u << 30, kStIncl = 2u << 30, kStCount = (1u << 30) - 1u;
 constexpr int kLookbackSpins = 1 << 22;
@@ -230,18 +234,20 @@ struct PassSet {
   int bysys[kMaxSortPasses];
 };
 
-// Adds 1 for every valid lane at hist[d]: lanes with equal digits find each
-// other with 8 ballots and the lowest of them adds the group's size (no LDS
-// atomics on one address from many lanes of a wave).
+// Adds 1 for every valid lane at hist[d].  A wave whose valid lanes all hold
+// one digit (the high digits of nearly sorted keys) adds its count with one
+// atomic; otherwise each lane adds its own.
 __device__ __forceinline__ void wave_count_digit(int *hist, int d, bool valid) {
-  uint64_t peers = ballot64(valid);
-#pragma unroll
-  for (int b = 0; b < 8; b++) {
-    const bool bit = (d >> b) & 1;
-    const uint64_t m = ballot64(bit);
-    peers &= bit ? m : ~m;
+  const uint64_t vm = ballot64(valid);
+  if (vm == 0) return;   // wave-uniform
+  const int first = __ffsll((unsigned long long)vm) - 1;
+  const int d0 = __shfl(d, first, kWave);
+  const uint64_t same = ballot64(valid && d == d0);
+  if (same == vm) {
+    if (lane_id() == first) atomicAdd(&hist[d0], __popcll(vm));
+  } else if (valid) {
+    atomicAdd(&hist[d], 1);
   }
-  if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&hist[d], __popcll(peers));
 }
 
 // grid kSortHistBlocks: block b counts the digits of every pass over a
@@ -302,8 +308,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_onesweep(
     const int *__restrict__ gcount, unsigned int *status, unsigned int *status_next, int *ctr,
     int *ctr_next, int ntiles, DevCounters *dc, uint64_t *__restrict__ keys_out,
     int *__restrict__ vals_out) {
-  __shared__ uint64_t skeys[kSortTile];   // the tile in sorted order
-  __shared__ int svals[kSortTile];
+  __shared__ uint64_t skeys[kOsTile];   // the tile in sorted order
+  __shared__ int svals[kOsTile];
   __shared__ int tstart[256];     // first tile-local position of each digit
   __shared__ int gbase[256];      // global position of this tile's first pair of each digit
   __shared__ int wcnt[4][256];    // per-wave digit counts -> per-wave digit bases
@@ -328,17 +334,17 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_onesweep(
   status_next[(size_t)tile * 256 + tid] = 0u;
   // Each wave ranks its own contiguous quarter of the tile (16 rounds of 64
   // pairs) against its private digit counters; pairs stay in registers.
-  const int wbase = tile * kSortTile + w * (kSortRounds * kWave);
-  uint64_t kk[kSortRounds];
-  int vv[kSortRounds], dd[kSortRounds], rr[kSortRounds];
+  const int wbase = tile * kOsTile + w * (kOsRounds * kWave);
+  uint64_t kk[kOsRounds];
+  int vv[kOsRounds], dd[kOsRounds], rr[kOsRounds];
 #pragma unroll
-  for (int r = 0; r < kSortRounds; r++) {
+  for (int r = 0; r < kOsRounds; r++) {
     const int i = wbase + r * kWave + lane;
     kk[r] = i < n ? keys[i] : 0ull;
     vv[r] = i < n ? vals[i] : 0;
   }
 #pragma unroll
-  for (int r = 0; r < kSortRounds; r++) {
+  for (int r = 0; r < kOsRounds; r++) {
     const int i = wbase + r * kWave + lane;
     const bool valid = i < n;
     const int d = valid ? digit_of<BYSYS>(kk[r], vv[r], src) : 0;
@@ -406,7 +412,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_onesweep(
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kSortRounds; r++) {
+  for (int r = 0; r < kOsRounds; r++) {
     if (dd[r] >= 0) {
       const int pos = wcnt[w][dd[r]] + rr[r];
       skeys[pos] = kk[r];
@@ -414,7 +420,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_onesweep(
     }
   }
   __syncthreads();
-  const int tile_n = min(kSortTile, n - tile * kSortTile);
+  const int tile_n = min(kOsTile, n - tile * kOsTile);
   for (int j = tid; j < tile_n; j += kSortThreads) {
     const uint64_t key = skeys[j];
     const int val = svals[j];
@@ -608,7 +614,8 @@ void radix_sort_pairs(Ctx &c, uint64_t *keys, int *vals, uint64_t *keys_tmp, int
   int *vs = vals, *vd = vals_tmp;
   if (c.sort_onesweep && (int)passes.size() <= kMaxSortPasses) {
     const int np = (int)passes.size();
-    if (nblocks > c.sort_tiles_cap || !c.sort_hpart) {
+    const int ntiles = (n + kOsTile - 1) / kOsTile;
+    if (ntiles > c.sort_tiles_cap || !c.sort_hpart) {
       dev_free_raw(c.sort_hpart);
       dev_free_raw(c.sort_gcount);
       dev_free_raw(c.sort_status);
@@ -616,14 +623,14 @@ void radix_sort_pairs(Ctx &c, uint64_t *keys, int *vals, uint64_t *keys_tmp, int
       c.sort_tiles_cap = 0;
       if (hipMalloc(&c.sort_hpart, sizeof(int) * kSortHistBlocks * kMaxSortPasses * 256) ||
           hipMalloc(&c.sort_gcount, sizeof(int) * kMaxSortPasses * 256) ||
-          hipMalloc(&c.sort_status, sizeof(unsigned int) * 2 * 256 * (size_t)nblocks) ||
+          hipMalloc(&c.sort_status, sizeof(unsigned int) * 2 * 256 * (size_t)ntiles) ||
           hipMalloc(&c.sort_ctr, sizeof(int) * 2)) {
         c.err = "sort scratch allocation failed";
         return;
       }
-      c.sort_tiles_cap = nblocks;
+      c.sort_tiles_cap = ntiles;
       hipLaunchKernelGGL(k_sort_clear, dim3(1024), dim3(256), 0, c.stream, c.sort_status,
-                         2 * 256 * nblocks, c.sort_ctr);
+                         2 * 256 * ntiles, c.sort_ctr);
       hipLaunchKernelGGL(k_sort_clear, dim3(1), dim3(64), 0, c.stream, c.sort_status, 0,
                          c.sort_ctr + 1);
     }
@@ -645,16 +652,16 @@ void radix_sort_pairs(Ctx &c, uint64_t *keys, int *vals, uint64_t *keys_tmp, int
       const int cur = k & 1, nxt = cur ^ 1;
       DigitSrc src{passes[k].shift, c.nsys, c.d_off};
       auto kern = passes[k].by_sys ? k_radix_onesweep<true> : k_radix_onesweep<false>;
-      hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kSortThreads), 0, c.stream, ks, vs, n, src,
+      hipLaunchKernelGGL(kern, dim3(ntiles), dim3(kSortThreads), 0, c.stream, ks, vs, n, src,
                          c.sort_gcount + 256 * k, c.sort_status + cur * words,
                          c.sort_status + nxt * words, c.sort_ctr + cur, c.sort_ctr + nxt,
-                         nblocks, c.dc, kd, vd);
+                         ntiles, c.dc, kd, vd);
       std::swap(ks, kd);
       std::swap(vs, vd);
     }
     if (np & 1)   // the last pass used buffer 0: clear it for the next sort
-      hipLaunchKernelGGL(k_sort_clear, dim3(grid_for(nblocks, 1, 1024)), dim3(256), 0, c.stream,
-                         c.sort_status, 256 * nblocks, c.sort_ctr);
+      hipLaunchKernelGGL(k_sort_clear, dim3(grid_for(ntiles, 1, 1024)), dim3(256), 0, c.stream,
+                         c.sort_status, 256 * ntiles, c.sort_ctr);
   } else {
   for (const Pass &p : passes) {
     DigitSrc src{p.shift, c.nsys, c.d_off};

@@ -208,33 +208,71 @@ __device__ __forceinline__ void sweep(const WaveShared<R> &S, int cnt, R eps2,
   Acc4T<R> b[IPL];
 #pragma unroll
   for (int k = 0; k < IPL; k++) b[k] = Acc4T<R>{R(0), R(0), R(0), R(0)};
-  auto one = [&](int slot) {
-    const V4<R> A = S.jA[slot];
+  // One entry's j-data, and its interactions with the lane's i-particles.
+  struct J {
+    V4<R> A, B, C;
+  };
+  auto load = [&](int slot) {
+    J j;
+    j.A = S.jA[slot];
     if (QUAD) {
-      const V4<R> B = S.jB[slot], C = S.jC[slot];
+      j.B = S.jB[slot];
+      j.C = S.jC[slot];
+    }
+    return j;
+  };
+  auto eval = [&](const J &j) {
+    if (QUAD) {
 #pragma unroll
-      for (int k = 0; k < IPL; k++) p2c_rec(b[k], ip.x[k], ip.y[k], ip.z[k], A, B, C, eps2);
+      for (int k = 0; k < IPL; k++) p2c_rec(b[k], ip.x[k], ip.y[k], ip.z[k], j.A, j.B, j.C, eps2);
     } else {
 #pragma unroll
-      for (int k = 0; k < IPL; k++) p2p(b[k], ip.x[k], ip.y[k], ip.z[k], A, eps2);
+      for (int k = 0; k < IPL; k++) p2p(b[k], ip.x[k], ip.y[k], ip.z[k], j.A, eps2);
     }
   };
+  // Entry j + 1 is read from LDS while entry j is evaluated, so the reads'
+  // latency is hidden behind one interaction's arithmetic.  The read past the
+  // last entry stays inside the kJSlots slots and its value is not used.
   if (IPL == 2) {
     const int hc = half_count<2>(cnt);
     const int base = lane >= 32 ? hc + 1 : 0;
+    J cur = load(base);
     if (cnt == kWave) {
 #pragma unroll BT_UNROLL_LDS2
-      for (int j = 0; j < kWave / 2; j++) one(base + j);
+      for (int j = 0; j < kWave / 2; j++) {
+        const J nxt = load(base + j + 1);
+        __builtin_amdgcn_sched_barrier(0);   // keep the reads ahead of the arithmetic
+        eval(cur);
+        cur = nxt;
+      }
     } else {
 #pragma unroll 2
-      for (int j = 0; j < hc; j++) one(base + j);
+      for (int j = 0; j < hc; j++) {
+        const J nxt = load(base + j + 1);
+        __builtin_amdgcn_sched_barrier(0);   // keep the reads ahead of the arithmetic
+        eval(cur);
+        cur = nxt;
+      }
     }
-  } else if (cnt == kWave) {
-#pragma unroll BT_UNROLL_LDS
-    for (int j = 0; j < kWave; j++) one(j);
   } else {
+    J cur = load(0);
+    if (cnt == kWave) {
+#pragma unroll BT_UNROLL_LDS
+      for (int j = 0; j < kWave; j++) {
+        const J nxt = load(j + 1);
+        __builtin_amdgcn_sched_barrier(0);   // keep the reads ahead of the arithmetic
+        eval(cur);
+        cur = nxt;
+      }
+    } else {
 #pragma unroll 2
-    for (int j = 0; j < cnt; j++) one(j);
+      for (int j = 0; j < cnt; j++) {
+        const J nxt = load(j + 1);
+        __builtin_amdgcn_sched_barrier(0);   // keep the reads ahead of the arithmetic
+        eval(cur);
+        cur = nxt;
+      }
+    }
   }
 #pragma unroll
   for (int k = 0; k < IPL; k++) fold<KAHAN, R>(ip.s[k], ip.c[k], b[k]);
@@ -830,21 +868,21 @@ void launch_walk(Ctx &c, const int *run_if) {
                                                   : k_walk<double, false, false, false, true, 1>;
     hipLaunchKernelGGL(kern, dim3(c.walk_blocks), dim3(kBlock), 0, c.stream, a);
   } else {
-    // variant 0: two i-particles per lane, batches split between the
-    // half-waves (LDS j-data); 1: scalar-load j-data, one i-particle per lane;
-    // 2: LDS j-data, one i-particle per lane
+    // variant 0: LDS j-data, one i-particle per lane; 1: scalar-load j-data,
+    // one i-particle per lane; 2: two i-particles per lane, batches split
+    // between the half-waves (LDS j-data)
     const WalkArgs<float> a = walk_args<float>(c, run_if);
     void (*kern)(WalkArgs<float>);
     const bool learn = c.opening == 1 && c.have_mlp;
     const int v = c.walk_variant;
 #define BT_PICK(K, LEARN, MONO)                                                     \
   (v == 1 ? k_walk<float, K, true, LEARN, MONO, 1>                                  \
-          : v == 2 ? k_walk<float, K, false, LEARN, MONO, 1> : k_walk<float, K, false, LEARN, MONO, 2>)
+          : v == 2 ? k_walk<float, K, false, LEARN, MONO, 2> : k_walk<float, K, false, LEARN, MONO, 1>)
     if (!c.quadrupole)
-      kern = c.kahan ? (v == 2 ? k_walk<float, true, false, false, true, 1>
-                               : k_walk<float, true, false, false, true, 2>)
-                     : (v == 2 ? k_walk<float, false, false, false, true, 1>
-                               : k_walk<float, false, false, false, true, 2>);
+      kern = c.kahan ? (v == 2 ? k_walk<float, true, false, false, true, 2>
+                               : k_walk<float, true, false, false, true, 1>)
+                     : (v == 2 ? k_walk<float, false, false, false, true, 2>
+                               : k_walk<float, false, false, false, true, 1>);
     else if (learn)
       kern = c.kahan ? BT_PICK(true, true, false) : BT_PICK(false, true, false);
     else

@@ -175,14 +175,14 @@ int bt_get_timings(bt_ctx *ctx, float ms[10]);
  *   a walk that needs more returns BT_ERR_STACK);
  * blocks_per_cu = persistent grid size (default 0 = automatic). */
 int bt_set_walk_options(bt_ctx *ctx, int kahan, int stack_ints, int blocks_per_cu);
-/* How the walk feeds interaction data to the lanes: 0 (default) = each lane
- * carries two of the group's i-particles, the wave stages each batch of 64
- * cells / particles in LDS and each half-wave sweeps half of it with
- * broadcast reads (every LDS read serves two interactions); 1 = one
- * i-particle per lane, each batch entry read by scalar loads straight from
- * the immutable cell / particle arrays and used from SGPRs; 2 = one
- * i-particle per lane, the whole batch swept from LDS.  All evaluate the same
- * interaction lists; forces agree to FP32 rounding. */
+/* How the walk feeds interaction data to the lanes: 0 (default) = one
+ * i-particle per lane, the wave stages each batch of 64 cells / particles in
+ * LDS and sweeps it with broadcast reads; 1 = one i-particle per lane, each
+ * batch entry read by scalar loads straight from the immutable cell /
+ * particle arrays and used from SGPRs; 2 = each lane carries two of the
+ * group's i-particles and each half-wave sweeps half of the LDS batch (every
+ * LDS read serves two interactions, at the cost of registers and occupancy).
+ * All evaluate the same interaction lists; forces agree to FP32 rounding. */
 int bt_set_walk_variant(bt_ctx *ctx, int variant);
 /* Learned-angle walk only (no reference counterpart): entries of each wave's
  * traversal stack kept in LDS, 512..1024 (default 1024); deeper entries spill

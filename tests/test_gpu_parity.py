@@ -160,20 +160,25 @@ def test_interaction_counts_oracle_moments(run):
         assert abs(a - b) <= 1e-4 * max(1.0, abs(b)), (stats, exp)
 
 
-def test_scalar_load_walk_matches(run):
-    """The scalar-load j-data walk (variant 1) evaluates the same interaction
-    lists as the default LDS-staged walk: counts equal, forces within FP32
-    rounding of each other and within the oracle bar."""
+@pytest.mark.parametrize("variant", [1, 2], ids=["scalar_loads", "two_per_lane"])
+def test_walk_variants_match(run, variant):
+    """The scalar-load j-data walk (variant 1) and the two-i-particles-per-lane
+    walk (variant 2) evaluate the same interaction lists as the default
+    LDS-staged walk: per-group counts equal, forces within FP32 rounding of
+    each other and within the oracle bar."""
     t, ref = run["t"], run["ref"]
     a0 = t.acc(2)
-    t.set_walk_variant(1)
+    g0 = t.group_counts()
+    t.set_walk_variant(variant)
     try:
         st1 = t.computeForces()
         a1 = t.acc(2)
+        g1 = t.group_counts()
     finally:
         t.set_walk_variant(0)
         t.computeForces()
     assert tuple(st1) == tuple(run["stats"])
+    assert np.array_equal(g1, g0)
     assert rel_rms(a1[:, :3], ref["acc"][:, :3]) <= 1e-5
     assert rel_rms(a1[:, :3], a0[:, :3]) <= 1e-5
     assert np.array_equal(t.acc(2), a0)  # the default walk is deterministic
