@@ -12,8 +12,10 @@ namespace bt {
 
 constexpr int kSortErr = 8;   // DevCounters::err bit: a sort pass's look-back wait timed out
 constexpr int kLevels = 32;   // level table slots (reference int2[32], code/Treecode.h:171)
-constexpr int kLevelBuildBlocks = 1024;  // persistent grid of the per-level tree kernel
-constexpr int kLevelTile = 128;          // parents per tile of the per-level tree kernel
+#ifndef BT_LEVEL_BLOCKS
+#define BT_LEVEL_BLOCKS 2048
+#endif
+constexpr int kLevelBlocks = BT_LEVEL_BLOCKS;  // fixed grid of the per-level tree kernels
 constexpr int kFp64Levels = 4;     // cells at levels <= this combine moments in FP64
 
 struct MlpDev;   // matrix-core operands of the opening-angle network (bt_learn.h)
@@ -66,10 +68,6 @@ struct DevCounters {
   int err_sticky;
   int stack_hw;
   int steps_done;
-  // level build: per-level tile counters and the build count that tags the
-  // tiles' status words (so that words of earlier builds never need clearing)
-  int lvl_ctr[kLevels + 2];
-  unsigned int build_epoch;
 };
 
 // The values captured graphs bake into their kernel arguments: a graph is
@@ -117,7 +115,7 @@ struct Ctx {
   unsigned int *sort_status = nullptr;  // 2 x tiles x 256
   int *sort_ctr = nullptr;          // 2 tile counters
   int sort_tiles_cap = 0;
-  bool sort_onesweep = true;
+  bool sort_onesweep = false;
   uint64_t *geo_keys = nullptr;   // sorted geometric keys (tree order)
 
   // scans
@@ -136,8 +134,8 @@ struct Ctx {
   float2 *cq1 = nullptr, *cls = nullptr;
   float4 *cscr = nullptr;      // 6 float4 per cell (bottom-up moments)
   float4 *cwalk = nullptr;     // 3 float4 per cell: walk record (com,M) (3Q) (3Qxz,3Qyz,trQ/2,1.5trQ)
-  unsigned long long *lvl_status = nullptr;  // per-tile status words of the level build
-  int lvl_status_cap = 0;
+  int *child_bnd = nullptr;    // 8 ints per parent of the current level
+  int *lvl_blk = nullptr;      // block sums / offsets of the level scan
   int2 *level_beg = nullptr;   // int2[32] (reference d_level_begIdx)
   int2 *root_cells = nullptr;  // per system (first level-1 cell, count)
 

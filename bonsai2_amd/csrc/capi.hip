@@ -82,7 +82,7 @@ void free_particle_arrays(Ctx &c) {
 
 void free_cell_arrays(Ctx &c) {
   dev_free(&c.cells); dev_free(&c.csize); dev_free(&c.cmono); dev_free(&c.cq0);
-  dev_free(&c.cq1); dev_free(&c.cls); dev_free(&c.cscr); dev_free(&c.lvl_status);
+  dev_free(&c.cq1); dev_free(&c.cls); dev_free(&c.cscr); dev_free(&c.child_bnd);
   dev_free(&c.cwalk);
 }
 
@@ -102,13 +102,8 @@ int alloc_cells(Ctx &c, int cap) {
   rc |= dev_alloc(c, &c.cls, n * w);
   rc |= dev_alloc(c, &c.cscr, n * 6 * w);
   rc |= dev_alloc(c, &c.cwalk, n * 3 * w);
-  // a level's parents are at most all cells (or the nsys roots)
-  c.lvl_status_cap = (cap + c.nsys + kLevelTile - 1) / kLevelTile + 1;
-  rc |= dev_alloc(c, &c.lvl_status, (size_t)c.lvl_status_cap);
-  if (rc) return BT_ERR_HIP;
-  // status words start clear (flag 0 = not published)
-  fill_int(c, reinterpret_cast<int *>(c.lvl_status), 0, 2 * c.lvl_status_cap);
-  return 0;
+  rc |= dev_alloc(c, &c.child_bnd, (size_t)cap * 8 + 8 * (size_t)c.nsys);
+  return rc ? BT_ERR_HIP : 0;
 }
 
 void release_graph(Ctx &c) {
@@ -387,13 +382,14 @@ int create_ctx(bt_ctx **out, double eps, double theta, int ncrit, bool fp64) {
     return BT_ERR_HIP;
   }
   c.own_stream = true;
-  // BT_SORT_ONESWEEP=0 selects the three-launch radix passes (histogram, scan,
-  // scatter) instead of the one-sweep passes: same result, for comparison
+  // BT_SORT_ONESWEEP=1 selects the one-sweep radix passes (one launch per pass,
+  // decoupled look-back) instead of the three-launch passes (histogram, scan,
+  // scatter): same result, measured slower on MI355X, kept for comparison
   if (const char *e = std::getenv("BT_SORT_ONESWEEP")) c.sort_onesweep = std::atoi(e) != 0;
   for (int k = 0; k < 12; k++) (void)hipEventCreate(&c.ev[k]);
   if (dev_alloc(c, &c.dc, 1) || dev_alloc(c, &c.level_beg, kLevels) ||
       dev_alloc(c, &c.d_wstats, 4 + 4 * (size_t)kWalkStatBlocks) || dev_alloc(c, &c.d_wpot, 1) ||
-      dev_alloc(c, &c.mlp_params, kMlpParams) ||
+      dev_alloc(c, &c.lvl_blk, kLevelBlocks) || dev_alloc(c, &c.mlp_params, kMlpParams) ||
       dev_alloc(c, &c.mlp_m, kMlpParams) || dev_alloc(c, &c.mlp_v, kMlpParams) ||
       dev_alloc(c, &c.mlp_dev, 1)) {
     delete h;
@@ -448,6 +444,7 @@ int bt_destroy(bt_ctx *h) {
   dev_free(&c.sort_hpart); dev_free(&c.sort_gcount); dev_free(&c.sort_status);
   dev_free(&c.sort_ctr);
   dev_free(&c.scan_blk);
+  dev_free(&c.lvl_blk);
   dev_free(&c.d_direct);
   dev_free(&c.mlp_params); dev_free(&c.mlp_m); dev_free(&c.mlp_v); dev_free(&c.mlp_dev);
   dev_free(&c.d_check); dev_free(&c.d_check_idx);

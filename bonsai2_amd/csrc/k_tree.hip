@@ -181,30 +181,20 @@ __global__ __launch_bounds__(kBlock) void k_gather3(int n, const int *__restrict
 
 // ---- level-synchronous cell emission ------------------------------------
 //
-// Parents of level L are the cells of level L-1 (the roots for L = 1).  One
-// launch per level: a persistent grid takes tiles of kLevelTile consecutive
-// parents from a counter.  Eight lanes serve one parent; lane o finds the end
-// of child octant o (first index in the parent's range whose digit L exceeds
-// o) by binary search on the sorted keys.  A tile publishes its child count,
-// finds the count of all earlier tiles by looking back through their status
-// words (decoupled look-back), and emits its parents' children from that
-// offset, so cells come out in parent order with one launch per level.
-//
-// A status word is (tag << 32) | (flag << 30) | count, tag = build count * 32
-// + L: words of other levels or builds are stale and read as not published.
-// flag 1 = the tile's own count, 2 = the inclusive count of tiles 0..t.
-
-constexpr unsigned long long kLvAgg = 1ull << 30, kLvIncl = 2ull << 30;
-constexpr unsigned long long kLvCount = (1ull << 30) - 1ull;
-constexpr int kLvSpins = 1 << 22;
+// Parents of level L are the cells of level L-1 (the roots for L = 1).  Eight
+// lanes serve one parent; lane o finds the end of child octant o (first index
+// in the parent's range whose digit L exceeds o) by binary search on the
+// sorted keys.  Parents are split into kLevelBlocks contiguous chunks so the
+// block sums give each parent's first child slot in parent order.
 
 struct LevelArgs {
   int L, nsys, nleaf, cap_cells, n;
   const int *off;
   const uint64_t *keys;
   uint4 *cells;
+  int *bnd;          // 8 ints per parent: end of each child octant
+  int *blk;          // kLevelBlocks block sums -> offsets
   int2 *root_cells;
-  unsigned long long *status;
   DevCounters *dc;
 };
 
@@ -226,162 +216,176 @@ __device__ __forceinline__ void parent_range(const LevelArgs &a, int p, int &b, 
   }
 }
 
-// End of child octant o of parent p: the first index in the parent's range
-// whose digit L exceeds o (the parent's begin when it does not split).
-__device__ __forceinline__ int child_end(const LevelArgs &a, int b, int e, bool splits, int o) {
-  if (!splits) return b;
-  const int shift = 63 - 3 * a.L;
-  int lo = b, hi = e;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    const int d = (int)((a.keys[mid] >> shift) & 7ull);
-    if (d <= o) lo = mid + 1; else hi = mid;
+// Per-lane child boundaries of one parent; returns the child count of octant o.
+__device__ __forceinline__ int child_count(const LevelArgs &a, int p, bool active, int o,
+                                           int *end_out) {
+  int b = 0, e = 0;
+  bool splits = false;
+  if (active) parent_range(a, p, b, e, splits);
+  int end = b;
+  if (active && splits) {
+    const int shift = 63 - 3 * a.L;
+    // first index in [b, e) with digit > o
+    int lo = b, hi = e;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      const int d = (int)((a.keys[mid] >> shift) & 7ull);
+      if (d <= o) lo = mid + 1; else hi = mid;
+    }
+    end = lo;
   }
-  return lo;
+  const int beg_lane = __shfl_up(end, 1, 8);
+  const int beg = (o == 0) ? b : beg_lane;
+  *end_out = end;
+  return (active && splits) ? end - beg : 0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_level_build(LevelArgs a) {
-  __shared__ int s_end[kLevelTile * 8];   // child octant ends of the tile's parents
-  __shared__ int s_pre[kLevelTile];       // tile-local first child of each parent
-  __shared__ int shw[kBlock / kWave];
-  __shared__ int s_tile, s_excl;
-  DevCounters *dc = a.dc;
-  const int L = a.L;
+__global__ __launch_bounds__(kBlock) void k_level_count(LevelArgs a) {
+  __shared__ int shw[4];
   const int np = level_parent_count(a);
-  const int ntiles = (np + kLevelTile - 1) / kLevelTile;
-  const int start = L == 1 ? 0 : dc->lvl_start[L - 1] + dc->lvl_count[L - 1];
-  const int pstart = L == 1 ? 0 : dc->lvl_start[L - 1];
-  const unsigned long long tag =
-      (unsigned long long)(unsigned)(dc->build_epoch * 32u + (unsigned)L) << 32;
+  const int chunk = (np + kLevelBlocks - 1) / kLevelBlocks;
+  const int p0 = blockIdx.x * chunk, p1 = min(np, p0 + chunk);
+  const int o = threadIdx.x & 7;
+  int sum = 0;
+  for (int pb = p0; pb < p1; pb += kBlock / 8) {
+    const int p = pb + (threadIdx.x >> 3);
+    const bool active = p < p1;
+    int end;
+    const int cnt = child_count(a, p, active, o, &end);
+    if (active) a.bnd[8 * p + o] = end;
+    sum += cnt > 0 ? 1 : 0;
+  }
+  sum = wave_sum(sum);
+  if (lane_id() == 0) shw[threadIdx.x >> 6] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) a.blk[blockIdx.x] = shw[0] + shw[1] + shw[2] + shw[3];
+}
+
+__global__ __launch_bounds__(kBlock) void k_level_scan(LevelArgs a) {
+  constexpr int kItems = (kLevelBlocks + kBlock - 1) / kBlock;
+  __shared__ int shw[kBlock / kWave];
+  __shared__ int s_run;
+  // thread t scans block sums [t * kItems, (t + 1) * kItems)
+  int v[kItems];
+  int own = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    const int i = threadIdx.x * kItems + k;
+    v[k] = i < kLevelBlocks ? a.blk[i] : 0;
+    own += v[k];
+  }
+  int incl = wave_incl_scan(own);
+  if (lane_id() == kWave - 1) shw[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  int pre = 0;
+  for (int w = 0; w < (int)(threadIdx.x >> 6); w++) pre += shw[w];
+  if (threadIdx.x == kBlock - 1) s_run = pre + incl;
+  int off = pre + incl - own;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int run = s_run;
+    DevCounters *dc = a.dc;
+    const int start = a.L == 1 ? 0 : dc->lvl_start[a.L - 1] + dc->lvl_count[a.L - 1];
+    int total = run;
+    if (start + total > a.cap_cells) {
+      dc->err |= 1;
+      dc->err_sticky |= 1;
+      total = 0;
+    }
+    dc->lvl_start[a.L] = start;
+    dc->lvl_count[a.L] = total;
+    if (total > 0) dc->nlevels = a.L;
+  }
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    const int i = threadIdx.x * kItems + k;
+    if (i < kLevelBlocks) a.blk[i] = off;
+    off += v[k];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_level_emit(LevelArgs a) {
+  __shared__ int shw[4];
+  __shared__ int sh_nodes, sh_leaves;
+  DevCounters *dc = a.dc;
+  const int total = dc->lvl_count[a.L];
+  if (total == 0) return;  // nothing at this level (or capacity exceeded)
+  if (threadIdx.x == 0) { sh_nodes = 0; sh_leaves = 0; }
+  const int np = level_parent_count(a);
+  const int chunk = (np + kLevelBlocks - 1) / kLevelBlocks;
+  const int p0 = blockIdx.x * chunk, p1 = min(np, p0 + chunk);
   const int o = threadIdx.x & 7;
   const int gsub = (threadIdx.x & 63) >> 3;  // 8-lane group within the wave
-  int nodes = 0, leaves = 0, capped = 0, over = 0;
-  for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(&dc->lvl_ctr[L], 1);
-    __syncthreads();
-    const int tile = s_tile;
-    if (tile >= ntiles) break;   // block-uniform
-    // child counts of the tile's parents
-    for (int q = 0; q < kLevelTile / 32; q++) {
-      const int lp = q * 32 + (threadIdx.x >> 3);
-      const int p = tile * kLevelTile + lp;
-      int b = 0, e = 0;
-      bool splits = false;
-      if (p < np) parent_range(a, p, b, e, splits);
-      const int end = child_end(a, b, e, splits, o);
-      // every lane joins the shuffle: inside the conditional it would run with
-      // the o == 0 lanes inactive, and octant 1 would read an inactive lane
-      const int prev_end = __shfl_up(end, 1, 8);
-      const int beg = o == 0 ? b : prev_end;
-      s_end[lp * 8 + o] = end;
-      const uint64_t mc = ballot64(end > beg);
-      if (o == 0) s_pre[lp] = __popc((uint32_t)((mc >> (gsub * 8)) & 0xFFull));
+  const int start = dc->lvl_start[a.L];
+  const int pstart = a.L == 1 ? 0 : dc->lvl_start[a.L - 1];
+  int run = a.blk[blockIdx.x];
+  int nodes = 0, leaves = 0, capped = 0;
+  __syncthreads();
+  for (int pb = p0; pb < p1; pb += kBlock / 8) {
+    const int p = pb + (threadIdx.x >> 3);
+    const bool active = p < p1;
+    int b = 0, e = 0;
+    bool splits = false;
+    if (active) parent_range(a, p, b, e, splits);
+    int end = b, beg = b;
+    if (active && splits) {
+      end = a.bnd[8 * p + o];
+      beg = (o == 0) ? b : a.bnd[8 * p + o - 1];
     }
-    __syncthreads();
-    // exclusive scan of the child counts over the tile
-    const int v = threadIdx.x < kLevelTile ? s_pre[threadIdx.x] : 0;
-    const int incl = wave_incl_scan(v);
-    if (lane_id() == kWave - 1) shw[threadIdx.x >> 6] = incl;
+    const int cnt = (active && splits) ? end - beg : 0;
+    const bool is_child = cnt > 0;
+    const bool is_node = is_child && cnt > a.nleaf && a.L < kMaxLevel;
+    // 8-bit masks of this parent's children / node children
+    const uint64_t mc = ballot64(is_child), mn = ballot64(is_node);
+    const int sh = gsub * 8;
+    const uint32_t gc = (uint32_t)((mc >> sh) & 0xFFull), gn = (uint32_t)((mn >> sh) & 0xFFull);
+    const int nchild = __popc(gc), nnode = __popc(gn);
+    // exclusive scan of nchild over the parents of this block, in order
+    const int per_parent = (o == 0) ? nchild : 0;
+    const int incl = wave_incl_scan(per_parent);
+    if (lane_id() == 63) shw[threadIdx.x >> 6] = incl;
     __syncthreads();
     int wpre = 0;
-    for (int k = 0; k < (int)(threadIdx.x >> 6); k++) wpre += shw[k];
-    const int total = shw[0] + shw[1] + shw[2] + shw[3];
-    if (threadIdx.x < kLevelTile) s_pre[threadIdx.x] = wpre + incl - v;
-    if (threadIdx.x == 0) {
-      // publish, look back over the earlier tiles, publish the inclusive count
-      unsigned long long *mine = a.status + tile;
-      __hip_atomic_store(mine, tag | (tile == 0 ? kLvIncl : kLvAgg) | (unsigned long long)total,
-                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      int excl = 0;
-      if (tile > 0) {
-        int j = tile - 1, spins = 0;
-        for (;;) {
-          const unsigned long long w = __hip_atomic_load(a.status + j, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT);
-          if ((w & (kLvAgg | kLvIncl)) == 0 || (w >> 32) != (tag >> 32)) {   // not published
-            if (++spins > kLvSpins) {
-              atomicOr(&dc->err, kSortErr);
-              atomicOr(&dc->err_sticky, kSortErr);
-              break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-          }
-          excl += (int)(w & kLvCount);
-          if ((w & kLvIncl) || j == 0) break;
-          j--;
-        }
-        __hip_atomic_store(mine, tag | kLvIncl | (unsigned long long)(excl + total),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      s_excl = excl;
-      if (tile == ntiles - 1) {
-        // the last tile knows the level's size
-        int cnt = excl + total;
-        if (start + cnt > a.cap_cells) {
-          dc->err |= 1;
-          dc->err_sticky |= 1;
-          cnt = 0;
-        }
-        dc->lvl_start[L] = start;
-        dc->lvl_count[L] = cnt;
-        if (cnt > 0) dc->nlevels = L;
-      }
-    }
+    for (int k = 0; k < (threadIdx.x >> 6); k++) wpre += shw[k];
+    const int tot = shw[0] + shw[1] + shw[2] + shw[3];
     __syncthreads();
-    const int base = start + s_excl;
-    // emit the children: [nodes in octant order][leaves in octant order]
-    for (int q = 0; q < kLevelTile / 32; q++) {
-      const int lp = q * 32 + (threadIdx.x >> 3);
-      const int p = tile * kLevelTile + lp;
-      const bool active = p < np;
-      int b = 0, e = 0;
-      bool splits = false;
-      if (active) parent_range(a, p, b, e, splits);
-      const int end = s_end[lp * 8 + o];
-      const int beg = o == 0 ? b : s_end[lp * 8 + o - 1];
-      const int cnt = (active && splits) ? end - beg : 0;
-      const bool is_child = cnt > 0;
-      const bool is_node = is_child && cnt > a.nleaf && L < kMaxLevel;
-      const uint64_t mc = ballot64(is_child), mn = ballot64(is_node);
-      const uint32_t gc = (uint32_t)((mc >> (gsub * 8)) & 0xFFull);
-      const uint32_t gn = (uint32_t)((mn >> (gsub * 8)) & 0xFFull);
-      const int nchild = __popc(gc), nnode = __popc(gn);
-      const int first = base + s_pre[lp];
-      if (is_child) {
-        const uint32_t below = (1u << o) - 1u;
-        const int rank = is_node ? __popc(gn & below) : nnode + __popc((gc & ~gn) & below);
-        const int ci = first + rank;
-        if (ci < a.cap_cells) {
-          uint4 cell;
-          cell.x = (L == 1 ? 0u : (uint32_t)(pstart + p)) | ((uint32_t)L << 27);
-          cell.y = is_node ? 0u : 0xFFFFFFFFu;  // a node's first/nchild are set one level down
-          cell.z = (uint32_t)beg;
-          cell.w = (uint32_t)end;
-          a.cells[ci] = cell;
-        } else {
-          over = 1;
-        }
-        if (is_node) nodes++; else leaves++;
-        if (cnt > a.nleaf && !is_node) capped = 1;
-      }
-      if (active && splits && o == 0) {
-        if (L == 1) a.root_cells[p] = make_int2(first, nchild);
-        else a.cells[pstart + p].y = nchild > 0 ? cell_pack_y(first, nchild) : 0xFFFFFFFFu;
+    // value at the group's lane 0 (its exclusive prefix)
+    const int excl_lane0 = __shfl(incl - per_parent, (threadIdx.x & 63) & ~7, kWave);
+    const int first = start + run + wpre + excl_lane0;
+    run += tot;
+    if (is_child) {
+      const uint32_t below = (1u << o) - 1u;
+      const int rank = is_node ? __popc(gn & below) : nnode + __popc((gc & ~gn) & below);
+      const int ci = first + rank;
+      const uint32_t parent_idx = a.L == 1 ? 0u : (uint32_t)(pstart + p);
+      uint4 cell;
+      cell.x = parent_idx | ((uint32_t)a.L << 27);
+      cell.y = is_node ? 0u : 0xFFFFFFFFu;  // node's first/nchild set one level down
+      cell.z = (uint32_t)beg;
+      cell.w = (uint32_t)end;
+      a.cells[ci] = cell;
+      if (is_node) nodes++; else leaves++;
+      if (cnt > a.nleaf && !is_node) capped = 1;
+    }
+    if (active && splits && o == 0) {
+      if (a.L == 1) {
+        a.root_cells[p] = make_int2(first, nchild);
+      } else {
+        a.cells[pstart + p].y = nchild > 0 ? cell_pack_y(first, nchild) : 0xFFFFFFFFu;
       }
     }
-    __syncthreads();   // the tile's shared arrays are free again
   }
   nodes = wave_sum(nodes);
   leaves = wave_sum(leaves);
   if (lane_id() == 0) {
-    if (nodes) atomicAdd(&dc->nnodes, nodes);
-    if (leaves) atomicAdd(&dc->nleaves, leaves);
+    atomicAdd(&sh_nodes, nodes);
+    atomicAdd(&sh_leaves, leaves);
   }
   if (capped) atomicOr(&dc->flags, (int)BT_FLAG_DEPTH_CAP);
-  if (over) {
-    atomicOr(&dc->err, 1);
-    atomicOr(&dc->err_sticky, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (sh_nodes) atomicAdd(&dc->nnodes, sh_nodes);
+    if (sh_leaves) atomicAdd(&dc->nleaves, sh_leaves);
   }
 }
 
@@ -389,10 +393,8 @@ __global__ void k_levels_init(DevCounters *dc) {
   if (threadIdx.x < kLevels + 2) {
     dc->lvl_start[threadIdx.x] = 0;
     dc->lvl_count[threadIdx.x] = 0;
-    dc->lvl_ctr[threadIdx.x] = 0;
   }
   if (threadIdx.x == 0) {
-    dc->build_epoch++;
     dc->nnodes = dc->nleaves = dc->nlevels = dc->ncells = 0;
     dc->flags = 0;
     dc->err &= ~1;
@@ -596,12 +598,15 @@ void launch_levels(Ctx &c) {
   a.off = c.d_off;
   a.keys = c.geo_keys;
   a.cells = c.cells;
+  a.bnd = c.child_bnd;
+  a.blk = c.lvl_blk;
   a.root_cells = c.root_cells;
-  a.status = c.lvl_status;
   a.dc = c.dc;
   for (int L = 1; L <= kMaxLevel; L++) {
     a.L = L;
-    hipLaunchKernelGGL(k_level_build, dim3(kLevelBuildBlocks), dim3(kBlock), 0, c.stream, a);
+    hipLaunchKernelGGL(k_level_count, dim3(kLevelBlocks), dim3(kBlock), 0, c.stream, a);
+    hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(kBlock), 0, c.stream, a);
+    hipLaunchKernelGGL(k_level_emit, dim3(kLevelBlocks), dim3(kBlock), 0, c.stream, a);
   }
   hipLaunchKernelGGL(k_levels_finish, dim3(1), dim3(64), 0, c.stream, c.dc, c.level_beg);
 }
