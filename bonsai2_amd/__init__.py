@@ -214,7 +214,7 @@ class Treecode:
         self._ck(self._L.bt_set_walk_variant(self._h, int(variant)), "set_walk_variant")
 
     def set_walk_lds_stack(self, entries):
-        """Learned-angle walk: stack entries per wave kept in LDS (512..1024)."""
+        """FP32 walks: stack entries per wave kept in LDS (512..1024)."""
         self._ck(self._L.bt_set_walk_lds_stack(self._h, int(entries)), "set_walk_lds_stack")
 
     def set_group_mode(self, mode):

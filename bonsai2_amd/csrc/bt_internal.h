@@ -39,7 +39,7 @@ struct LearnBuffers {
 // then 4 words of partials per block.
 constexpr int kWalkStatBlocks = 256;
 
-// Learned walk: stack entries per wave kept in LDS (bt_set_walk_lds_stack).
+// FP32 walks: stack entries per wave kept in LDS (bt_set_walk_lds_stack).
 constexpr int kLdsStack = 1024;
 #ifndef BT_LDS_STACK_MIN
 #define BT_LDS_STACK_MIN 512   // the children one walk step can push (8 * 64)
@@ -158,7 +158,7 @@ struct Ctx {
   int *stack = nullptr;
   size_t stack_alloc = 0;
   int stack_ints = 16384;
-  int lds_stack = kLdsStack;   // learned walk: stack entries per wave kept in LDS
+  int lds_stack = kLdsStack;   // FP32 walks: stack entries per wave kept in LDS
   int group_mode = 0;   // 0: PH-key groups (makeGroups); 1: leaves; 2: tree cells <= ncrit
   int walk_blocks = 0;         // persistent grid (blocks of 256)
   int blocks_per_cu = 0;

@@ -28,6 +28,12 @@ namespace {
 constexpr int kSortThreads = 256;
 constexpr int kSortRounds = 16;
 constexpr int kSortTile = kSortThreads * kSortRounds;
+// The three-launch passes run the same 4096-pair tile with 512 threads (8
+// waves of 8 rounds): twice the waves per CU for the same LDS.
+constexpr int kCThreads = 512;
+constexpr int kCRounds = kSortTile / kCThreads;
+constexpr int kCWaves = kCThreads / kWave;
+static_assert(kCThreads * kCRounds == kSortTile, "tile of the three-launch passes");
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
@@ -79,32 +85,33 @@ __device__ __forceinline__ int block_incl_scan(int v, int *sh_wave, int *total) 
 }
 
 template <bool BYSYS>
-__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__restrict__ keys,
-                                                             const int *__restrict__ vals,
-                                                             int n, DigitSrc src, int nblocks,
-                                                             int *__restrict__ ghist) {
+__global__ __launch_bounds__(kCThreads) void k_radix_hist(const uint64_t *__restrict__ keys,
+                                                          const int *__restrict__ vals,
+                                                          int n, DigitSrc src, int nblocks,
+                                                          int *__restrict__ ghist) {
   __shared__ int hist[256];
-  hist[threadIdx.x] = 0;
+  const int tid = threadIdx.x;
+  if (tid < 256) hist[tid] = 0;
   const int base = blockIdx.x * kSortTile;
   // all of the tile's loads first, then the LDS counting
-  int dg[kSortRounds];
+  int dg[kCRounds];
 #pragma unroll
-  for (int r = 0; r < kSortRounds; r++) {
-    const int i = base + r * kSortThreads + threadIdx.x;
+  for (int r = 0; r < kCRounds; r++) {
+    const int i = base + r * kCThreads + tid;
     const int ic = min(i, n - 1);   // unconditional loads: no branch between them
     const int d = digit_of<BYSYS>(BYSYS ? 0ull : keys[ic], BYSYS ? vals[ic] : 0, src);
     dg[r] = i < n ? d : -1;
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kSortRounds; r++)
+  for (int r = 0; r < kCRounds; r++)
     if (dg[r] >= 0) atomicAdd(&hist[dg[r]], 1);
   __syncthreads();
-  ghist[threadIdx.x * nblocks + blockIdx.x] = hist[threadIdx.x];
+  if (tid < 256) ghist[tid * nblocks + blockIdx.x] = hist[tid];
 }
 
 template <bool BYSYS>
-__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
+__global__ __launch_bounds__(kCThreads) void k_radix_scatter(
     const uint64_t *__restrict__ keys, const int *__restrict__ vals, int n, DigitSrc src,
     int nblocks, const int *__restrict__ goff, uint64_t *__restrict__ keys_out,
     int *__restrict__ vals_out) {
@@ -112,40 +119,46 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
   __shared__ int svals[kSortTile];
   __shared__ int tstart[256];     // first tile-local position of each digit
   __shared__ int gbase[256];      // global position of this tile's first pair of each digit
-  __shared__ int wcnt[4][256];    // per-wave digit counts -> per-wave digit bases
-  __shared__ int sh_wave[4];
+  __shared__ int wcnt[kCWaves][256];   // per-wave digit counts -> per-wave digit bases
+  __shared__ int sh_wave[kCWaves];
   const int tid = threadIdx.x;
   const int w = tid >> 6;
   const int lane = lane_id();
   const int hist_n = 256 * nblocks;
   {
-    // this tile's count of digit tid from the scanned histogram (digit-major:
-    // the next entry is the same digit's next tile, or the next digit's first)
-    const int idx = tid * nblocks + blockIdx.x;
-    const int g0 = goff[idx];
-    const int g1 = idx + 1 < hist_n ? goff[idx + 1] : n;
+    // thread d < 256: this tile's count of digit d from the scanned histogram
+    // (digit-major: the next entry is the same digit's next tile, or the next
+    // digit's first); every thread joins the block scan
+    int g0 = 0, cnt = 0;
+    if (tid < 256) {
+      const int idx = tid * nblocks + blockIdx.x;
+      g0 = goff[idx];
+      const int g1 = idx + 1 < hist_n ? goff[idx + 1] : n;
+      cnt = g1 - g0;
+    }
     int tot;
-    const int incl = block_incl_scan<OpSum>(g1 - g0, sh_wave, &tot);
-    tstart[tid] = incl - (g1 - g0);
-    gbase[tid] = g0;
+    const int incl = block_incl_scan<OpSum>(cnt, sh_wave, &tot);
+    if (tid < 256) {
+      tstart[tid] = incl - cnt;
+      gbase[tid] = g0;
+    }
   }
-#pragma unroll
-  for (int k = 0; k < 4; k++) wcnt[k][tid] = 0;
+  for (int k = tid; k < kCWaves * 256; k += kCThreads) (&wcnt[0][0])[k] = 0;
   __syncthreads();
-  // Each wave ranks its own contiguous quarter of the tile (16 rounds of 64
+  // Each wave ranks its own contiguous eighth of the tile (8 rounds of 64
   // pairs) against its private digit counters; pairs stay in registers.
-  const int wbase = blockIdx.x * kSortTile + w * (kSortRounds * kWave);
-  uint64_t kk[kSortRounds];
-  int vv[kSortRounds], dd[kSortRounds], rr[kSortRounds];
-  // the wave's 16 rounds of pairs are all loaded before the ranking starts
+  const int wbase = blockIdx.x * kSortTile + w * (kCRounds * kWave);
+  uint64_t kk[kCRounds];
+  int vv[kCRounds], dd[kCRounds], rr[kCRounds];
+  // the wave's rounds of pairs are all loaded before the ranking starts
 #pragma unroll
-  for (int r = 0; r < kSortRounds; r++) {
+  for (int r = 0; r < kCRounds; r++) {
     const int i = wbase + r * kWave + lane;
     kk[r] = i < n ? keys[i] : 0ull;
     vv[r] = i < n ? vals[i] : 0;
   }
 #pragma unroll
-  for (int r = 0; r < kSortRounds; r++) {
+  for (int r = 0; r < kCRounds; r++) {
     const int i = wbase + r * kWave + lane;
     const bool valid = i < n;
     const uint64_t key = kk[r];
@@ -167,11 +180,11 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     rr[r] = before + rank;
   }
   __syncthreads();
-  {
+  if (tid < 256) {
     // digit tid: the waves' pairs follow each other in tile order
     int acc = tstart[tid];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < kCWaves; k++) {
       const int c = wcnt[k][tid];
       wcnt[k][tid] = acc;
       acc += c;
@@ -179,7 +192,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kSortRounds; r++) {
+  for (int r = 0; r < kCRounds; r++) {
     if (dd[r] >= 0) {
       const int pos = wcnt[w][dd[r]] + rr[r];
       skeys[pos] = kk[r];
@@ -189,7 +202,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
   __syncthreads();
   const int base = blockIdx.x * kSortTile;
   const int tile_n = min(kSortTile, n - base);
-  for (int j = tid; j < tile_n; j += kSortThreads) {
+  for (int j = tid; j < tile_n; j += kCThreads) {
     const uint64_t key = skeys[j];
     const int val = svals[j];
     const int d = digit_of<BYSYS>(key, val, src);
@@ -198,7 +211,6 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     vals_out[dst] = val;
   }
 }
-
 
 // ---- one-sweep passes ------------------------------------------------------
 // The digit counts of every pass come from one histogram launch over the
@@ -667,10 +679,10 @@ void radix_sort_pairs(Ctx &c, uint64_t *keys, int *vals, uint64_t *keys_tmp, int
     DigitSrc src{p.shift, c.nsys, c.d_off};
     auto hist = p.by_sys ? k_radix_hist<true> : k_radix_hist<false>;
     auto scatter = p.by_sys ? k_radix_scatter<true> : k_radix_scatter<false>;
-    hipLaunchKernelGGL(hist, dim3(nblocks), dim3(kSortThreads), 0, c.stream, ks, vs, n, src,
+    hipLaunchKernelGGL(hist, dim3(nblocks), dim3(kCThreads), 0, c.stream, ks, vs, n, src,
                        nblocks, c.sort_hist);
     scan_exclusive_sum(c, c.sort_hist, c.sort_hist, hist_n, nullptr);
-    hipLaunchKernelGGL(scatter, dim3(nblocks), dim3(kSortThreads), 0, c.stream, ks, vs, n, src,
+    hipLaunchKernelGGL(scatter, dim3(nblocks), dim3(kCThreads), 0, c.stream, ks, vs, n, src,
                        nblocks, c.sort_hist, kd, vd);
     std::swap(ks, kd);
     std::swap(vs, vd);

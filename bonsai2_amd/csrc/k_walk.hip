@@ -62,13 +62,13 @@ constexpr int kMaxWalkSteps = 1 << 20;
 #define BT_XCD_QUEUE 1
 #endif
 constexpr int kXcd = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin over them
-// Entries of a learned walk's stack held in LDS per wave (kLdsStack = 1024,
-// 4 KiB).  The learned walk runs at 4 waves per SIMD, so 4 blocks share a CU's
-// 160 KiB of LDS; the fixed-angle walk runs at up to 8 and keeps its whole
-// stack in global memory.  A deeper stack spills its oldest entries to the
-// wave's global stack.  The part used (WalkArgs::lstk_cap) is at least the
-// 8 * 64 children one step can push, so a step's pushes always fit after a
-// spill.
+// Entries of an FP32 walk's stack held in LDS per wave (kLdsStack = 1024,
+// 4 KiB).  The FP32 walks run at 4 waves per SIMD (their registers allow no
+// more), so 4 blocks of 34 KiB share a CU's 160 KiB of LDS; the FP64 walk
+// keeps its whole stack in global memory.  A deeper stack spills its oldest
+// entries to the wave's global stack.  The part used (WalkArgs::lstk_cap) is
+// at least the 8 * 64 children one step can push, so a step's pushes always
+// fit after a spill.
 
 // Next group for this wave.  With per-XCD queues the groups (PH order) are cut
 // into kXcd contiguous ranges; a wave drains the range of its own XCD first,
@@ -415,14 +415,14 @@ void k_walk(WalkArgs<R> a) {
   static_assert(!(LEARN || SJ) || sizeof(R) == 4, "learned angle and scalar loads: FP32 only");
   __shared__ WaveShared<R> sh[kWavesPerBlock];
   __shared__ float s_mlp[LEARN ? 64 : 1];
-  __shared__ int s_stack[LEARN ? kWavesPerBlock * kLdsStack : 1];
+  constexpr bool LSTK = sizeof(R) == 4;
+  __shared__ int s_stack[LSTK ? kWavesPerBlock * kLdsStack : 1];
   const int lane = lane_id();
   const int wv = threadIdx.x >> 6;
   WaveShared<R> &S = sh[wv];
   // LSTK: the top of the stack (up to kLdsStack entries) lives in LDS, the
   // older entries below it in global memory (stk[0, gtop)); otherwise the
   // whole stack is in global memory (stk[0, top))
-  constexpr bool LSTK = LEARN;
   int *stk = a.stack + (size_t)(blockIdx.x * kWavesPerBlock + wv) * (size_t)a.stack_ints;
   int *lstk = s_stack + (LSTK ? wv * kLdsStack : 0);
   half8_t ma1, ma2_0, ma2_1;

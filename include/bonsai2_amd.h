@@ -184,10 +184,11 @@ int bt_set_walk_options(bt_ctx *ctx, int kahan, int stack_ints, int blocks_per_c
  * LDS read serves two interactions, at the cost of registers and occupancy).
  * All evaluate the same interaction lists; forces agree to FP32 rounding. */
 int bt_set_walk_variant(bt_ctx *ctx, int variant);
-/* Learned-angle walk only (no reference counterpart): entries of each wave's
+/* Treecode<float> walks (no reference counterpart): entries of each wave's
  * traversal stack kept in LDS, 512..1024 (default 1024); deeper entries spill
  * to the global stack of bt_set_walk_options and come back in LIFO order, so
- * the traversal and the forces do not depend on it. */
+ * the traversal and the forces do not depend on it.  A walk whose stack
+ * needs more than these entries plus stack_ints returns BT_ERR_STACK. */
 int bt_set_walk_lds_stack(bt_ctx *ctx, int entries);
 /* Groups of the walk (no reference use; code/buildTree.cu:954-958 collects the
  * leaf list d_leafList and leaves it unused): mode 0 (default) = makeGroups'

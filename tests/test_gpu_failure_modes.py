@@ -84,14 +84,18 @@ def initial_cell_capacity(n, nsys=1):
 # ---- walk stack ------------------------------------------------------------
 
 def test_walk_stack_overflow_is_an_error(bt):
-    n = 1 << 15
-    pos, vel = bt.plummer(n, seed=21)
-    t = forces_of(bt, pos, vel)
+    """The fixed-angle walk keeps up to 512 stack entries in LDS here and
+    spills the rest to a 64-entry global stack: a tree whose walk needs more
+    than 576 entries overflows, in a walk, an eager step and a captured step."""
+    n = 1 << 20
+    pos, vel = bt.clustered(n, seed=22)
+    t = forces_of(bt, pos, vel, eps=0.002)
     try:
         ref = t.acc(bt.ORDER_PH)
         deepest = t.walk_stack_max()
-        assert deepest > 64
-        print("deepest stack %d entries; walking with 64" % deepest, flush=True)
+        print("deepest stack %d entries; walking with 512 in LDS + 64" % deepest, flush=True)
+        assert deepest > 512 + 64
+        t.set_walk_lds_stack(512)
         t.set_walk_options(kahan=1, stack_ints=64)
         with pytest.raises(bt.BonsaiError, match=r"\(%d\).*stack" % BT_ERR_STACK):
             t.computeForces()
