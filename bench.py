@@ -203,7 +203,7 @@ def config_c1(bt):
     ek1, ep1 = t.computeEnergies()
     r.update(p50_rel_err=p50, p99_rel_err=p99, after_step={"p50_rel_err": q50, "p99_rel_err": q99,
              "dt": 1.0 / 64}, error_samples=w["n"], step_ms=sum(t.timings().values()),
-             rel_energy_change=float(abs((ek1 + ep1) - (ek0 + ep0)) / abs(ek0 + ep0)))
+             rel_energy_change=float(abs((ek1 + ep1).sum() - (ek0 + ep0).sum()) / abs((ek0 + ep0).sum())))
     t.close()
     return r
 
@@ -224,7 +224,7 @@ def config_c2(bt):
     ek1, ep1 = t.computeEnergies()
     p50, p99 = rel_errors(t, bt, sample_idx(w["n"], 4096, SEED + 3))
     r.update(steps=64, dt=1.0 / 128, graph_ms_per_step=ms,
-             rel_energy_drift=float(abs((ek1 + ep1) - (ek0 + ep0)) / abs(ek0 + ep0)),
+             rel_energy_drift=float(abs((ek1 + ep1).sum() - (ek0 + ep0).sum()) / abs((ek0 + ep0).sum())),
              p50_rel_err=p50, p99_rel_err=p99, error_samples=4096)
     t.close()
     return r
@@ -378,6 +378,9 @@ def main():
                          "of known size, for scaling HBM byte counters)")
     ap.add_argument("--monopole", action="store_true",
                     help="monopole-only P2C (the reference without QUADRUPOLE; 20 flops each)")
+    ap.add_argument("--phase-steps", type=int, default=3,
+                    help="steps after the timed ones with the phases in sequence, for "
+                         "per-phase times and HBM fractions")
     ap.add_argument("--compare-steps", type=int, default=3,
                     help="timed fixed-theta steps for the comparison (0 = walk only)")
     args = ap.parse_args()
@@ -451,6 +454,15 @@ def main():
     t.step_graph(args.dt, args.nleaf, args.level_split, args.ncrit, nsteps=args.steps)
     ms_graph = (time.perf_counter() - tg0) * 1e3 / args.steps
     log("headline: %.2f ms/step eager, %.2f ms/step graph" % (ms_step, ms_graph))
+    # each phase alone (outside the timed region): the timed steps run the
+    # groups beside the level build and multipoles, which shares the GPU
+    # between the two branches and inflates both branches' event times
+    t.set_overlap(False)
+    phase_serial = []
+    for _ in range(args.phase_steps):
+        t.step(args.dt, args.nleaf, args.level_split, args.ncrit)
+        phase_serial.append(t.timings())
+    t.set_overlap(True)
 
     walk_ms = np.array(walk_ms)
     flops = np.array(flops)
@@ -466,7 +478,9 @@ def main():
     c = t.counts()
     st_main, _ = t.walk_stats()
     phase_ms = {k: float(np.mean([p[k] for p in phase])) for k in phase[0]}
-    hbm = hbm_phases(t, phase_ms, n, sort_passes=bt.SORT_PASSES)
+    phase_ms_serial = ({k: float(np.mean([p[k] for p in phase_serial])) for k in phase_serial[0]}
+                       if phase_serial else None)
+    hbm = hbm_phases(t, phase_ms_serial or phase_ms, n, sort_passes=bt.SORT_PASSES)
     compare = None
     if learned:
         # the metric's comparison on the same particles and tree: the fixed
@@ -557,9 +571,14 @@ def main():
         "energy": {"kinetic": float(ek.sum()), "potential": float(ep.sum()),
                    "E_start": e_start, "E_end": e_end,
                    "rel_drift": abs(e_end - e_start) / abs(e_start),
-                   "steps_integrated": args.warmup + 2 * args.steps + 1},
+                   "steps_integrated": args.warmup + 2 * args.steps + 1 + args.phase_steps},
         "tree": {"ncells": c["ncells"], "nlevels": c["nlevels"], "ngroups": c["ngroups"]},
+        # timed steps: ph_keys, ph_sort and groups (second stream) overlap
+        # levels and multipoles, each timed with the other branch beside it
         "phase_ms": phase_ms,
+        # the same phases run in sequence (bt_set_overlap off), after the timed
+        # steps; phase_hbm divides algorithmic bytes by these times
+        "phase_ms_serial": phase_ms_serial,
         "phase_hbm": hbm,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,

@@ -63,6 +63,7 @@ _SIGS = {
     "bt_set_walk_options": [P, I, I, I],
     "bt_set_walk_variant": [P, I],
     "bt_set_walk_lds_stack": [P, I],
+    "bt_set_overlap": [P, I],
     "bt_get_walk_stack_max": [P, P],
     "bt_set_group_mode": [P, I],
     "bt_set_quadrupole": [P, I],
@@ -216,6 +217,12 @@ class Treecode:
     def set_walk_lds_stack(self, entries):
         """FP32 walks: stack entries per wave kept in LDS (512..1024)."""
         self._ck(self._L.bt_set_walk_lds_stack(self._h, int(entries)), "set_walk_lds_stack")
+
+    def set_overlap(self, on):
+        """Steps run makeGroups' work on a second stream beside the level build
+        and multipoles (True, default) or every phase in sequence (False; the
+        phase timings are then each phase alone).  Results are bitwise equal."""
+        self._ck(self._L.bt_set_overlap(self._h, int(bool(on))), "set_overlap")
 
     def set_group_mode(self, mode):
         """0 = makeGroups' PH-key groups (default), 1 = one group per leaf, 2 = one

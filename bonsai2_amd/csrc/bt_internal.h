@@ -76,6 +76,7 @@ struct GraphKey {
   double dt, eps, theta;
   int nleaf, level_split, ncrit;
   int opening, kahan, quadrupole, walk_variant, stack_ints, blocks_per_cu, lds_stack, group_mode;
+  int overlap;
   float learn_shift, learn_theta_lo, learn_theta_hi;
   int check_ns;
   float check_p99;
@@ -211,6 +212,15 @@ struct Ctx {
 
   // timing
   hipEvent_t ev[12] = {};
+  // A step runs the groups (PH keys, PH sort, groups) on stream2 beside the
+  // level build and multipoles on stream: both need only the tree-order
+  // particles and sorted keys (overlap = false runs them in sequence on
+  // stream).  forked: the last step's phase events follow that layout
+  // (multipoles end at ev[10], groups at ev[11]).
+  hipStream_t stream2 = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  bool overlap = true;
+  bool forked = false;
   float ms[10] = {};
   bool timing = true;
 

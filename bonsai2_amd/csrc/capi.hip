@@ -155,7 +155,8 @@ int read_counters(Ctx &c) {
 
 // ---- phases (asynchronous; no host synchronisation) ----------------------
 
-int phase_build(Ctx &c) {
+// Box, geometric keys, their sort and the tree-order gather.
+void phase_sort(Ctx &c) {
   record(c, 0);
   launch_bbox(c);
   launch_geo_keys(c);
@@ -163,26 +164,32 @@ int phase_build(Ctx &c) {
   radix_sort_pairs(c, c.geo_keys, c.valA, c.keyB, c.valB, c.n, 63, true);
   record(c, 2);
   launch_gather_tree(c);
+}
+
+int phase_build(Ctx &c) {
+  c.forked = false;
+  phase_sort(c);
   launch_levels(c);
   record(c, 3);
   BT_LAUNCH_CK();
   return 0;
 }
 
-int phase_multipoles(Ctx &c) {
+// end_ev: the event recorded when the phase's work is queued.
+int phase_multipoles(Ctx &c, int end_ev = 4) {
   launch_multipoles(c);
   if (!c.fp64) launch_sys_scale(c);   // the learned angle's scales (Treecode<float> only)
-  record(c, 4);
+  record(c, end_ev);
   BT_LAUNCH_CK();
   return 0;
 }
 
-int phase_groups(Ctx &c) {
+int phase_groups(Ctx &c, int end_ev = 7) {
   if (c.group_mode != 0) {
     record(c, 5);
     record(c, 6);
     launch_leaf_groups(c);
-    record(c, 7);
+    record(c, end_ev);
     BT_LAUNCH_CK();
     return 0;
   }
@@ -192,9 +199,38 @@ int phase_groups(Ctx &c) {
   record(c, 6);
   launch_gather_ph(c);
   launch_groups(c);
-  record(c, 7);
+  record(c, end_ev);
   BT_LAUNCH_CK();
   return 0;
+}
+
+// The tree and the groups of a step, the groups on stream2 beside the level
+// build and the multipoles: after the tree-order gather, the groups read
+// only the particles, the root box and their own scratch (PH keys, the sort
+// buffers, the group arrays), the levels and multipoles only the sorted
+// geometric keys, the particles and the cell arrays, and the device counters
+// they share are updated with atomics.  The walk waits for both.
+int phase_tree_and_groups(Ctx &c) {
+  phase_sort(c);
+  c.forked = true;
+  BT_CK(hipEventRecord(c.fork_ev, c.stream));
+  BT_CK(hipStreamWaitEvent(c.stream2, c.fork_ev, 0));
+  std::swap(c.stream, c.stream2);   // the launches below go to stream2
+  record(c, 4);
+  const int rc_groups = phase_groups(c, 11);
+  const hipError_t e_join = hipEventRecord(c.join_ev, c.stream);
+  std::swap(c.stream, c.stream2);
+  launch_levels(c);
+  record(c, 3);
+  const int rc_mp = phase_multipoles(c, 10);
+  // join before any return, so that a capture never ends with stream2 forked
+  if (e_join == hipSuccess) BT_CK(hipStreamWaitEvent(c.stream, c.join_ev, 0));
+  record(c, 7);
+  if (e_join != hipSuccess) {
+    c.err = std::string("hipEventRecord: ") + hipGetErrorString(e_join);
+    return BT_ERR_HIP;
+  }
+  return rc_groups ? rc_groups : rc_mp;
 }
 
 int phase_forces(Ctx &c) {
@@ -279,8 +315,12 @@ int build_checked(Ctx &c) {
 // the new positions and the closing half kick.  Queued without host
 // synchronisation.
 int step_second_half(Ctx &c, double dt) {
-  if (phase_build(c) || phase_multipoles(c) || phase_groups(c) || phase_forces(c))
+  if (c.overlap) {
+    if (phase_tree_and_groups(c)) return BT_ERR_HIP;
+  } else if (phase_build(c) || phase_multipoles(c) || phase_groups(c)) {
     return BT_ERR_HIP;
+  }
+  if (phase_forces(c)) return BT_ERR_HIP;
   launch_learned_check(c);
   launch_kick(c, 0.5 * dt, true);
   record(c, 9);
@@ -313,9 +353,14 @@ int finish_timings(Ctx &c, int last) {
   for (int k = 0; k < 10; k++) c.ms[k] = 0.f;
   // phases: 0 bbox+keys, 1 sort, 2 levels, 3 multipoles, 4 ph keys, 5 ph sort, 6 groups,
   // 7 walk, 8 integrate, 9 energies
+  // phase k runs from event from[k] to event to[k]; after a forked step the
+  // multipoles end at ev[10] and the groups (on stream2) at ev[11]
   for (int k = 0; k < last && k < 10; k++) {
+    int to = k + 1;
+    if (c.forked && k == 3) to = 10;
+    if (c.forked && k == 6) to = 11;
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, c.ev[k], c.ev[k + 1]) == hipSuccess) c.ms[k] = ms;
+    if (hipEventElapsedTime(&ms, c.ev[k], c.ev[to]) == hipSuccess) c.ms[k] = ms;
   }
   return 0;
 }
@@ -382,6 +427,12 @@ int create_ctx(bt_ctx **out, double eps, double theta, int ncrit, bool fp64) {
     return BT_ERR_HIP;
   }
   c.own_stream = true;
+  if (hipStreamCreateWithFlags(&c.stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c.fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c.join_ev, hipEventDisableTiming) != hipSuccess) {
+    bt_destroy(h);
+    return BT_ERR_HIP;
+  }
   // BT_SORT_ONESWEEP=1 selects the one-sweep radix passes (one launch per pass,
   // decoupled look-back) instead of the three-launch passes (histogram, scan,
   // scatter): same result, measured slower on MI355X, kept for comparison
@@ -431,6 +482,7 @@ int bt_create_f64(bt_ctx **out, double eps, double theta, int ncrit) {
 int bt_destroy(bt_ctx *h) {
   if (!h) return 0;
   Ctx &c = h->c;
+  if (c.stream2) (void)hipStreamSynchronize(c.stream2);
   (void)hipStreamSynchronize(c.stream);
   release_graph(c);
   free_particle_arrays(c);
@@ -451,6 +503,9 @@ int bt_destroy(bt_ctx *h) {
   for (int k = 0; k < 12; k++)
     if (c.ev[k]) (void)hipEventDestroy(c.ev[k]);
   if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
+  if (c.stream2) (void)hipStreamDestroy(c.stream2);
+  if (c.fork_ev) (void)hipEventDestroy(c.fork_ev);
+  if (c.join_ev) (void)hipEventDestroy(c.join_ev);
   delete h;
   return 0;
 }
@@ -523,6 +578,13 @@ int bt_set_walk_variant(bt_ctx *h, int variant) {
   if (!h) return BT_ERR_ARG;
   if (variant < 0 || variant > 2) return fail(h->c, BT_ERR_ARG, "walk variant must be 0, 1 or 2");
   h->c.walk_variant = variant;
+  return 0;
+}
+
+int bt_set_overlap(bt_ctx *h, int on) {
+  if (!h) return BT_ERR_ARG;
+  if (on != 0 && on != 1) return fail(h->c, BT_ERR_ARG, "overlap must be 0 or 1");
+  h->c.overlap = on != 0;
   return 0;
 }
 
@@ -1475,6 +1537,7 @@ int bt_step_graph(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit, in
   key.blocks_per_cu = c.blocks_per_cu;
   key.lds_stack = c.lds_stack;
   key.group_mode = c.group_mode;
+  key.overlap = c.overlap;
   if (c.opening == 1) {
     key.learn_shift = c.learn_shift;
     key.learn_theta_lo = c.learn_theta_lo;

@@ -159,7 +159,7 @@ void launch_gather_ph(Ctx &c) {
 
 // A new set of groups starts without the group error bit.
 __global__ void k_groups_init(DevCounters *dc) {
-  if (threadIdx.x == 0) dc->err &= ~2;
+  if (threadIdx.x == 0) atomicAnd(&dc->err, ~2);   // the level build may run beside this
 }
 
 // Tree groups: flag the first particle of every group cell.  ncrit = 0: the

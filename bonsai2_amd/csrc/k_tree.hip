@@ -288,8 +288,8 @@ __global__ __launch_bounds__(kBlock) void k_level_scan(LevelArgs a) {
     const int start = a.L == 1 ? 0 : dc->lvl_start[a.L - 1] + dc->lvl_count[a.L - 1];
     int total = run;
     if (start + total > a.cap_cells) {
-      dc->err |= 1;
-      dc->err_sticky |= 1;
+      atomicOr(&dc->err, 1);
+      atomicOr(&dc->err_sticky, 1);
       total = 0;
     }
     dc->lvl_start[a.L] = start;
@@ -397,7 +397,7 @@ __global__ void k_levels_init(DevCounters *dc) {
   if (threadIdx.x == 0) {
     dc->nnodes = dc->nleaves = dc->nlevels = dc->ncells = 0;
     dc->flags = 0;
-    dc->err &= ~1;
+    atomicAnd(&dc->err, ~1);   // other bits belong to phases that may run beside this one
   }
 }
 

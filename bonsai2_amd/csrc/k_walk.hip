@@ -815,7 +815,7 @@ __global__ void k_walk_reset(DevCounters *dc, const int *run_if) {
     dc->queue = 0;
     dc->stack_max = 0;
     for (int x = 0; x < kXcd; x++) dc->xq[x] = 0;
-    dc->err &= ~4;
+    atomicAnd(&dc->err, ~4);
   }
 }
 

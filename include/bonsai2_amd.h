@@ -61,7 +61,10 @@ int bt_destroy(bt_ctx *ctx);
  * systems ("forest"); offsets[0..nsys] partition [0, n) (null when nsys=1).
  * A later call with a larger n or different systems reallocates. */
 int bt_alloc(bt_ctx *ctx, int n, int nsys, const int *offsets);
-/* Queue all work on this stream (default: the context's own stream). */
+/* Queue all work on this stream (default: the context's own stream).  A step
+ * (bt_step, bt_step_graph) also runs part of its work on a second stream the
+ * context owns, forked from and joined back into this one (bt_set_overlap), so
+ * work queued on this stream after a step sees all of it. */
 int bt_set_stream(bt_ctx *ctx, void *hip_stream);
 /* eps/theta are public members in the reference (code/Treecode.h:133). */
 int bt_set_params(bt_ctx *ctx, float eps, float theta);
@@ -168,7 +171,10 @@ int bt_direct_fp64(bt_ctx *ctx, int ns, const int *ph_idx, double *out4);
 int bt_direct_fp64_noalloc(bt_ctx *ctx, int ns, const int *d_ph_idx, double *d_out4);
 /* Milliseconds of the phases of the last step, measured with HIP events on
  * the context stream: {bbox+keys, sort, tree levels, multipoles, ph keys,
- * ph sort, groups, walk, integrate, energies}. */
+ * ph sort, groups, walk, integrate, energies}.  In a step run with
+ * bt_set_overlap(ctx, 1) the last three of the first seven (on a second
+ * stream) overlap the levels and multipoles, and each is timed with the
+ * other branch beside it. */
 int bt_get_timings(bt_ctx *ctx, float ms[10]);
 /* Walk options: kahan = compensated accumulators (default 1);
  * stack_ints = per-wave traversal stack capacity (default 16384, at least 64;
@@ -190,6 +196,15 @@ int bt_set_walk_variant(bt_ctx *ctx, int variant);
  * the traversal and the forces do not depend on it.  A walk whose stack
  * needs more than these entries plus stack_ints returns BT_ERR_STACK. */
 int bt_set_walk_lds_stack(bt_ctx *ctx, int entries);
+/* Steps of bt_step and bt_step_graph (no reference counterpart: the
+ * reference's phases run in sequence on one stream, code/treecode.cu:137-165):
+ * on = 1 (default) runs makeGroups' work (PH keys, their sort, the groups) on
+ * a second stream beside the level build and the multipoles, both after the
+ * geometric-key sort, and the walk after both; on = 0 runs the phases in
+ * sequence, so that bt_get_timings gives each phase alone.  The results are
+ * bitwise the same.  bt_build_tree, bt_compute_multipoles and bt_make_groups
+ * always run on the context stream. */
+int bt_set_overlap(bt_ctx *ctx, int on);
 /* Groups of the walk (no reference use; code/buildTree.cu:954-958 collects the
  * leaf list d_leafList and leaves it unused): mode 0 (default) = makeGroups'
  * PH-key groups (code/makeGroups.cu); mode 1 = one group per leaf cell, in

@@ -1,8 +1,9 @@
 """GPU tests of the batched paths: a forest of independent systems in one
 context (one persistent walk over all systems' groups) against the oracle run
 on each system alone (PH groups and leaf groups), the HIP-graph step against
-the eager step (fixed angle, and learned angle with its per-step check), and
-the autotuner.
+the eager step (fixed angle, and learned angle with its per-step check), steps
+with makeGroups' work on a second stream against steps in sequence, and the
+autotuner.
 
 Bars: per system, forces within 1e-5 relative RMS of the oracle walk on that
 system alone and interaction counts equal; graph replays bit-identical to the
@@ -77,6 +78,52 @@ def test_graph_step_matches_eager_step(bt):
     finally:
         eager.close()
         graph.close()
+
+
+@pytest.mark.parametrize("nsys", [1, 5])
+def test_overlapped_step_matches_serial_step(bt, nsys):
+    """bt_set_overlap: the groups branch on a second stream gives bitwise the
+    results of the phases in sequence, eager and graph-replayed."""
+    if nsys == 1:
+        pos, vel = bt.plummer(1 << 15, seed=13)
+        off = None
+        n = len(pos)
+    else:
+        pos, vel, off = bt.ensemble(nsys=nsys, seed=13, log2_min=11, log2_max=13)
+        n = int(off[-1])
+    ts = []
+    for overlap in (True, False, True, False):
+        t = bt.Treecode(EPS, THETA, 64)
+        t.set_overlap(overlap)
+        if off is None:
+            t.alloc(n)
+        else:
+            t.alloc(n, off)
+        t.ptcl_h2d(pos, vel)
+        t.buildTree(16)
+        t.computeMultipoles()
+        t.makeGroups(5, 64)
+        t.computeForces()
+        ts.append(t)
+    try:
+        for step in range(3):
+            ts[0].step(1.0 / 128, 16, 5, 64)
+            ts[1].step(1.0 / 128, 16, 5, 64)
+            ts[2].step_graph(1.0 / 128, 16, 5, 64, nsteps=1)
+            ts[3].step_graph(1.0 / 128, 16, 5, 64, nsteps=1)
+            p0, v0 = ts[0].ptcl_d2h(bt.ORDER_INPUT)
+            a0 = ts[0].acc(bt.ORDER_INPUT)
+            g0 = ts[0].group_counts()
+            for k in (1, 2, 3):
+                pk, vk = ts[k].ptcl_d2h(bt.ORDER_INPUT)
+                assert np.array_equal(p0, pk) and np.array_equal(v0, vk), (step, k)
+                assert np.array_equal(a0, ts[k].acc(bt.ORDER_INPUT)), (step, k)
+                assert np.array_equal(g0, ts[k].group_counts()), (step, k)
+        tm = ts[1].timings()
+        assert all(tm[k] > 0 for k in ("sort", "levels", "multipoles", "ph_sort", "groups", "walk"))
+    finally:
+        for t in ts:
+            t.close()
 
 
 def test_autotune_picks_a_candidate_and_stays_correct(bt, oracle):
