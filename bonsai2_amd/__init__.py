@@ -79,6 +79,7 @@ _SIGS = {
     "bt_set_opening": [P, I, F, F, F],
     "bt_set_opening_check": [P, I, F, ctypes.c_uint64],
     "bt_get_opening_check": [P, P],
+    "bt_get_check_samples": [P, P],
     "bt_ic_plummer": [I, ctypes.c_uint64, D, P, P],
     "bt_ic_clustered": [I, I, ctypes.c_uint64, D, D, D, P, P],
     "bt_ic_halo_disk": [I, ctypes.c_uint64, D, D, D, P, P],
@@ -166,6 +167,7 @@ class Treecode:
         self.nLeaf = 16
         self.nPtcl = 0
         self.nsys = 1
+        self._check_ns = 0
         self.offsets = None
         self.grav_potential = 0.0
 
@@ -324,6 +326,7 @@ class Treecode:
     def set_opening_check(self, nsamples=256, p99_max=1e-3, seed=1):
         self._ck(self._L.bt_set_opening_check(self._h, int(nsamples), float(p99_max), int(seed)),
                  "set_opening_check")
+        self._check_ns = int(nsamples)
 
     def use_learned_theta(self, path=None, check_samples=256, p99_max=1e-3, budget=None):
         """Loads a trained network (bonsai2_amd/theta_mlp.json by default) and
@@ -342,6 +345,13 @@ class Treecode:
         self._ck(self._L.bt_get_opening_check(self._h, _p(o)), "opening_check")
         return dict(p50=o[0], p99=o[1], max=o[2], fell_back=bool(o[3]), fallbacks=int(o[4]),
                     checks=int(o[5]))
+
+    def check_samples(self):
+        """PH indices of the last re-check's samples, in draw order."""
+        ns = min(self._check_ns, self.nPtcl)
+        idx = np.empty(ns, np.int32)
+        self._ck(self._L.bt_get_check_samples(self._h, _p(idx)), "check_samples")
+        return idx
 
     # -- particles ---------------------------------------------------------
     def ptcl_h2d(self, pos, vel):

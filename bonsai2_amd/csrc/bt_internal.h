@@ -181,8 +181,21 @@ struct Ctx {
   int check_ns = 0;              // FP64 samples re-checked after each learned walk
   float check_p99 = 1e-3f;       // above this p99 the step falls back to fixed theta
   unsigned long long check_seed = 1;
+  // 4 * ns ints: the draws' indices into the particle arrays as they were when
+  // drawn, the draws' input ids sorted, the draw slot of each sorted id, and
+  // each draw's PH index (filled once the step's groups exist)
   int *d_check_idx = nullptr;
   double *d_check = nullptr;     // direct results (4 * ns) + {p50, p99, max}
+  // A step sums the check's FP64 forces on stream3 beside the build, from the
+  // particles as the drift left them (the tree-order gather writes the other
+  // particle buffer).  check_state: 0 no sum queued for the coming check, 1
+  // queued on the context stream, 2 queued on stream3 (joins the context
+  // stream at chk_join_ev).  check_slices: source slices of the sum (fewer
+  // slices leave the build kernels room on the CUs)
+  hipStream_t stream3 = nullptr;
+  hipEvent_t chk_fork_ev = nullptr, chk_join_ev = nullptr;
+  int check_state = 0;
+  int check_slices = 192;
 
   // energies / direct sum scratch
   double *d_energy = nullptr;  // 2 * nsys
@@ -261,22 +274,27 @@ int mlp_grad_blocks();
 void launch_mlp_grad(Ctx &c, const float *params, int n, const float *feat, const float *target,
                      int batch, unsigned long long seed, int step, float *partial, double *grad);
 void launch_mlp_adam(Ctx &c, float *p, float *m, float *v, const double *grad, float lr, int step);
-// sample indices of the next check, drawn on the device from check_seed and
-// the device's check count
+// targets of the next check, drawn on the device from check_seed and the
+// device's check count: indices into the current particle arrays (c.pos,
+// c.id), and their input ids sorted for launch_check_map
 void launch_check_idx(Ctx &c, int ns);
+// each target's PH index (after the step's groups), from c.id_ph
+void launch_check_map(Ctx &c, int ns);
 // p50/p99/max of the relative force errors into out[0..2]; sets the device's
 // fallback flag (p99 > check_p99) and counts checks and fallbacks
 void launch_learn_check(Ctx &c, int ns, const int *d_idx, const double *d_direct, double *out);
 
 // closing: the second half kick of a step (counted in steps_done)
 void launch_kick(Ctx &c, double dt_half, bool closing = false);
-void launch_drift(Ctx &c, double dt);
+// the opening half kick (dth) and the drift (dt) of a step, in one pass
+void launch_kick_drift(Ctx &c, double dth, double dt);
 void launch_clear_sticky(Ctx &c);
 void launch_energies(Ctx &c);
 int direct_slices(int ns);   // d_out of launch_direct: ns * (slices + 1) double4
 void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out, bool check = false);
+// src: the particles targets and sources are read from (null: c.pos_ph)
 void launch_direct_parts(Ctx &c, int ns, const int *d_idx, double *d_out, double *d_part,
-                         int slices, bool check = false);
+                         int slices, bool check = false, const void *src = nullptr);
 
 // host-side helpers
 int grid_for(int n, int block, int cap);

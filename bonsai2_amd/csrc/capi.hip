@@ -263,15 +263,63 @@ double *check_result(Ctx &c) {
 // their p99 exceeds check_p99 the fixed-angle walk (the fallback) recomputes
 // the forces, otherwise its launches return at once.  No host
 // synchronisation, so a captured step holds the check too.
-void launch_learned_check(Ctx &c) {
-  if (c.opening != 1 || !c.have_mlp || c.check_ns <= 0) return;
+bool check_active(const Ctx &c) { return c.opening == 1 && c.have_mlp && c.check_ns > 0; }
+
+// Makes the context stream wait for a check sum queued on stream3.
+int join_check(Ctx &c) {
+  if (c.check_state == 2) {
+    c.check_state = 1;
+    BT_CK(hipStreamWaitEvent(c.stream, c.chk_join_ev, 0));
+  }
+  return 0;
+}
+
+// The learned walk's check, first part: draw the targets among the particle
+// arrays as they are now and sum their FP64 forces.  The step queues it
+// before its build, with fork = true on stream3 beside the build (the
+// tree-order gather writes the other particle buffer, so the arrays read
+// here stay unchanged until the next step's gather, which follows the join).
+int queue_check_direct(Ctx &c, bool fork) {
+  if (join_check(c)) return BT_ERR_HIP;   // one left forked by a call that failed
+  c.check_state = 0;
+  if (!check_active(c)) return 0;
   const int ns = std::min(c.check_ns, c.n);
+  const int slices = std::max(1, std::min(c.check_slices, direct_slices(ns)));
+  if (fork) {
+    BT_CK(hipEventRecord(c.chk_fork_ev, c.stream));
+    BT_CK(hipStreamWaitEvent(c.stream3, c.chk_fork_ev, 0));
+    std::swap(c.stream, c.stream3);   // the launches below go to stream3
+  }
   launch_check_idx(c, ns);
-  launch_direct(c, ns, c.d_check_idx, c.d_check, true);
-  launch_learn_check(c, ns, c.d_check_idx, c.d_check, check_result(c));
+  launch_direct_parts(c, ns, c.d_check_idx, c.d_check, c.d_check + (size_t)4 * ns, slices, true,
+                      c.pos);
+  if (fork) {
+    const hipError_t e = hipEventRecord(c.chk_join_ev, c.stream);
+    std::swap(c.stream, c.stream3);
+    if (e != hipSuccess) {
+      c.err = std::string("hipEventRecord: ") + hipGetErrorString(e);
+      return BT_ERR_HIP;
+    }
+  }
+  c.check_state = fork ? 2 : 1;
+  return 0;
+}
+
+// The check, second part (after the walk): each target's PH index, the
+// relative errors of the walk's forces, the decision, and the fixed-angle
+// walk, which runs only when the check failed.
+int launch_learned_check(Ctx &c) {
+  if (!check_active(c)) return 0;
+  if (c.check_state == 0 && queue_check_direct(c, false)) return BT_ERR_HIP;
+  const int ns = std::min(c.check_ns, c.n);
+  launch_check_map(c, ns);
+  if (join_check(c)) return BT_ERR_HIP;
+  c.check_state = 0;
+  launch_learn_check(c, ns, c.d_check_idx + (size_t)3 * ns, c.d_check, check_result(c));
   c.opening = 0;
   launch_walk(c, &c.dc->fallback);
   c.opening = 1;
+  return 0;
 }
 
 int check_build_errors(Ctx &c) {
@@ -315,13 +363,13 @@ int build_checked(Ctx &c) {
 // the new positions and the closing half kick.  Queued without host
 // synchronisation.
 int step_second_half(Ctx &c, double dt) {
+  if (queue_check_direct(c, c.overlap)) return BT_ERR_HIP;
   if (c.overlap) {
     if (phase_tree_and_groups(c)) return BT_ERR_HIP;
   } else if (phase_build(c) || phase_multipoles(c) || phase_groups(c)) {
     return BT_ERR_HIP;
   }
-  if (phase_forces(c)) return BT_ERR_HIP;
-  launch_learned_check(c);
+  if (phase_forces(c) || launch_learned_check(c)) return BT_ERR_HIP;
   launch_kick(c, 0.5 * dt, true);
   record(c, 9);
   BT_LAUNCH_CK();
@@ -428,8 +476,11 @@ int create_ctx(bt_ctx **out, double eps, double theta, int ncrit, bool fp64) {
   }
   c.own_stream = true;
   if (hipStreamCreateWithFlags(&c.stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c.stream3, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c.fork_ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c.join_ev, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c.join_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c.chk_fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c.chk_join_ev, hipEventDisableTiming) != hipSuccess) {
     bt_destroy(h);
     return BT_ERR_HIP;
   }
@@ -437,6 +488,10 @@ int create_ctx(bt_ctx **out, double eps, double theta, int ncrit, bool fp64) {
   // decoupled look-back) instead of the three-launch passes (histogram, scan,
   // scatter): same result, measured slower on MI355X, kept for comparison
   if (const char *e = std::getenv("BT_SORT_ONESWEEP")) c.sort_onesweep = std::atoi(e) != 0;
+  // BT_CHECK_SLICES: source slices of the learned walk's per-step FP64 check
+  // (1..2048; a tuning knob, the results differ only in FP64 rounding)
+  if (const char *e = std::getenv("BT_CHECK_SLICES"))
+    c.check_slices = std::max(1, std::min(2048, std::atoi(e)));
   for (int k = 0; k < 12; k++) (void)hipEventCreate(&c.ev[k]);
   if (dev_alloc(c, &c.dc, 1) || dev_alloc(c, &c.level_beg, kLevels) ||
       dev_alloc(c, &c.d_wstats, 4 + 4 * (size_t)kWalkStatBlocks) || dev_alloc(c, &c.d_wpot, 1) ||
@@ -483,6 +538,7 @@ int bt_destroy(bt_ctx *h) {
   if (!h) return 0;
   Ctx &c = h->c;
   if (c.stream2) (void)hipStreamSynchronize(c.stream2);
+  if (c.stream3) (void)hipStreamSynchronize(c.stream3);
   (void)hipStreamSynchronize(c.stream);
   release_graph(c);
   free_particle_arrays(c);
@@ -504,8 +560,11 @@ int bt_destroy(bt_ctx *h) {
     if (c.ev[k]) (void)hipEventDestroy(c.ev[k]);
   if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
   if (c.stream2) (void)hipStreamDestroy(c.stream2);
+  if (c.stream3) (void)hipStreamDestroy(c.stream3);
   if (c.fork_ev) (void)hipEventDestroy(c.fork_ev);
   if (c.join_ev) (void)hipEventDestroy(c.join_ev);
+  if (c.chk_fork_ev) (void)hipEventDestroy(c.chk_fork_ev);
+  if (c.chk_join_ev) (void)hipEventDestroy(c.chk_join_ev);
   delete h;
   return 0;
 }
@@ -750,8 +809,10 @@ int bt_compute_forces(bt_ctx *h, int intcount, double stats[4]) {
     return fail(c, BT_ERR_STATE, "computeMultipoles and makeGroups first");
   launch_clear_sticky(c);
   record(c, 7);
-  if (phase_forces(c)) return BT_ERR_HIP;
-  launch_learned_check(c);
+  // the check's sum runs after this walk, from the tree-order particles
+  if (join_check(c)) return BT_ERR_HIP;
+  c.check_state = 0;
+  if (phase_forces(c) || launch_learned_check(c)) return BT_ERR_HIP;
   // a moveParticles before this evaluation owes its step's closing half kick
   if (c.pending_kick != 0.0) launch_kick(c, c.pending_kick, true);
   BT_LAUNCH_CK();
@@ -789,8 +850,7 @@ int bt_move_particles_f64(bt_ctx *h, double dt) {
   Ctx &c = h->c;
   if (!c.have_acc) return fail(c, BT_ERR_STATE, "computeForces first");
   launch_clear_sticky(c);
-  launch_kick(c, 0.5 * dt);
-  launch_drift(c, dt);
+  launch_kick_drift(c, 0.5 * dt, dt);
   BT_LAUNCH_CK();
   BT_CK(hipStreamSynchronize(c.stream));
   // positions moved: the tree, moments, groups and forces are stale; the
@@ -850,8 +910,7 @@ int bt_step_f64(bt_ctx *h, double dt, int nleaf, int level_split, int ncrit) {
     if (rc) return rc;
   }
   launch_clear_sticky(c);
-  launch_kick(c, 0.5 * dt);
-  launch_drift(c, dt);
+  launch_kick_drift(c, 0.5 * dt, dt);
   c.have_tree = c.have_mp = c.have_groups = c.have_acc = false;
   int rc = step_second_half(c, dt);
   if (!rc) rc = finish_step(c, dt);
@@ -1478,7 +1537,7 @@ int bt_set_opening_check(bt_ctx *h, int nsamples, float p99_max, uint64_t seed) 
     release_graph(c);
     c.check_cap = 0;
     if (dev_alloc(c, &c.d_check, (size_t)nsamples * (direct_slices(nsamples) + 1) * 4 + 4) ||
-        dev_alloc(c, &c.d_check_idx, nsamples))
+        dev_alloc(c, &c.d_check_idx, (size_t)nsamples * 4))
       return BT_ERR_HIP;
     c.check_cap = nsamples;
   }
@@ -1493,6 +1552,15 @@ int bt_set_opening_check(bt_ctx *h, int nsamples, float p99_max, uint64_t seed) 
     BT_CK(hipMemcpyAsync(check_result(c), z3, sizeof(z3), hipMemcpyHostToDevice, c.stream));
   }
   BT_CK(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int bt_get_check_samples(bt_ctx *h, int *ph_idx) {
+  if (!h || !ph_idx) return BT_ERR_ARG;
+  Ctx &c = h->c;
+  if (c.check_ns <= 0 || !c.d_check_idx) return fail(c, BT_ERR_STATE, "no opening check set");
+  const int ns = std::min(c.check_ns, c.n);
+  BT_CK(d2h(c, ph_idx, c.d_check_idx + (size_t)3 * ns, sizeof(int) * ns));
   return 0;
 }
 
@@ -1552,6 +1620,8 @@ int bt_step_graph(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit, in
     c.nleaf = nleaf;
     c.ncrit = ncrit;
     c.level_split = key.level_split;
+    if (join_check(c)) return BT_ERR_HIP;   // nothing forked may precede the capture
+    c.check_state = 0;
     BT_CK(hipStreamSynchronize(c.stream));
     const bool timing = c.timing;
     c.timing = false;
@@ -1565,10 +1635,10 @@ int bt_step_graph(bt_ctx *h, float dt, int nleaf, int level_split, int ncrit, in
         rc = BT_ERR_HIP;
         break;
       }
-      launch_kick(c, 0.5 * (double)dt);
-      launch_drift(c, dt);
+      launch_kick_drift(c, 0.5 * (double)dt, dt);
       rc = step_second_half(c, dt);   // the build swaps the buffers on the host side
       if (hipStreamEndCapture(c.stream, &c.graph[g]) != hipSuccess) rc = BT_ERR_HIP;
+      c.check_state = 0;   // captured work is not queued: nothing of it to join
       if (!rc && hipGraphInstantiate(&c.graph_exec[g], c.graph[g], nullptr, nullptr, 0) !=
                      hipSuccess)
         rc = BT_ERR_HIP;

@@ -441,19 +441,74 @@ __global__ void k_mlp_adam(float *__restrict__ p, float *__restrict__ m, float *
 
 // ---- per-step check of the learned walk -------------------------------------------
 
-// Relative force errors of the walk against the FP64 direct sum on ns
-// samples, then (one block) a sort: out = {p50, p99, max}.
-// Sample indices of a check: idx[k] = mix64(seed ^ (calls << 32) ^ k) mod n,
-// calls = the checks drawn before; one block.
+// Targets of a check (one block of 1024 threads, ns <= 1024): draw k is
+// idx[k] = mix64(seed ^ (calls << 32) ^ k) mod n, an index into the particle
+// arrays as they are when the draw runs (calls = the checks drawn before).
+// Their input ids, sorted with the draw slot as tie-break, go to
+// idx[ns + j] and the slots to idx[2 ns + j], for k_check_map.
 __global__ __launch_bounds__(1024) void k_check_idx(int ns, int n, unsigned long long seed,
-                                                    DevCounters *dc, int *__restrict__ idx) {
+                                                    DevCounters *dc, const int *__restrict__ id,
+                                                    int *__restrict__ idx) {
+  __shared__ int s_id[1024];
+  __shared__ int s_slot[1024];
   const unsigned long long calls = (unsigned long long)(unsigned)dc->check_calls;
-  for (int k = threadIdx.x; k < ns; k += blockDim.x)
-    idx[k] = (int)(mix64(seed ^ (calls << 32) ^ (unsigned long long)k) % (unsigned long long)n);
+  const int t = threadIdx.x;
+  int key = 0x7FFFFFFF;
+  if (t < ns) {
+    const int k =
+        (int)(mix64(seed ^ (calls << 32) ^ (unsigned long long)t) % (unsigned long long)n);
+    idx[t] = k;
+    key = id[k];
+  }
+  s_id[t] = key;
+  s_slot[t] = t;
   __syncthreads();
-  if (threadIdx.x == 0) dc->check_calls = (int)(calls + 1);
+  // bitonic sort of (id, slot) pairs, ascending; the slots make keys distinct
+  for (int k = 2; k <= 1024; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int p = t ^ j;
+      if (p > t) {
+        const int ai = s_id[t], bi = s_id[p], as = s_slot[t], bs = s_slot[p];
+        const bool gt = ai > bi || (ai == bi && as > bs);
+        if (gt == ((t & k) == 0)) {
+          s_id[t] = bi; s_id[p] = ai;
+          s_slot[t] = bs; s_slot[p] = as;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (t < ns) {
+    idx[ns + t] = s_id[t];
+    idx[2 * ns + t] = s_slot[t];
+  }
+  if (t == 0) dc->check_calls = (int)(calls + 1);
 }
 
+// PH index of each check target: particle p (PH order) looks its input id up
+// among the sorted target ids (binary search in LDS) and writes p into
+// idx[3 ns + slot] for every draw of that id.  ids are a permutation of
+// [0, n), so every slot is written exactly once.
+__global__ __launch_bounds__(kBlock) void k_check_map(int n, int ns, const int *__restrict__ id_ph,
+                                                      int *__restrict__ idx) {
+  __shared__ int s_id[1024];
+  for (int k = threadIdx.x; k < ns; k += kBlock) s_id[k] = idx[ns + k];
+  __syncthreads();
+  for (int p = blockIdx.x * kBlock + threadIdx.x; p < n; p += gridDim.x * kBlock) {
+    const int i = id_ph[p];
+    int lo = 0, hi = ns;
+    while (lo < hi) {
+      const int m = (lo + hi) >> 1;
+      if (s_id[m] < i) lo = m + 1;
+      else hi = m;
+    }
+    for (int q = lo; q < ns && s_id[q] == i; q++) idx[3 * ns + idx[2 * ns + q]] = p;
+  }
+}
+
+// Relative force errors of the walk against the FP64 direct sum on ns
+// samples (d_idx: their PH indices), then (one block) a sort: out = {p50,
+// p99, max}.
 __global__ __launch_bounds__(1024) void k_learn_check(int ns, const int *__restrict__ idx,
                                                       const double4 *__restrict__ d,
                                                       const float4 *__restrict__ acc,
@@ -587,7 +642,12 @@ void launch_mlp_adam(Ctx &c, float *p, float *m, float *v, const double *grad, f
 
 void launch_check_idx(Ctx &c, int ns) {
   hipLaunchKernelGGL(k_check_idx, dim3(1), dim3(1024), 0, c.stream, ns, c.n,
-                     (unsigned long long)c.check_seed, c.dc, c.d_check_idx);
+                     (unsigned long long)c.check_seed, c.dc, c.id, c.d_check_idx);
+}
+
+void launch_check_map(Ctx &c, int ns) {
+  hipLaunchKernelGGL(k_check_map, dim3(grid_for(c.n, kBlock, 2048)), dim3(kBlock), 0, c.stream,
+                     c.n, ns, c.id_ph, c.d_check_idx);
 }
 
 void launch_learn_check(Ctx &c, int ns, const int *d_idx, const double *d_direct, double *out) {

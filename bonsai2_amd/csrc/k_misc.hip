@@ -47,18 +47,30 @@ __global__ __launch_bounds__(kBlock) void k_kick(int n, const int *__restrict__ 
   }
 }
 
+// The opening half of a kick-drift-kick step in one pass: v += a dth for
+// the particle at tree index perm[k], then x += v dt for the same particle;
+// the arithmetic of a kick pass followed by a drift pass (so the same
+// results), with v kept in registers instead of written and read back.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_drift(int n, const V4<R> *__restrict__ vel,
-                                                  V4<R> *__restrict__ pos, R dt,
-                                                  const DevCounters *dc) {
+__global__ __launch_bounds__(kBlock) void k_kick_drift(int n, const int *__restrict__ perm,
+                                                       const V4<R> *__restrict__ acc,
+                                                       V4<R> *__restrict__ vel,
+                                                       V4<R> *__restrict__ pos, R dth, R dt,
+                                                       const DevCounters *dc) {
   if (dc->err_sticky) return;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const V4<R> v = vel[i];
-    V4<R> p = pos[i];
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+    const int t = perm[k];
+    const V4<R> a = acc[k];
+    V4<R> v = vel[t];
+    V4<R> p = pos[t];
+    v.x += a.x * dth;
+    v.y += a.y * dth;
+    v.z += a.z * dth;
+    vel[t] = v;
     p.x += v.x * dt;
     p.y += v.y * dt;
     p.z += v.z * dt;
-    pos[i] = p;
+    pos[t] = p;
   }
 }
 
@@ -216,14 +228,14 @@ void launch_kick(Ctx &c, double dth, bool closing) {
                        (float)dth, c.dc, closing ? 1 : 0);
 }
 
-void launch_drift(Ctx &c, double dt) {
+void launch_kick_drift(Ctx &c, double dth, double dt) {
   const dim3 g(grid_for(c.n, kBlock, 8192));
   if (c.fp64)
-    hipLaunchKernelGGL(k_drift<double>, g, dim3(kBlock), 0, c.stream, c.n, as4<double>(c.vel),
-                       as4<double>(c.pos), dt, c.dc);
+    hipLaunchKernelGGL(k_kick_drift<double>, g, dim3(kBlock), 0, c.stream, c.n, c.perm,
+                       as4<double>(c.acc), as4<double>(c.vel), as4<double>(c.pos), dth, dt, c.dc);
   else
-    hipLaunchKernelGGL(k_drift<float>, g, dim3(kBlock), 0, c.stream, c.n, c.vel, c.pos, (float)dt,
-                       c.dc);
+    hipLaunchKernelGGL(k_kick_drift<float>, g, dim3(kBlock), 0, c.stream, c.n, c.perm, c.acc,
+                       c.vel, c.pos, (float)dth, (float)dt, c.dc);
 }
 
 void launch_clear_sticky(Ctx &c) {
@@ -260,7 +272,8 @@ void launch_direct(Ctx &c, int ns, const int *d_idx, double *d_out, bool check) 
 // it is read (nothing needs clearing).  The sums of slices are added in slice
 // order, so equal slices give equal results.
 void launch_direct_parts(Ctx &c, int ns, const int *d_idx, double *d_out, double *d_part,
-                         int slices, bool check) {
+                         int slices, bool check, const void *src) {
+  if (!src) src = c.pos_ph;
   double4 *part = reinterpret_cast<double4 *>(d_part);
   // the softening the step uses: float(eps) squared in FP64 (Treecode<float>),
   // the FP64 eps squared (Treecode<double>)
@@ -274,7 +287,7 @@ void launch_direct_parts(Ctx &c, int ns, const int *d_idx, double *d_out, double
     if (soft) kern = forest ? k_direct<2, true, true, double> : k_direct<2, false, true, double>;
     else kern = forest ? k_direct<2, true, false, double> : k_direct<2, false, false, double>;
     hipLaunchKernelGGL(kern, grid, dim3(kBlock), 0, c.stream, ns, d_idx, c.n, c.nsys, c.d_off,
-                       as4<double>(c.pos_ph), eps2, part);
+                       static_cast<const double4 *>(src), eps2, part);
   } else {
     void (*kern)(int, const int *, int, int, const int *, const float4 *, double, double4 *);
     if (soft) {
@@ -285,7 +298,7 @@ void launch_direct_parts(Ctx &c, int ns, const int *d_idx, double *d_out, double
       else kern = check ? k_direct<1, false, false, float> : k_direct<2, false, false, float>;
     }
     hipLaunchKernelGGL(kern, grid, dim3(kBlock), 0, c.stream, ns, d_idx, c.n, c.nsys, c.d_off,
-                       c.pos_ph, eps2, part);
+                       static_cast<const float4 *>(src), eps2, part);
   }
   const int tpb = kBlock / kWave;   // targets per reduce block
   hipLaunchKernelGGL(k_direct_reduce, dim3((ns + tpb - 1) / tpb), dim3(kBlock), 0, c.stream, ns,

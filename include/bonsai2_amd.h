@@ -289,13 +289,20 @@ int bt_set_opening(bt_ctx *ctx, int mode, float budget, float theta_lo, float th
 /* After each learned-angle walk, nsamples (<= 1024; 0 = off) seeded particles
  * are re-checked against the FP64 direct sum; if their p99 relative force
  * error exceeds p99_max the forces are recomputed with the fixed angle.  The
- * decision is taken on the device (no host synchronisation); the samples of
- * check k are mix64(seed ^ (k << 32) ^ j) mod n, j < nsamples, k counted
- * from this call. */
+ * decision is taken on the device (no host synchronisation).  The samples of
+ * check k are the particles at positions mix64(seed ^ (k << 32) ^ j) mod n,
+ * j < nsamples, of the particle arrays when the check is drawn (k counted
+ * from this call): in bt_step and bt_step_graph the drifted particles before
+ * the step's build, whose FP64 sums then run on a third stream beside the
+ * build (bt_set_overlap(ctx, 0): before it, on the context stream); in
+ * bt_compute_forces the tree order, after the walk. */
 int bt_set_opening_check(bt_ctx *ctx, int nsamples, float p99_max, uint64_t seed);
 /* {p50, p99, max of the last check, last call fell back (0/1), fallbacks,
  * checks} */
 int bt_get_opening_check(bt_ctx *ctx, double out[6]);
+/* The PH indices of the last check's samples (min(nsamples, n) ints), in draw
+ * order; no reference counterpart (the check itself has none). */
+int bt_get_check_samples(bt_ctx *ctx, int *ph_idx);
 
 /* ---- Treecode<double> ----------------------------------------------------
  * The reference declares Particle4<double> (code/Particle4.h:106-125) but

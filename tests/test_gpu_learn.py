@@ -1,7 +1,8 @@
 """GPU tests of the learned opening angle against the oracle: the network's
 matrix-core inference and its gradient, the training-pair generator, the walk
 with per-pair angles, training on the GPU, the per-step FP64 check with its
-fixed-angle fallback, and what the trained network gains over fixed angles.
+fixed-angle fallback (and its statistics against the host's recomputation
+from its samples), and what the trained network gains over fixed angles.
 
 Bars: network outputs within 1e-3 absolute of the oracle (FP16 activations
 may round differently when the GPU's SiLU differs from the oracle's in the
@@ -271,3 +272,94 @@ def test_graph_step_with_check_matches_eager(bt, p99_max):
     finally:
         eager.close()
         graph.close()
+
+
+def check_stats_host(t, bt):
+    """The last check's p50/p99/max recomputed on the host from its samples'
+    PH indices: the walk's forces against bt_direct_fp64 on those particles,
+    relative errors rounded to FP32 and ranked as the device ranks them."""
+    idx = t.check_samples()
+    d = t.direct_fp64(idx)[:, :3]
+    a = t.acc(bt.ORDER_PH)[idx, :3].astype(np.float64)
+    e = np.sort((np.linalg.norm(a - d, axis=1) / np.linalg.norm(d, axis=1)).astype(np.float32))
+    ns = len(e)
+    return idx, dict(p50=float(e[(ns - 1) // 2]), p99=float(e[int(0.99 * (ns - 1))]),
+                     max=float(e[-1]))
+
+
+@pytest.mark.parametrize("overlap", [True, False], ids=["beside_build", "in_sequence"])
+def test_check_statistics_match_host(bt, overlap):
+    """The check's FP64 sums start from the drifted particles before the build
+    (on a third stream beside it, or in sequence), its targets are mapped to
+    PH indices by input id after the groups: its p50/p99/max equal the host's
+    recomputation from those PH indices (a wrong mapping would compare a
+    particle's force with another particle's direct sum), after a
+    computeForces, eager steps and a graph replay."""
+    from bonsai2_amd import learn
+    params, budget, lo, hi = learn.load()
+    n = 1 << 16
+    pos, vel = bt.plummer(n, seed=15)
+    t = bt.Treecode(0.01, 0.4, 64)
+    try:
+        t.set_overlap(overlap)
+        t.alloc(n)
+        t.ptcl_h2d(pos, vel)
+        t.buildTree(16)
+        t.computeMultipoles()
+        t.makeGroups(5, 64)
+        t.set_mlp_params(params)
+        t.set_opening(1, budget, lo, hi)
+        t.set_opening_check(256, 1.0, 9)
+        for tag in ("computeForces", "step 1", "step 2", "graph 1", "graph 2"):
+            if tag == "computeForces":
+                t.computeForces()
+            elif tag.startswith("graph"):
+                t.step_graph(1.0 / 128, 16, 5, 64, nsteps=1)
+            else:
+                t.step(1.0 / 128, 16, 5, 64)
+            chk = t.opening_check()
+            idx, host = check_stats_host(t, bt)
+            assert idx.min() >= 0 and idx.max() < n, tag
+            assert len(np.unique(idx)) >= 250, tag
+            assert 1e-6 < host["p50"] < 1e-2, (tag, host)
+            for k in ("p50", "p99", "max"):
+                assert abs(chk[k] - host[k]) <= 1e-6 * host[k], (tag, k, chk[k], host[k])
+            assert not chk["fell_back"], tag
+    finally:
+        t.close()
+
+
+def test_learned_steps_equal_with_and_without_overlap(bt):
+    """Learned steps with the check (its sums on a third stream, the groups on
+    a second) equal the same steps with every phase in sequence, bitwise:
+    particles, forces and check statistics."""
+    from bonsai2_amd import learn
+    params, budget, lo, hi = learn.load()
+    n = 1 << 16
+    pos, vel = bt.plummer(n, seed=16)
+    ts = []
+    try:
+        for overlap in (True, False):
+            t = bt.Treecode(0.01, 0.4, 64)
+            ts.append(t)
+            t.set_overlap(overlap)
+            t.alloc(n)
+            t.ptcl_h2d(pos, vel)
+            t.buildTree(16)
+            t.computeMultipoles()
+            t.makeGroups(5, 64)
+            t.set_mlp_params(params)
+            t.set_opening(1, budget, lo, hi)
+            t.set_opening_check(256, 1e-3, 9)
+            t.computeForces()
+        for step in range(3):
+            for t in ts:
+                t.step(1.0 / 128, 16, 5, 64)
+            a, b = ts
+            assert np.array_equal(a.ptcl_d2h(bt.ORDER_INPUT)[0], b.ptcl_d2h(bt.ORDER_INPUT)[0])
+            assert np.array_equal(a.acc(bt.ORDER_INPUT), b.acc(bt.ORDER_INPUT)), step
+            assert a.opening_check() == b.opening_check(), step
+            assert np.array_equal(a.check_samples(), b.check_samples()), step
+    finally:
+        for t in ts:
+            t.close()
