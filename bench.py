@@ -436,7 +436,8 @@ def main():
         t.step(args.dt, args.nleaf, args.level_split, args.ncrit)
 
     walk_ms, phase = [], []
-    flops, inter = [], []
+    flops, inter, fell = [], [], []
+    chk_a = t.opening_check() if learned else None
     t0 = time.perf_counter()
     for _ in range(args.steps):
         t.step(args.dt, args.nleaf, args.level_split, args.ncrit)  # synchronises at the end
@@ -446,13 +447,19 @@ def main():
         _, tot = t.walk_stats()
         flops.append(20.0 * tot[0] + flops_p2c * tot[1])
         inter.append(tot[0] + tot[1])
+        # after a fallback the statistics are the fixed-angle walk's, while the
+        # walk's events time the learned walk: such a step stays out of the rates
+        fell.append(bool(t.opening_check()["fell_back"]) if learned else False)
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / args.steps
+    chk_b = t.opening_check() if learned else None
     # the same step replayed from its HIP graph (one host synchronisation)
     t.step_graph(args.dt, args.nleaf, args.level_split, args.ncrit, nsteps=1)
+    chk_g = t.opening_check() if learned else None
     tg0 = time.perf_counter()
     t.step_graph(args.dt, args.nleaf, args.level_split, args.ncrit, nsteps=args.steps)
     ms_graph = (time.perf_counter() - tg0) * 1e3 / args.steps
+    chk_c = t.opening_check() if learned else None
     log("headline: %.2f ms/step eager, %.2f ms/step graph" % (ms_step, ms_graph))
     # each phase alone (outside the timed region): the timed steps run the
     # groups beside the level build and multipoles, which shares the GPU
@@ -464,9 +471,12 @@ def main():
         phase_serial.append(t.timings())
     t.set_overlap(True)
 
-    walk_ms = np.array(walk_ms)
-    flops = np.array(flops)
-    inter = np.array(inter)
+    keep = ~np.array(fell, bool)
+    if not keep.any():
+        keep[:] = True
+    walk_ms = np.array(walk_ms)[keep]
+    flops = np.array(flops)[keep]
+    inter = np.array(inter)[keep]
     achieved = float((flops / (walk_ms * 1e-3)).mean() / 1e12)
     chk = t.opening_check() if learned else None
     # accuracy against the FP64 direct sum on seeded samples (outside the timed region)
@@ -587,12 +597,19 @@ def main():
                      "traffic": None,
                      "kernel": "k_walk<learned>" if learned else "k_walk",
                      "flops_per_launch": float(flops.mean()),
-                     "avg_launch_ms": float(walk_ms.mean())},
+                     "avg_launch_ms": float(walk_ms.mean()),
+                     "launches": int(keep.sum())},
     }
     st = st_main
     out["interactions_per_particle"] = {"p2p": st[0], "p2c": st[2], "p2p_max": st[1],
                                         "p2c_max": st[3]}
     if learned:
+        # checks and fallbacks to the fixed angle inside the timed eager steps
+        # and inside the graph replays (a fallback adds a fixed-angle walk)
+        chk["timed_steps"] = {"checks": chk_b["checks"] - chk_a["checks"],
+                              "fallbacks": chk_b["fallbacks"] - chk_a["fallbacks"]}
+        chk["graph_steps"] = {"checks": chk_c["checks"] - chk_g["checks"],
+                              "fallbacks": chk_c["fallbacks"] - chk_g["fallbacks"]}
         out["opening_check"] = chk
         out["fixed_theta"] = compare
     if not args.no_configs:

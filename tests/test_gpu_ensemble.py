@@ -217,3 +217,52 @@ def test_graph_step_with_learned_check_in_a_forest(bt):
     finally:
         eager.close()
         graph.close()
+
+
+def test_full_ensemble_matches_systems_alone(bt):
+    """The 256-system ensemble at full size (N_k = 2^U[10, 16], 3.87 M
+    particles, theta 0.5, eps 0.01) in one context: the smallest, a middle
+    and the largest system equal the same system run alone in a context of
+    its own: the same groups with the same per-group interaction counts, and
+    forces within 1e-6 relative RMS."""
+    pos, vel, off = bt.ensemble(256, seed=19810614)
+    n = int(off[-1])
+    sizes = np.diff(off)
+    t = bt.Treecode(EPS, THETA, 64)
+    try:
+        t.alloc(n, off)
+        t.ptcl_h2d(pos, vel)
+        t.buildTree(16)
+        t.computeMultipoles()
+        t.makeGroups(5, 64)
+        t.computeForces()
+        acc = t.acc(bt.ORDER_INPUT)
+        cnt = t.counts()
+        groups = t.groups()
+        gcnt = t.group_counts()
+    finally:
+        t.close()
+    assert cnt["nsys"] == 256 and n > 3_000_000
+    order = np.argsort(sizes, kind="stable")
+    picks = [int(order[0]), int(order[128]), int(order[-1])]
+    for k in picks:
+        s = bt.Treecode(EPS, THETA, 64)
+        try:
+            s.alloc(int(sizes[k]))
+            s.ptcl_h2d(pos[off[k]:off[k + 1]], vel[off[k]:off[k + 1]])
+            s.buildTree(16)
+            s.computeMultipoles()
+            s.makeGroups(5, 64)
+            s.computeForces()
+            a_k = s.acc(bt.ORDER_INPUT)
+            g_k = s.groups()
+            c_k = s.group_counts()
+        finally:
+            s.close()
+        mine = (groups[:, 0] >= off[k]) & (groups[:, 0] < off[k + 1])
+        assert np.array_equal(groups[mine, 0] - off[k], g_k[:, 0]), k
+        assert np.array_equal(groups[mine, 1], g_k[:, 1]), k
+        assert np.array_equal(gcnt[mine], c_k), k
+        print("system %d (n=%d): rel RMS %.2e" % (k, sizes[k],
+                                                 rel_rms(acc[off[k]:off[k + 1], :3], a_k[:, :3])))
+        assert rel_rms(acc[off[k]:off[k + 1], :3], a_k[:, :3]) <= 1e-6, k

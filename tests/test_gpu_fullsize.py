@@ -1,11 +1,14 @@
-"""GPU checks at the benchmark sizes (N = 2^22 Plummer, N = 2^24 halo+disk),
-where the CPU oracle would take too long: size-independent properties of
-every stage.  Tree: the invariants of SWtest/treebuild.cu:1477 (leaf counts
-sum to N, leaves <= NLEAF, nodes > NLEAF); PH keys sorted and the permutation
-a bijection; groups tile [0, N) with <= NCRIT each; root moments equal the FP64
-sums over all particles; forces against the FP64 direct sum on seeded samples;
-two force evaluations bit-identical; total momentum of the forces small; and,
-for 2^22, energy conserved over 8 leapfrog steps."""
+"""GPU checks at the benchmark sizes (N = 2^22 Plummer, N = 2^24 halo+disk,
+N = 2^24 clustered), where the CPU oracle would take too long:
+size-independent properties of every stage.  Tree: the invariants of
+SWtest/treebuild.cu:1477 (leaf counts sum to N, leaves <= NLEAF, nodes >
+NLEAF); PH keys sorted and the permutation a bijection; groups tile [0, N)
+with <= NCRIT each; root moments equal the FP64 sums over all particles;
+forces against the FP64 direct sum on seeded samples; two force evaluations
+bit-identical; total momentum of the forces small; energy conserved over 8
+leapfrog steps (Plummer and halo+disk); and, on the deep clustered tree,
+compensated against plain FP32 accumulation over the same interaction
+lists."""
 import numpy as np
 import pytest
 
@@ -15,14 +18,31 @@ CASES = [
     # name, n, ic, theta, eps, nleaf, ncrit, p99 bound
     ("plummer_2p22", 1 << 22, "plummer", 0.5, 0.005, 16, 64, 1e-3),
     ("halo_disk_2p24", 1 << 24, "halo_disk", 0.4, 0.002, 16, 64, 1e-3),
+    ("clustered_2p24", 1 << 24, "clustered", 0.5, 0.002, 16, 64, 1e-3),
 ]
+
+
+def rel_rms(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def p99_vs_direct(t, n, seed=3, ns=1024):
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(n, size=ns, replace=False)).astype(np.int32)
+    d = t.direct_fp64(idx)
+    a = t.acc(2)[idx, :3].astype(np.float64)
+    e = np.linalg.norm(a - d[:, :3], axis=1) / np.linalg.norm(d[:, :3], axis=1)
+    return np.percentile(e, [50, 99])
 
 
 @pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])
 def big(request, bt):
     name, n, ic, theta, eps, nleaf, ncrit, p99 = request.param
     print("case", name, flush=True)
-    pos, vel = bt.plummer(n) if ic == "plummer" else bt.halo_disk(n)
+    make = {"plummer": bt.plummer, "halo_disk": bt.halo_disk, "clustered": bt.clustered}[ic]
+    pos, vel = make(n)
     t = bt.Treecode(eps, theta, ncrit)
     t.alloc(n)
     t.ptcl_h2d(pos, vel)
@@ -68,15 +88,36 @@ def test_ph_order_and_groups_full_size(big):
 
 
 def test_forces_against_fp64_direct_full_size(big):
-    t = big["t"]
-    rng = np.random.default_rng(3)
-    idx = np.sort(rng.choice(big["n"], size=1024, replace=False)).astype(np.int32)
-    d = t.direct_fp64(idx)
-    a = t.acc(2)[idx, :3].astype(np.float64)
-    e = np.linalg.norm(a - d[:, :3], axis=1) / np.linalg.norm(d[:, :3], axis=1)
-    p50, p99 = np.percentile(e, [50, 99])
+    p50, p99 = p99_vs_direct(big["t"], big["n"])
     print("%s p50 %.2e p99 %.2e" % (big["name"], p50, p99))
     assert p99 <= big["p99"]
+
+
+def test_compensated_and_plain_sums_full_size(big):
+    """The clustered configuration's comparison: plain FP32 accumulation
+    (kahan = 0) walks the same lists (equal interaction statistics), its
+    forces stay within 1e-5 relative RMS of the compensated ones, and both
+    meet the p99 bound against the FP64 direct sum."""
+    if big["name"] != "clustered_2p24":
+        pytest.skip("compensated against plain sums: the clustered case")
+    t = big["t"]
+    st_k = t.computeForces()
+    a_k = t.acc(2).copy()
+    k50, k99 = p99_vs_direct(t, big["n"], seed=5)
+    t.set_walk_options(kahan=0)
+    try:
+        st_p = t.computeForces()
+        a_p = t.acc(2)
+        p50, p99 = p99_vs_direct(t, big["n"], seed=5)
+    finally:
+        t.set_walk_options(kahan=1)
+        t.computeForces()   # leave the compensated forces for the tests that follow
+    d = rel_rms(a_p[:, :3], a_k[:, :3])
+    print("compensated p50 %.4e p99 %.4e; plain p50 %.4e p99 %.4e; rel RMS difference %.2e; "
+          "levels %d" % (k50, k99, p50, p99, d, t.counts()["nlevels"]))
+    assert tuple(st_p) == tuple(st_k)
+    assert 0.0 < d <= 1e-5
+    assert k99 <= big["p99"] and p99 <= big["p99"]
 
 
 def test_forces_deterministic_and_momentum_full_size(big):
@@ -94,8 +135,8 @@ def test_forces_deterministic_and_momentum_full_size(big):
 
 
 def test_energy_conserved_over_steps(big):
-    if big["n"] > (1 << 22):
-        pytest.skip("energy run at 2^22 only")
+    if big["name"] == "clustered_2p24":
+        pytest.skip("the clustered configuration specifies no integration")
     t = big["t"]
     ek0, ep0 = t.computeEnergies()
     e0 = ek0.sum() + ep0.sum()
