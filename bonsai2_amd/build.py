@@ -25,6 +25,8 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
              "-Wno-unused-function", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(HERE, "..", "include")]
+# BONSAI_HIPCC_EXTRA: further device-compiler options (e.g. -DBT_BUILD_WAVE_PRIO=0)
+HIP_FLAGS += os.environ.get("BONSAI_HIPCC_EXTRA", "").split()
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(HERE, "..", "include")]
 # what `hipconfig --cpp_config` prints: HIP's vector types (double4, ...) for
 # host-compiled code that includes include/Treecode.h

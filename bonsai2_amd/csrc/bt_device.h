@@ -8,6 +8,16 @@ namespace bt {
 constexpr int kWave = 64;
 constexpr int kMaxLevel = 21;  // octant digits in a 63-bit geometric key
 
+// The kernels of the build (box, keys, sorts, gathers, levels, multipoles,
+// groups) raise their waves' issue priority above the default 0: during a
+// step the learned walk's FP64 check sum runs beside them on another stream,
+// and on a SIMD they share, the build's latency-bound waves then issue ahead
+// of its arithmetic.
+#ifndef BT_BUILD_WAVE_PRIO
+#define BT_BUILD_WAVE_PRIO 1
+#endif
+#define BT_BUILD_PRIO() __builtin_amdgcn_s_setprio(BT_BUILD_WAVE_PRIO)
+
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
 __device__ __forceinline__ uint64_t ballot64(bool p) { return (uint64_t)__ballot(p); }

@@ -53,6 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_ph_keys(int n, int nsys, const int *
                                                     const V4<R> *__restrict__ pos,
                                                     uint64_t *__restrict__ keys,
                                                     int *__restrict__ vals) {
+  BT_BUILD_PRIO();
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const int r = nsys > 1 ? system_of(i, nsys, off) : 0;
     const V4<R> b = box[r];
@@ -79,6 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_ph(int n, const int *__restri
                                                       const int *__restrict__ id,
                                                       T *__restrict__ pos_ph,
                                                       int *__restrict__ id_ph) {
+  BT_BUILD_PRIO();
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const int j = perm[i];
     pos_ph[i] = pos[j];
@@ -90,6 +92,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_ph(int n, const int *__restri
 __global__ __launch_bounds__(kBlock) void k_run_starts(int n, int nsys, const int *__restrict__ off,
                                                        const uint64_t *__restrict__ keys,
                                                        uint64_t mask, int *__restrict__ v) {
+  BT_BUILD_PRIO();
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     bool start = i == 0;
     if (!start) {
@@ -103,6 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_run_starts(int n, int nsys, const in
 __global__ __launch_bounds__(kBlock) void k_group_flags(int n, int ncrit,
                                                         const int *__restrict__ run_start,
                                                         int *__restrict__ flag) {
+  BT_BUILD_PRIO();
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
     flag[i] = ((i - run_start[i]) % ncrit) == 0 ? 1 : 0;
 }
@@ -113,6 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_group_emit(int n, int nsys, const in
                                                        int cap, int2 *__restrict__ groups,
                                                        int *__restrict__ group_sys,
                                                        DevCounters *dc) {
+  BT_BUILD_PRIO();
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     if (!flag[i]) continue;
     const int g = slot[i];
@@ -128,6 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_group_emit(int n, int nsys, const in
 // Groups tile [0, n) in order, so a group ends where the next begins.
 __global__ __launch_bounds__(kBlock) void k_group_sizes(int n, int cap, int2 *__restrict__ groups,
                                                         DevCounters *dc) {
+  BT_BUILD_PRIO();
   const int ng = min(dc->ngroups, cap);
   for (int g = blockIdx.x * kBlock + threadIdx.x; g < ng; g += gridDim.x * kBlock) {
     const int next = g + 1 < ng ? groups[g + 1].x : n;

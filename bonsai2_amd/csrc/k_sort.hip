@@ -89,6 +89,7 @@ __global__ __launch_bounds__(kCThreads) void k_radix_hist(const uint64_t *__rest
                                                           const int *__restrict__ vals,
                                                           int n, DigitSrc src, int nblocks,
                                                           int *__restrict__ ghist) {
+  BT_BUILD_PRIO();
   __shared__ int hist[256];
   const int tid = threadIdx.x;
   if (tid < 256) hist[tid] = 0;
@@ -115,6 +116,7 @@ __global__ __launch_bounds__(kCThreads) void k_radix_scatter(
     const uint64_t *__restrict__ keys, const int *__restrict__ vals, int n, DigitSrc src,
     int nblocks, const int *__restrict__ goff, uint64_t *__restrict__ keys_out,
     int *__restrict__ vals_out) {
+  BT_BUILD_PRIO();
   __shared__ uint64_t skeys[kSortTile];   // the tile in sorted order
   __shared__ int svals[kSortTile];
   __shared__ int tstart[256];     // first tile-local position of each digit
@@ -453,6 +455,7 @@ __global__ void k_sort_clear(unsigned int *status, int nwords, int *ctr) {
 template <class Op>
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int *__restrict__ in, int n,
                                                               int *__restrict__ blk) {
+  BT_BUILD_PRIO();
   __shared__ int shw[4];
   const int base = blockIdx.x * kScanTile;
   int v = Op::id();
@@ -470,6 +473,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int *__restr
 template <class Op>
 __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(int *__restrict__ blk, int nb,
                                                               int *__restrict__ total_dev) {
+  BT_BUILD_PRIO();
   __shared__ int shw[4];
   int carry = Op::id();
   for (int b0 = 0; b0 < nb; b0 += kScanThreads) {
@@ -496,6 +500,7 @@ template <class Op, bool kInclusive>
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(const int *__restrict__ in, int n,
                                                             const int *__restrict__ blk,
                                                             int *__restrict__ out) {
+  BT_BUILD_PRIO();
   __shared__ int sh[kScanTile];
   __shared__ int shw[4];
   const int base = blockIdx.x * kScanTile;

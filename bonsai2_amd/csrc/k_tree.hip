@@ -78,6 +78,7 @@ template <typename R>
 __global__ __launch_bounds__(kBlock) void k_bbox_partial(const V4<R> *__restrict__ pos,
                                                          const int *__restrict__ off, int nb,
                                                          V4<R> *__restrict__ part) {
+  BT_BUILD_PRIO();
   __shared__ Ext<R> sh[kBlock / kWave];
   const int r = blockIdx.y;
   const int b = off[r], e = off[r + 1];
@@ -145,6 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_geo_keys(int n, int nsys, const int 
                                                      const V4<R> *__restrict__ pos,
                                                      uint64_t *__restrict__ keys,
                                                      int *__restrict__ vals) {
+  BT_BUILD_PRIO();
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const int r = nsys > 1 ? system_of(i, nsys, off) : 0;
     const V4<R> b = box[r];
@@ -171,6 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_gather3(int n, const int *__restrict
                                                     T *__restrict__ a2,
                                                     T *__restrict__ b2,
                                                     int *__restrict__ c2) {
+  BT_BUILD_PRIO();
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const int j = perm[i];
     a2[i] = a[j];
@@ -241,6 +244,7 @@ __device__ __forceinline__ int child_count(const LevelArgs &a, int p, bool activ
 }
 
 __global__ __launch_bounds__(kBlock) void k_level_count(LevelArgs a) {
+  BT_BUILD_PRIO();
   __shared__ int shw[4];
   const int np = level_parent_count(a);
   const int chunk = (np + kLevelBlocks - 1) / kLevelBlocks;
@@ -262,6 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_level_count(LevelArgs a) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_level_scan(LevelArgs a) {
+  BT_BUILD_PRIO();
   constexpr int kItems = (kLevelBlocks + kBlock - 1) / kBlock;
   __shared__ int shw[kBlock / kWave];
   __shared__ int s_run;
@@ -305,6 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_level_scan(LevelArgs a) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_level_emit(LevelArgs a) {
+  BT_BUILD_PRIO();
   __shared__ int shw[4];
   __shared__ int sh_nodes, sh_leaves;
   DevCounters *dc = a.dc;
@@ -510,6 +516,7 @@ __device__ void node_moments(const MpArgs<T> &a, int c, const uint4 cd) {
 // kFp64Levels and in FP64 at and above it.  Treecode<double>: FP64 throughout.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_multipole_level(MpArgs<T> a) {
+  BT_BUILD_PRIO();
   const int cnt = a.dc->lvl_count[a.L];
   const int c0 = a.dc->lvl_start[a.L];
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < cnt; t += gridDim.x * kBlock) {
