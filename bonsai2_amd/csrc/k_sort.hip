@@ -37,6 +37,9 @@ static_assert(kCThreads * kCRounds == kSortTile, "tile of the three-launch passe
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
+// up to this many tiles a scan takes two launches (the down-sweep reduces the
+// tile totals before it, at most 1024 ints per block); beyond, three
+constexpr int kScanFusedTiles = 1024;
 
 struct DigitSrc {
   int shift;        // bit offset of the digit
@@ -496,14 +499,27 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(int *__restrict__ 
   if (threadIdx.x == 0 && total_dev) *total_dev = carry;
 }
 
-template <class Op, bool kInclusive>
+// kPrefix: blk holds the tile totals and each block reduces those of the
+// tiles before it itself (for few tiles: one launch instead of a separate
+// scan of the totals); the last block writes the grand total to total_dev.
+// Otherwise blk holds the tiles' exclusive prefixes (k_scan_blocks).
+template <class Op, bool kInclusive, bool kPrefix>
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(const int *__restrict__ in, int n,
                                                             const int *__restrict__ blk,
-                                                            int *__restrict__ out) {
+                                                            int *__restrict__ out,
+                                                            int *__restrict__ total_dev) {
   BT_BUILD_PRIO();
   __shared__ int sh[kScanTile];
   __shared__ int shw[4];
   const int base = blockIdx.x * kScanTile;
+  int prefix;
+  if constexpr (kPrefix) {
+    int v = Op::id();
+    for (int k = threadIdx.x; k < (int)blockIdx.x; k += kScanThreads) v = Op::op(v, blk[k]);
+    (void)block_incl_scan<Op>(v, shw, &prefix);   // prefix = the reduction over the block
+  } else {
+    prefix = blk[blockIdx.x];
+  }
 #pragma unroll
   for (int k = 0; k < kScanItems; k++) {
     const int i = base + k * kScanThreads + threadIdx.x;
@@ -526,13 +542,15 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const int *__restric
     const int w = threadIdx.x >> 6;
     for (int k = 0; k < w; k++) exc = Op::op(exc, shw[k]);
   }
-  int acc = Op::op(blk[blockIdx.x], exc);
+  int acc = Op::op(prefix, exc);
 #pragma unroll
   for (int k = 0; k < kScanItems; k++) {
     const int nv = Op::op(acc, vals[k]);
     sh[threadIdx.x * kScanItems + k] = kInclusive ? nv : acc;
     acc = nv;
   }
+  if (kPrefix && total_dev && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1)
+    *total_dev = Op::op(prefix, total);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kScanItems; k++) {
@@ -565,10 +583,15 @@ void scan_impl(Ctx &c, const int *in, int *out, int n, int *total_dev) {
   }
   hipLaunchKernelGGL((k_scan_reduce<Op>), dim3(nb), dim3(kScanThreads), 0, c.stream, in, n,
                      c.scan_blk);
+  if (nb <= kScanFusedTiles) {
+    hipLaunchKernelGGL((k_scan_down<Op, kInclusive, true>), dim3(nb), dim3(kScanThreads), 0,
+                       c.stream, in, n, c.scan_blk, out, total_dev);
+    return;
+  }
   hipLaunchKernelGGL((k_scan_blocks<Op>), dim3(1), dim3(kScanThreads), 0, c.stream, c.scan_blk,
                      nb, total_dev);
-  hipLaunchKernelGGL((k_scan_down<Op, kInclusive>), dim3(nb), dim3(kScanThreads), 0, c.stream,
-                     in, n, c.scan_blk, out);
+  hipLaunchKernelGGL((k_scan_down<Op, kInclusive, false>), dim3(nb), dim3(kScanThreads), 0,
+                     c.stream, in, n, c.scan_blk, out, nullptr);
 }
 
 template <typename T>
