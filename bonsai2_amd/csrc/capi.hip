@@ -106,6 +106,15 @@ int alloc_cells(Ctx &c, int cap) {
   return rc ? BT_ERR_HIP : 0;
 }
 
+// Waits for all of the context's work, including work a step forked onto its
+// second and third streams (joined into the context stream unless the call
+// that queued it failed), before buffers are freed or replaced.
+void sync_streams(Ctx &c) {
+  if (c.stream2) (void)hipStreamSynchronize(c.stream2);
+  if (c.stream3) (void)hipStreamSynchronize(c.stream3);
+  (void)hipStreamSynchronize(c.stream);
+}
+
 void release_graph(Ctx &c) {
   for (int k = 0; k < 2; k++) {
     if (c.graph_exec[k]) (void)hipGraphExecDestroy(c.graph_exec[k]);
@@ -297,6 +306,7 @@ int queue_check_direct(Ctx &c, bool fork) {
     const hipError_t e = hipEventRecord(c.chk_join_ev, c.stream);
     std::swap(c.stream, c.stream3);
     if (e != hipSuccess) {
+      (void)hipStreamSynchronize(c.stream3);   // nothing left running unjoined
       c.err = std::string("hipEventRecord: ") + hipGetErrorString(e);
       return BT_ERR_HIP;
     }
@@ -337,7 +347,7 @@ int regrow_cells(Ctx &c) {
   const long long cap = 3LL * c.n + 64LL * c.nsys + 1024;
   if (cap > 0x1FFFFFFF || cap <= c.cap_cells)
     return fail(c, BT_ERR_CAPACITY, "cell capacity exceeded");
-  (void)hipStreamSynchronize(c.stream);
+  sync_streams(c);
   release_graph(c);
   free_cell_arrays(c);
   if (alloc_cells(c, (int)cap)) return BT_ERR_HIP;
@@ -537,9 +547,7 @@ int bt_create_f64(bt_ctx **out, double eps, double theta, int ncrit) {
 int bt_destroy(bt_ctx *h) {
   if (!h) return 0;
   Ctx &c = h->c;
-  if (c.stream2) (void)hipStreamSynchronize(c.stream2);
-  if (c.stream3) (void)hipStreamSynchronize(c.stream3);
-  (void)hipStreamSynchronize(c.stream);
+  sync_streams(c);
   release_graph(c);
   free_particle_arrays(c);
   free_cell_arrays(c);
@@ -574,7 +582,7 @@ const char *bt_last_error(const bt_ctx *h) { return h ? h->c.err.c_str() : "null
 int bt_set_stream(bt_ctx *h, void *s) {
   if (!h) return BT_ERR_ARG;
   Ctx &c = h->c;
-  (void)hipStreamSynchronize(c.stream);
+  sync_streams(c);
   if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
   c.own_stream = false;
   c.stream = (hipStream_t)s;
@@ -672,7 +680,7 @@ int bt_alloc(bt_ctx *h, int n, int nsys, const int *offsets) {
     off[0] = 0;
     off[1] = n;
   }
-  (void)hipStreamSynchronize(c.stream);
+  sync_streams(c);
   release_graph(c);
   const bool regrow = n > c.cap_n || nsys != c.nsys || !c.pos;
   c.n = n;
@@ -1531,9 +1539,11 @@ int bt_set_opening_check(bt_ctx *h, int nsamples, float p99_max, uint64_t seed) 
   Ctx &c = h->c;
   if (nsamples < 0 || nsamples > 1024 || !(p99_max > 0.f))
     return fail(c, BT_ERR_ARG, "samples must be in [0, 1024], p99_max > 0");
+  // no check sum of an earlier call may still run on the check's stream
+  sync_streams(c);
+  c.check_state = 0;
   if (nsamples > c.check_cap) {
     // captured steps hold the old buffers: drop them before the buffers go
-    (void)hipStreamSynchronize(c.stream);
     release_graph(c);
     c.check_cap = 0;
     if (dev_alloc(c, &c.d_check, (size_t)nsamples * (direct_slices(nsamples) + 1) * 4 + 4) ||
