@@ -204,6 +204,18 @@ def config_c1(bt):
     r.update(p50_rel_err=p50, p99_rel_err=p99, after_step={"p50_rel_err": q50, "p99_rel_err": q99,
              "dt": 1.0 / 64}, error_samples=w["n"], step_ms=sum(t.timings().values()),
              rel_energy_change=float(abs((ek1 + ep1).sum() - (ek0 + ep0).sum()) / abs((ek0 + ep0).sum())))
+    # wall clock per step at this size (about 300 small launches per step):
+    # eager steps, then the same step replayed from its HIP graph
+    t.step(1.0 / 64, 16, 5, 64)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        t.step(1.0 / 64, 16, 5, 64)
+    eager = (time.perf_counter() - t0) * 1e3 / 10
+    t.step_graph(1.0 / 64, 16, 5, 64, nsteps=1)
+    t0 = time.perf_counter()
+    t.step_graph(1.0 / 64, 16, 5, 64, nsteps=10)
+    graph = (time.perf_counter() - t0) * 1e3 / 10
+    r["wall_ms_per_step"] = {"eager": eager, "graph": graph}
     t.close()
     return r
 
