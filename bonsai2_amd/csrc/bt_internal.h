@@ -195,7 +195,7 @@ struct Ctx {
   hipStream_t stream3 = nullptr;
   hipEvent_t chk_fork_ev = nullptr, chk_join_ev = nullptr;
   int check_state = 0;
-  int check_slices = 512;
+  int check_slices = 768;
 
   // energies / direct sum scratch
   double *d_energy = nullptr;  // 2 * nsys

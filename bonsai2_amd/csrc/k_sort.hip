@@ -120,16 +120,19 @@ __global__ __launch_bounds__(kCThreads) void k_radix_scatter(
     int nblocks, const int *__restrict__ goff, uint64_t *__restrict__ keys_out,
     int *__restrict__ vals_out) {
   BT_BUILD_PRIO();
+  // LDS kept under a third of a CU's 160 KiB, so three blocks share a CU:
+  // 16-bit per-wave digit counters (a tile holds 4096 pairs) and, per digit,
+  // only (global base - tile-local start)
   __shared__ uint64_t skeys[kSortTile];   // the tile in sorted order
   __shared__ int svals[kSortTile];
-  __shared__ int tstart[256];     // first tile-local position of each digit
-  __shared__ int gbase[256];      // global position of this tile's first pair of each digit
-  __shared__ int wcnt[kCWaves][256];   // per-wave digit counts -> per-wave digit bases
+  __shared__ int gdelta[256];     // global position of digit d's pairs minus their tile start
+  __shared__ unsigned short wcnt[kCWaves][256];   // per-wave digit counts -> digit bases
   __shared__ int sh_wave[kCWaves];
   const int tid = threadIdx.x;
   const int w = tid >> 6;
   const int lane = lane_id();
   const int hist_n = 256 * nblocks;
+  int tstart = 0;   // thread d < 256: first tile-local position of digit d
   {
     // thread d < 256: this tile's count of digit d from the scanned histogram
     // (digit-major: the next entry is the same digit's next tile, or the next
@@ -144,8 +147,8 @@ __global__ __launch_bounds__(kCThreads) void k_radix_scatter(
     int tot;
     const int incl = block_incl_scan<OpSum>(cnt, sh_wave, &tot);
     if (tid < 256) {
-      tstart[tid] = incl - cnt;
-      gbase[tid] = g0;
+      tstart = incl - cnt;
+      gdelta[tid] = g0 - tstart;
     }
   }
   for (int k = tid; k < kCWaves * 256; k += kCThreads) (&wcnt[0][0])[k] = 0;
@@ -179,7 +182,7 @@ __global__ __launch_bounds__(kCThreads) void k_radix_scatter(
     const int rank = __popcll(peers & lanemask_lt());
     const int before = wcnt[w][d];
     wave_lds_sync();
-    if (valid && rank == 0) wcnt[w][d] = before + __popcll(peers);
+    if (valid && rank == 0) wcnt[w][d] = (unsigned short)(before + __popcll(peers));
     wave_lds_sync();
     dd[r] = valid ? d : -1;
     rr[r] = before + rank;
@@ -187,11 +190,11 @@ __global__ __launch_bounds__(kCThreads) void k_radix_scatter(
   __syncthreads();
   if (tid < 256) {
     // digit tid: the waves' pairs follow each other in tile order
-    int acc = tstart[tid];
+    int acc = tstart;
 #pragma unroll
     for (int k = 0; k < kCWaves; k++) {
       const int c = wcnt[k][tid];
-      wcnt[k][tid] = acc;
+      wcnt[k][tid] = (unsigned short)acc;
       acc += c;
     }
   }
@@ -211,7 +214,7 @@ __global__ __launch_bounds__(kCThreads) void k_radix_scatter(
     const uint64_t key = skeys[j];
     const int val = svals[j];
     const int d = digit_of<BYSYS>(key, val, src);
-    const int dst = gbase[d] + (j - tstart[d]);
+    const int dst = gdelta[d] + j;
     keys_out[dst] = key;
     vals_out[dst] = val;
   }
