@@ -448,7 +448,7 @@ def main():
         t.step(args.dt, args.nleaf, args.level_split, args.ncrit)
 
     walk_ms, phase = [], []
-    flops, inter, fell = [], [], []
+    flops, inter = [], []
     chk_a = t.opening_check() if learned else None
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -459,12 +459,16 @@ def main():
         _, tot = t.walk_stats()
         flops.append(20.0 * tot[0] + flops_p2c * tot[1])
         inter.append(tot[0] + tot[1])
-        # after a fallback the statistics are the fixed-angle walk's, while the
-        # walk's events time the learned walk: such a step stays out of the rates
-        fell.append(bool(t.opening_check()["fell_back"]) if learned else False)
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / args.steps
     chk_b = t.opening_check() if learned else None
+    # after a fallback the statistics are the fixed-angle walk's (several times
+    # the learned walk's interactions) while the walk's events time the learned
+    # walk: such a step stays out of the rates
+    fell = [False] * len(flops)
+    if learned and chk_b["fallbacks"] > chk_a["fallbacks"]:
+        med = float(np.median(flops))
+        fell = [f > 1.5 * med for f in flops]
     # the same step replayed from its HIP graph (one host synchronisation)
     t.step_graph(args.dt, args.nleaf, args.level_split, args.ncrit, nsteps=1)
     chk_g = t.opening_check() if learned else None
