@@ -168,6 +168,10 @@ struct Ctx {
   int quadrupole = 1;          // 0: monopole-only P2C (the reference without QUADRUPOLE)
   int num_cus = 256;
   unsigned long long *d_wstats = nullptr;  // direct_sum, direct_max, approx_sum, approx_max
+  // host copy of d_wstats fetched with the device counters; valid until the
+  // next walk is queued
+  unsigned long long h_wstats[4] = {};
+  bool h_wstats_valid = false;
   double *d_wpot = nullptr;
 
   // learned opening angle

@@ -158,7 +158,11 @@ hipError_t d2h(Ctx &c, void *dst, const void *src, size_t bytes) {
 
 int read_counters(Ctx &c) {
   BT_CK(hipMemcpyAsync(&c.hc, c.dc, sizeof(DevCounters), hipMemcpyDeviceToHost, c.stream));
+  if (c.d_wstats)   // the last walk's statistics ride along (one synchronisation)
+    BT_CK(hipMemcpyAsync(c.h_wstats, c.d_wstats, sizeof(c.h_wstats), hipMemcpyDeviceToHost,
+                         c.stream));
   BT_CK(hipStreamSynchronize(c.stream));
+  c.h_wstats_valid = c.d_wstats != nullptr;
   return 0;
 }
 
@@ -829,8 +833,7 @@ int bt_compute_forces(bt_ctx *h, int intcount, double stats[4]) {
   c.pending_kick = 0.0;
   c.have_acc = true;
   if (intcount && stats) {
-    unsigned long long w[4];
-    BT_CK(d2h(c, w, c.d_wstats, sizeof(w)));
+    const unsigned long long *w = c.h_wstats;   // fetched by read_counters above
     stats[0] = (double)w[0] / c.n;
     stats[1] = (double)w[1];
     stats[2] = (double)w[2] / c.n;
@@ -1019,9 +1022,8 @@ int bt_get_walk_stats(bt_ctx *h, double stats[4], double totals2[2]) {
   if (!h || !stats) return BT_ERR_ARG;
   Ctx &c = h->c;
   if (!c.have_acc) return fail(c, BT_ERR_STATE, "no forces");
-  unsigned long long w[4];
-  BT_CK(hipMemcpyAsync(w, c.d_wstats, sizeof(w), hipMemcpyDeviceToHost, c.stream));
-  BT_CK(hipStreamSynchronize(c.stream));
+  if (!c.h_wstats_valid && read_counters(c)) return BT_ERR_HIP;
+  const unsigned long long *w = c.h_wstats;
   stats[0] = (double)w[0] / c.n;
   stats[1] = (double)w[1];
   stats[2] = (double)w[2] / c.n;

@@ -860,6 +860,7 @@ WalkArgs<R> walk_args(Ctx &c, const int *run_if) {
 }  // namespace
 
 void launch_walk(Ctx &c, const int *run_if) {
+  c.h_wstats_valid = false;   // d_wstats changes with this walk
   hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, c.stream, c.dc, run_if);
   if (c.fp64) {
     // Treecode<double>: one i-particle per lane, LDS j-data, FP64 sums
